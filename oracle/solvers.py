@@ -1,0 +1,171 @@
+"""Solvers of the oracle (test infrastructure only).
+
+* ``cg``                  -- lib/gpt/algorithms/inverter/cg.py:48-112, the same
+  recursion and stopping rule (|r|^2 <= eps^2 |b|^2 on the iterated
+  residual, one history entry per iteration)
+* ``eo2_ne``              -- lib/gpt/qcd/fermion/preconditioner/even_odd_sites.py:61-72
+  over lib/gpt/algorithms/preconditioner/schur_complement_two.py:58-152 and
+  normal_equation.py:35-65 with D = ``parity`` sites (odd by default)
+* ``preconditioned``      -- lib/gpt/algorithms/inverter/preconditioned.py:32-54
+  (dst = L Mpc^-1 R src + S src, initial guess L^-1 dst = DD P_D dst)
+* ``defect_correcting``   -- lib/gpt/algorithms/inverter/defect_correcting.py:74-142
+* ``block_cg``            -- lib/gpt/algorithms/inverter/block_cg.py:36-149
+
+Pinned by: the 16-value correlator of tests/qcd/fermion_operators.py:193-218
+(eo2_ne + CG on the Wilson operator with complex boundary phases); by
+brute force on a 2^4 / 4^4 lattice against numpy.linalg.solve of the dense
+operator; and by the Schur determinant identity
+det M = det DD det CC det Mpc (tests/test_oracle_solvers.py).
+"""
+import numpy as np
+
+from . import conventions as cv
+
+
+def cg(mat, src, psi=None, eps=1e-15, maxiter=1000000, eps_abs=None, miniter=0, history=None):
+    """Returns (psi, iterations, converged)."""
+    psi = np.zeros_like(src) if psi is None else psi.copy()
+    mmp = mat(psi)
+    r = src - mmp
+    p = r.copy()
+    cp = cv.norm2(p)
+    ssq = cv.norm2(src)
+    if ssq == 0.0:
+        return np.zeros_like(src), 0, True
+    rsq = eps**2 * ssq
+    k = -1
+    for k in range(maxiter):
+        c = cp
+        mmp = mat(p)
+        d = cv.inner_product(p, mmp).real
+        a = c / d
+        r = r - a * mmp
+        cp = cv.norm2(r)
+        b = cp / c
+        psi = psi + a * p
+        p = b * p + r
+        res = abs(cp)
+        if history is not None:
+            history.append(res)
+        if k + 1 >= miniter:
+            if eps_abs is not None and res <= eps_abs**2:
+                return psi, k + 1, True
+            if res <= rsq:
+                return psi, k + 1, True
+    return psi, k + 1, False
+
+
+class eo2_ne:
+    """Schur complement two with D = ``parity`` and its normal equation."""
+
+    def __init__(self, op, parity=1):
+        self.op = op
+        self.pD = parity
+        self.pC = 1 - parity
+
+    def _mask(self, x, p):
+        return self.op._mask(x, p)
+
+    def Mpc(self, i_d):
+        op, pD, pC = self.op, self.pD, self.pC
+        t = op.MooeeInv(i_d, pD)
+        t = op.Meooe(t, pD)
+        t = op.MooeeInv(t, pC)
+        t = op.Meooe(t, pC)
+        return i_d - t
+
+    def MpcDag(self, i_d):
+        op, pD, pC = self.op, self.pD, self.pC
+        t = op.MeooeDag(i_d, pD)
+        t = op.MooeeInvDag(t, pC)
+        t = op.MeooeDag(t, pC)
+        t = op.MooeeInvDag(t, pD)
+        return i_d - t
+
+    def Mpc_ne(self, i_d):
+        return self.MpcDag(self.Mpc(i_d))
+
+    def R(self, src):
+        op, pD, pC = self.op, self.pD, self.pC
+        t = op.MooeeInv(self._mask(src, pC), pC)
+        t = op.Meooe(t, pC)
+        return self._mask(src, pD) - t
+
+    def R_ne(self, src):
+        return self.MpcDag(self.R(src))
+
+    def L(self, pc_dst):
+        op, pD, pC = self.op, self.pD, self.pC
+        d = op.MooeeInv(pc_dst, pD)
+        c = -op.MooeeInv(op.Meooe(d, pD), pC)
+        return d + c
+
+    def L_inv(self, dst):
+        return self.op.Mooee(self._mask(dst, self.pD), self.pD)
+
+    def S(self, src):
+        return self.op.MooeeInv(self._mask(src, self.pC), self.pC)
+
+
+def preconditioned(op, src, parity=1, guess=None, **cg_params):
+    """Solve M dst = src through eo2_ne + CG; returns (dst, iterations, converged)."""
+    pc = eo2_ne(op, parity)
+    pc_src = pc.R_ne(src)
+    guess = np.zeros_like(src) if guess is None else guess
+    pc_dst0 = pc.L_inv(guess)
+    pc_dst, it, conv = cg(pc.Mpc_ne, pc_src, pc_dst0, **cg_params)
+    return pc.L(pc_dst) + pc.S(src), it, conv
+
+
+def defect_correcting(outer_mat, inner_inv, src, eps=1e-15, maxiter=1000000, psi=None):
+    """Outer true-residual loop; inner_inv(rhs) solves approximately.
+    Returns (psi, outer iterations, converged)."""
+    psi = np.zeros_like(src) if psi is None else psi.copy()
+    n2src = cv.norm2(src)
+    if n2src == 0.0:
+        n2src = cv.norm2(outer_mat(psi)) or 1.0
+    for i in range(maxiter):
+        s = src - outer_mat(psi)
+        e = (cv.norm2(s) / n2src) ** 0.5
+        if e < eps:
+            return psi, i, True
+        s = s / n2src**0.5
+        psi = psi + inner_inv(s) * n2src**0.5
+    return psi, maxiter, False
+
+
+def block_cg(mat, srcs, eps=1e-15, maxiter=1000000):
+    """Block CG with Cholesky QR (block_cg.py:36-149); returns (X, iterations)."""
+    n = len(srcs)
+    shp = srcs[0].shape
+    B = np.stack([s.reshape(-1) for s in srcs], axis=1)
+    X = np.zeros_like(B)
+    R = B - np.stack([mat(X[:, j].reshape(shp)).reshape(-1) for j in range(n)], axis=1)
+    ssq = np.array([np.vdot(B[:, j], B[:, j]).real for j in range(n)])
+
+    def qr(M):
+        G = M.conj().T @ M
+        C = np.linalg.cholesky(G)  # G = C C^dag
+        Q = M @ np.linalg.inv(C.conj().T)
+        return Q, C.conj().T
+
+    Q, Cr = qr(R)
+    S = np.eye(n, dtype=np.complex128)
+    P = Q.copy()
+    it = 0
+    for it in range(1, maxiter + 1):
+        AP = np.stack([mat(P[:, j].reshape(shp)).reshape(-1) for j in range(n)], axis=1)
+        alpha = np.linalg.solve(P.conj().T @ AP, S.conj().T)
+        X = X + P @ (alpha @ Cr)
+        Qn, Sn = qr(Q - AP @ alpha)
+        # residual norms: columns of Q_{k+1} S_{k+1} ... C
+        S = Sn
+        Cr_new = S @ Cr
+        res = np.linalg.norm(Cr_new, axis=0) ** 2
+        Q = Qn
+        P = Q + P @ S.conj().T
+        Cr = Cr_new
+        S = np.eye(n, dtype=np.complex128)
+        if np.all(res <= eps**2 * ssq):
+            break
+    return [X[:, j].reshape(shp) for j in range(n)], it

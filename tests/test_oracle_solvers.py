@@ -1,0 +1,125 @@
+"""Solvers of the oracle: the reference's correlator KAT
+(tests/qcd/fermion_operators.py:150-218), brute force on tiny lattices, and
+the C restatement against the numpy restatement."""
+import cmath
+
+import numpy as np
+import pytest
+
+from oracle import conventions as cv
+from oracle import cpu as C
+from oracle import gauge as G
+from oracle import rng as R
+from oracle import solvers as S
+from oracle import wilson_clover as W
+
+
+def _dense(fn, shape):
+    n = int(np.prod(shape)) * 12
+    M = np.zeros((n, n), complex)
+    for j in range(n):
+        e = np.zeros(n, complex)
+        e[j] = 1.0
+        M[:, j] = fn(e.reshape(tuple(shape) + (4, 3))).reshape(-1)
+    return M
+
+
+@pytest.fixture(scope="module")
+def tiny_op():
+    g = R.grid([2, 2, 2, 4])
+    rng = R.random("tiny")
+    U = G.random_links(g, rng, scale=0.7)
+    return W.wilson_clover(U, mass=0.2, csw_r=1.0, csw_t=1.2, boundary_phases=[1, 1, 1, -1]), g
+
+
+def test_eo2_ne_solve_matches_dense_inverse(tiny_op):
+    w, g = tiny_op
+    M = _dense(w.M, g.shape)
+    b = R.random("rhs").vspincolor(g)
+    x_exact = np.linalg.solve(M, b.reshape(-1)).reshape(b.shape)
+    x, it, conv = S.preconditioned(w, b, eps=1e-13, maxiter=500)
+    assert conv
+    assert np.linalg.norm(x - x_exact) / np.linalg.norm(x_exact) < 1e-11
+    # the C restatement gives the same solution
+    xc, itc = C.wilson_clover_cpu(w).solve(b, eps=1e-13, maxiter=500)
+    assert np.linalg.norm(xc - x_exact) / np.linalg.norm(x_exact) < 1e-11
+    assert abs(itc - it) <= 1
+
+
+def test_schur_determinant_identity(tiny_op):
+    """det M = det DD det CC det Mpc with D = odd (schur_complement_two.py:24-36)."""
+    w, g = tiny_op
+    pc = S.eo2_ne(w, parity=1)
+    par = cv.parity(g.shape).reshape(-1)
+    sel = {p: np.repeat(par == p, 12) for p in (0, 1)}
+    M = _dense(w.M, g.shape)
+    Mpc = _dense(pc.Mpc, g.shape)[np.ix_(sel[1], sel[1])]
+    DD = M[np.ix_(sel[1], sel[1])]
+    CC = M[np.ix_(sel[0], sel[0])]
+    lhs = np.linalg.slogdet(M)
+    rhs = [np.linalg.slogdet(a) for a in (DD, CC, Mpc)]
+    assert abs(lhs[1] - sum(r[1] for r in rhs)) < 1e-10
+    assert abs(lhs[0] - np.prod([r[0] for r in rhs])) < 1e-10
+
+
+def test_cg_recursion_numpy_vs_c():
+    g = R.grid([4, 4, 4, 8])
+    rng = R.random("cg")
+    U = G.random_links(g, rng, scale=0.5)
+    w = W.wilson_clover(U, mass=0.1, csw_r=1.0, csw_t=1.0, boundary_phases=[1, 1, 1, -1])
+    pc = S.eo2_ne(w, parity=1)
+    b = pc.R_ne(rng.vspincolor(g))
+    hist = []
+    x, it, conv = S.cg(pc.Mpc_ne, b, eps=1e-10, maxiter=1000, history=hist)
+    wc = C.wilson_clover_cpu(w)
+    xc, itc, hc = wc.cg_ne(C.half(b, 1), 1, eps=1e-10, maxiter=1000, history=True)
+    assert conv and itc == it
+    assert np.allclose(hc, hist, rtol=1e-8)
+    assert np.linalg.norm(C.half(x, 1) - xc) / np.linalg.norm(xc) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["aniso", "open", "twisted"])
+def test_c_operator_matches_numpy_per_site(case):
+    g = R.grid([4, 4, 4, 8])
+    rng = R.random("percase")
+    U = G.random_links(g, rng)
+    src = rng.vspincolor(g)
+    params = {
+        "aniso": dict(kappa=0.135, csw_r=1.5, csw_t=1.951, xi_0=1.33111, nu=2.61,
+                      boundary_phases=[1.0, -1.0, 1.0, -1.0]),
+        "open": dict(kappa=0.135, csw_r=1.978, csw_t=1.978, cF=1.3, boundary_phases=[1, 1, 1, 0.0]),
+        "twisted": dict(kappa=0.137, boundary_phases=[cmath.exp(1j * k) for k in (1, 2, 3, 4)]),
+    }[case]
+    w = W.wilson_clover(U, **params)
+    wc = C.wilson_clover_cpu(w)
+    for pin in (0, 1):
+        for dag in (False, True):
+            ref = C.half((w.MeooeDag if dag else w.Meooe)(src, pin), 1 - pin)
+            got = wc.meooe(C.half(src, pin), pin, dag)
+            err = np.abs(got - ref).max(axis=(1, 2)) / np.abs(ref).max(axis=(1, 2)).clip(1e-300)
+            assert err[np.abs(ref).max(axis=(1, 2)) > 0].max() < 1e-13
+        for inv in (False, True):
+            ref = C.half((w.MooeeInv if inv else w.Mooee)(src, pin), pin)
+            got = wc.mooee(C.half(src, pin), pin, inv)
+            assert np.abs(got - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_correlator_known_answer(kats):
+    k = kats["correlator"]
+    g = R.grid(k["fdims"])
+    rng = R.random(k["seed"])
+    U = [u.astype(np.complex64).astype(np.complex128) for u in G.random_links(g, rng)]
+    p = k["params"]
+    w = W.wilson_clover(U, kappa=p["kappa"], csw_r=0.0, csw_t=0.0, xi_0=1.0, nu=1.0,
+                        boundary_phases=[cmath.exp(1j * a) for a in p["boundary_phase_angles"]])
+    wc = C.wilson_clover_cpu(w)
+    x0, y0, z0, t0 = k["point_source_xyzt"]
+    corr = np.zeros(g.shape[0])
+    for s in range(4):
+        for c in range(3):
+            src = np.zeros(g.shape + (4, 3), complex)
+            src[t0, z0, y0, x0, s, c] = 1.0
+            dst, it = wc.solve(src, eps=1e-6, maxiter=1000)
+            assert it > 0
+            corr += (np.abs(dst) ** 2).sum(axis=(1, 2, 3, 4, 5))
+    assert np.linalg.norm(corr - np.array(k["reference"])) < k["tol_norm"]
