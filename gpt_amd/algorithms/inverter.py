@@ -1,0 +1,194 @@
+"""Inverters of gpt_amd (lib/gpt/algorithms/inverter/__init__.py).
+
+* ``cg``                -- lib/gpt/algorithms/inverter/cg.py:24-117.  On the
+  eo2_ne normal operator of a gpt_amd fermion it runs the native fused loop
+  (wc_cg_eo2_ne: hop kernels with fused clover/axpy/dot epilogues, device
+  scalars); on any other matrix_operator the same recursion with the HIP
+  BLAS-1 kernels.  ``history`` holds |r|^2 per iteration as gpt's
+  base_iterative.log_convergence does (lib/gpt/algorithms/base.py:76-106).
+* ``preconditioned``    -- lib/gpt/algorithms/inverter/preconditioned.py:24-54
+* ``defect_correcting`` -- lib/gpt/algorithms/inverter/defect_correcting.py:74-142
+* ``mixed_precision``   -- lib/gpt/algorithms/inverter/mixed_precision.py:23-32
+"""
+import ctypes
+
+import numpy as np
+
+from .. import _lib
+from .. import core
+from ..core import lattice, matrix_operator
+
+
+class cg:
+    def __init__(self, params=None, **kw):
+        p = {"eps": 1e-15, "maxiter": 1000000, "eps_abs": None, "miniter": 0, "prec": None,
+             "fail_if_not_converged": False}
+        p.update(params or {})
+        p.update(kw)
+        self.params = p
+        self.eps = p["eps"]
+        self.eps_abs = p["eps_abs"]
+        self.maxiter = p["maxiter"]
+        self.miniter = p["miniter"]
+        self.fail_if_not_converged = p["fail_if_not_converged"]
+        self.history = []
+        self.converged = None
+
+    def modified(self, **params):
+        return cg({**self.params, **params})
+
+    def _native_ok(self, mat):
+        return (getattr(mat, "eo2_ne", None) is not None and self.eps_abs is None
+                and self.miniter == 0 and self.params["prec"] is None)
+
+    def _finish(self, it, conv, rsq_last, rsq):
+        self.converged = conv
+        if not conv and self.fail_if_not_converged:
+            raise ValueError("FATAL error: CG not converged")
+
+    def __call__(self, mat):
+        if self._native_ok(mat):
+            pc = mat.eo2_ne
+
+            def inv_native(psi, src):
+                hist = np.zeros(max(self.maxiter, 1))
+                st = _lib.wc_solver_stats()
+                _lib.check(_lib.load().wc_cg_eo2_ne(
+                    pc.op.handle, pc.D.tag, psi.ptr(), src.ptr(), float(self.eps), int(self.maxiter),
+                    hist.ctypes.data, ctypes.byref(st), core.stream_ptr()))
+                psi.checkerboard(pc.D)
+                self.history = list(hist[: st.iterations])
+                self._finish(st.iterations, bool(st.converged), st.r2, self.eps**2 * st.b2)
+
+            return matrix_operator(inv_native, inv_mat=mat.mat, make_dst=mat.make_dst,
+                                   accept_guess=(True, False))
+
+        def inv(psi, src):
+            if src is psi:
+                raise ValueError("src must differ from psi")
+            p, mmp, r = lattice(src), lattice(src), lattice(src)
+            mat.mat(mmp, psi)
+            core.axpy(r, -1.0, mmp, src)
+            core.copy(p, r)
+            cp = core.norm2(p)
+            ssq = core.norm2(src)
+            self.history = []
+            if ssq == 0.0:
+                psi[:] = 0
+                self._finish(0, True, 0.0, 0.0)
+                return
+            rsq = self.eps**2 * ssq
+            for k in range(self.maxiter):
+                c = cp
+                mat.mat(mmp, p)
+                d = core.inner_product(p, mmp).real
+                a = c / d
+                cp = core.axpy_norm2(r, -a, mmp, r)
+                b = cp / c
+                core.axpy(psi, a, p, psi)
+                core.axpy(p, b, p, r)
+                res = abs(cp)
+                self.history.append(res)
+                if k + 1 >= self.miniter:
+                    if self.eps_abs is not None and res <= self.eps_abs**2:
+                        self._finish(k + 1, True, res, rsq)
+                        return
+                    if res <= rsq:
+                        self._finish(k + 1, True, res, rsq)
+                        return
+            self._finish(self.maxiter, False, res, rsq)
+
+        return matrix_operator(inv, inv_mat=mat.mat, make_dst=mat.make_dst, accept_guess=(True, False))
+
+
+class preconditioned:
+    """M^-1 = L Mpc^-1 R + S (preconditioned.py:24-54)."""
+
+    def __init__(self, preconditioner, inverter, params=None):
+        self.preconditioner = preconditioner
+        self.inverter = inverter
+
+    def __call__(self, mat):
+        matrix = self.preconditioner(mat)
+        inv_mat = self.inverter(matrix.Mpc)
+        native = (isinstance(self.inverter, cg) and self.inverter._native_ok(matrix.Mpc))
+
+        def inv(dst, src):
+            if native:
+                # one fused call: R_ne, the guess L^-1 dst, CG, L x + S src
+                it = self.inverter
+                hist = np.zeros(max(it.maxiter, 1))
+                st = _lib.wc_solver_stats()
+                _lib.check(_lib.load().wc_solve_eo2_ne(
+                    matrix.op.handle, matrix.D.tag, dst.ptr(), src.ptr(), float(it.eps),
+                    int(it.maxiter), hist.ctypes.data, ctypes.byref(st), core.stream_ptr()))
+                it.history = list(hist[: st.iterations])
+                it._finish(st.iterations, bool(st.converged), st.r2, it.eps**2 * st.b2)
+                return
+            pc_src = matrix.R(src)
+            pc_dst = lattice(pc_src)
+            matrix.L.inv_mat(pc_dst, dst)
+            inv_mat(pc_dst, pc_src)
+            a = matrix.L(pc_dst)
+            b = matrix.S(src)
+            core.axpy(dst, 1.0, a, b)
+
+        return matrix_operator(inv, inv_mat=mat.mat, make_dst=mat.make_dst, accept_guess=(True, False))
+
+
+class defect_correcting:
+    """FP64 outer true-residual loop around an inner inverter (defect_correcting.py:74-142)."""
+
+    def __init__(self, inner_inverter, params=None, **kw):
+        p = {"eps": 1e-15, "maxiter": 1000000}
+        p.update(params or {})
+        p.update(kw)
+        self.eps = p["eps"]
+        self.maxiter = p["maxiter"]
+        self.inner_inverter = inner_inverter
+        self.history = []
+
+    def __call__(self, outer_mat):
+        inner_inv_mat = self.inner_inverter(outer_mat)
+
+        def inv(psi, src):
+            n2src = core.norm2(src)
+            if n2src == 0.0:
+                n2src = core.norm2(outer_mat(psi)) or 1.0
+            self.history = []
+            s = lattice(src)
+            for i in range(self.maxiter):
+                outer_mat.mat(s, psi)
+                core.axpy(s, -1.0, s, src)
+                eps = (core.norm2(s) / n2src) ** 0.5
+                self.history.append(eps)
+                if eps < self.eps:
+                    return
+                core.scale(s, 1.0 / n2src**0.5, s)
+                d = inner_inv_mat(s)
+                core.axpy(psi, n2src**0.5, d, psi)
+
+        return matrix_operator(inv, inv_mat=outer_mat.mat, make_dst=outer_mat.make_dst,
+                               accept_guess=(True, False))
+
+
+class mixed_precision:
+    """Run an inverter on the operator converted to inner_precision, converting
+    fields at the boundary (mixed_precision.py:23-32, matrix_operator.py:139-178)."""
+
+    def __init__(self, inverter, inner_precision, outer_precision):
+        self.inverter = inverter
+        self.inner_precision = inner_precision
+        self.outer_precision = outer_precision
+
+    def __call__(self, mat):
+        inner = mat.converted(self.inner_precision)
+        inner_inv = self.inverter(inner)
+
+        def inv(dst, src):
+            s = core.convert(src, self.inner_precision)
+            d = core.convert(dst, self.inner_precision)
+            inner_inv(d, s)
+            core.convert(dst, d)
+
+        return matrix_operator(inv, inv_mat=mat.mat, make_dst=mat.make_dst, accept_guess=(True, False))

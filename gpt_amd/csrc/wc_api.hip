@@ -1,0 +1,687 @@
+// C ABI of the MI355X even-odd Wilson-clover operator (declared in
+// include/gpt_amd_wc.h).  Host-side orchestration: operator state, opcode
+// dispatch, the Schur-complement compositions and the CG driver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpt_amd_wc.h"
+#include "wc_kernels.h"
+
+
+using wck::HopLaunch;
+
+static thread_local std::string g_err;
+
+#define WC_FAIL(code, ...)                       \
+  do {                                           \
+    char _b[512];                                \
+    snprintf(_b, sizeof _b, __VA_ARGS__);        \
+    g_err = _b;                                  \
+    return code;                                 \
+  } while (0)
+
+#define WC_HIP(x)                                                                                    \
+  do {                                                                                               \
+    hipError_t _e = (x);                                                                             \
+    if (_e != hipSuccess)                                                                            \
+      WC_FAIL(WC_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+
+// launches are asynchronous: surface launch-configuration errors right away
+#define WC_LAUNCHED() WC_HIP(hipGetLastError())
+
+struct wc_operator {
+  wc_params p{};
+  Geom g{};
+  int prec = WC_DOUBLE;
+  int nl = 9;
+  size_t csz = 16;  // bytes per complex value
+  double mass = 0, diag = 0, cs = 0, ct = -0.5;
+  bool has_clover = false, open_bc = false;
+  void *W[2] = {nullptr, nullptr};
+  void *A[2] = {nullptr, nullptr};
+  void *Ai[2] = {nullptr, nullptr};
+  void *Unat = nullptr;  // FP64 links [4][V][9], kept to rebuild the clover on set_mass
+  int *err = nullptr;    // device flag of the clover build
+  std::vector<void *> tmp;  // half-field scratch
+  double *partials = nullptr;
+  int npartials = 0;
+  double *st = nullptr;     // device scalars: CG state [0..7], reductions [8..15]
+  double *host = nullptr;   // pinned host mirror
+};
+
+static size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
+
+static int ensure_tmp(wc_operator *op, size_t n) {
+  while (op->tmp.size() < n) {
+    void *q = nullptr;
+    WC_HIP(hipMalloc(&q, half_bytes(op)));
+    op->tmp.push_back(q);
+  }
+  return WC_OK;
+}
+
+static void free_op(wc_operator *op) {
+  if (!op) return;
+  for (int p = 0; p < 2; ++p) {
+    (void)hipFree(op->W[p]);
+    (void)hipFree(op->A[p]);
+    (void)hipFree(op->Ai[p]);
+  }
+  (void)hipFree(op->Unat);
+  (void)hipFree(op->err);
+  for (void *q : op->tmp) (void)hipFree(q);
+  (void)hipFree(op->partials);
+  (void)hipFree(op->st);
+  if (op->host) (void)hipHostFree(op->host);
+  delete op;
+}
+
+// gamma matrices of gpt's chiral basis (lib/gpt/core/gamma.py:28-42)
+typedef std::complex<double> cdbl;
+static void gamma_mat(int mu, cdbl g[4][4]) {
+  const cdbl I(0, 1);
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) g[i][j] = 0;
+  switch (mu) {
+    case 0: g[0][3] = I; g[1][2] = I; g[2][1] = -I; g[3][0] = -I; break;
+    case 1: g[0][3] = -1; g[1][2] = 1; g[2][1] = 1; g[3][0] = -1; break;
+    case 2: g[0][2] = I; g[1][3] = -I; g[2][0] = -I; g[3][1] = I; break;
+    default: g[0][2] = 1; g[1][3] = 1; g[2][0] = 1; g[3][1] = 1; break;
+  }
+}
+
+static wck::CloverParams clover_params(const wc_operator *op) {
+  wck::CloverParams cp{};
+  const wc_params &p = op->p;
+  int pl = 0;
+  for (int mu = 0; mu < 4; ++mu)
+    for (int nu = mu + 1; nu < 4; ++nu, ++pl) {
+      double c = (nu == 3) ? p.csw_t : p.csw_r / p.xi_0;
+      cp.coef[pl] = -0.5 * c;
+      cdbl gm[4][4], gn[4][4];
+      gamma_mat(mu, gm);
+      gamma_mat(nu, gn);
+      for (int ch = 0; ch < 2; ++ch)
+        for (int a = 0; a < 2; ++a)
+          for (int b = 0; b < 2; ++b) {
+            int i = 2 * ch + a, j = 2 * ch + b;
+            cdbl s = 0;
+            for (int k = 0; k < 4; ++k) s += gm[i][k] * gn[k][j] - gn[i][k] * gm[k][j];
+            s *= 0.5;
+            cp.sig[pl][ch][a * 2 + b][0] = s.real();
+            cp.sig[pl][ch][a * 2 + b][1] = s.imag();
+          }
+    }
+  cp.diag = op->diag;
+  cp.csw_t = p.csw_t;
+  cp.cF = p.cF;
+  cp.open_bc = op->open_bc ? 1 : 0;
+  return cp;
+}
+
+static int build_all(wc_operator *op, hipStream_t s) {
+  const Geom &g = op->g;
+  if (op->prec == WC_DOUBLE)
+    wck::build_links<double>(g, op->Unat, op->nl, op->W[0], op->W[1], s);
+  else
+    wck::build_links<float>(g, op->Unat, op->nl, op->W[0], op->W[1], s);
+  WC_LAUNCHED();
+  if (op->has_clover) {
+    WC_HIP(hipMemsetAsync(op->err, 0, sizeof(int), s));
+    wck::CloverParams cp = clover_params(op);
+    if (op->prec == WC_DOUBLE)
+      wck::build_clover<double>(g, op->Unat, cp, op->A[0], op->A[1], op->Ai[0], op->Ai[1], op->err, s);
+    else
+      wck::build_clover<float>(g, op->Unat, cp, op->A[0], op->A[1], op->Ai[0], op->Ai[1], op->err, s);
+    WC_LAUNCHED();
+    int herr = 0;
+    WC_HIP(hipMemcpyAsync(&herr, op->err, sizeof(int), hipMemcpyDeviceToHost, s));
+    WC_HIP(hipStreamSynchronize(s));
+    if (herr) WC_FAIL(WC_ERR_NUMERIC, "clover block is singular or near-singular at some site (mass %g)", op->mass);
+  }
+  return WC_OK;
+}
+
+static void set_mass_values(wc_operator *op, double mass) {
+  op->mass = mass;
+  op->diag = mass + 1.0 + 3.0 * op->p.nu / op->p.xi_0;  // 1/(2 kappa)
+}
+
+extern "C" {
+
+const char *wc_last_error(void) { return g_err.c_str(); }
+
+int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out) {
+  if (!p || !U || !out) WC_FAIL(WC_ERR_ARG, "wc_create: null argument");
+  *out = nullptr;
+  for (int mu = 0; mu < 4; ++mu)
+    if (p->fdims[mu] < 2 || (p->fdims[mu] & 1))
+      WC_FAIL(WC_ERR_ARG, "wc_create: fdims[%d] = %d must be even and >= 2", mu, p->fdims[mu]);
+  if (p->precision != WC_DOUBLE && p->precision != WC_SINGLE)
+    WC_FAIL(WC_ERR_ARG, "wc_create: unknown precision %d", p->precision);
+  if (!(p->xi_0 > 0.0)) WC_FAIL(WC_ERR_ARG, "wc_create: xi_0 must be positive");
+  const bool open_bc = (p->boundary_phases[3][0] == 0.0 && p->boundary_phases[3][1] == 0.0);
+  if (!open_bc && p->cF != 1.0) WC_FAIL(WC_ERR_ARG, "wc_create: cF != 1 needs an open temporal boundary");
+  if (open_bc && (p->xi_0 != 1.0 || p->nu != 1.0 || p->csw_r != p->csw_t))
+    WC_FAIL(WC_ERR_ARG, "wc_create: open boundary needs the isotropic case with csw_r == csw_t");
+  hipStream_t s = (hipStream_t)stream;
+  wc_operator *op = new wc_operator();
+  op->p = *p;
+  op->prec = p->precision;
+  op->csz = p->precision == WC_DOUBLE ? 16 : 8;
+  op->nl = p->compress_links ? 6 : 9;
+  op->open_bc = open_bc;
+  op->has_clover = (p->csw_r != 0.0 || p->csw_t != 0.0);
+  for (int mu = 0; mu < 4; ++mu) op->g.L[mu] = p->fdims[mu];
+  op->g.Lxh = p->fdims[0] / 2;
+  op->g.V = (long)p->fdims[0] * p->fdims[1] * p->fdims[2] * p->fdims[3];
+  op->g.Vh = op->g.V / 2;
+  op->cs = -0.5 * p->nu / p->xi_0;
+  op->ct = -0.5;
+  set_mass_values(op, p->mass);
+  int rc = WC_OK;
+  auto chk = [&](hipError_t e, const char *what) {
+    if (e != hipSuccess && rc == WC_OK) {
+      char b[256];
+      snprintf(b, sizeof b, "wc_create: %s failed: %s", what, hipGetErrorString(e));
+      g_err = b;
+      rc = WC_ERR_HIP;
+    }
+  };
+  const size_t Vh = (size_t)op->g.Vh;
+  for (int q = 0; q < 2; ++q) {
+    chk(hipMalloc(&op->W[q], Vh * 8 * op->nl * op->csz), "hipMalloc(links)");
+    if (op->has_clover) {
+      chk(hipMalloc(&op->A[q], Vh * 36 * op->csz), "hipMalloc(clover)");
+      chk(hipMalloc(&op->Ai[q], Vh * 36 * op->csz), "hipMalloc(clover inverse)");
+    }
+  }
+  chk(hipMalloc(&op->Unat, (size_t)op->g.V * 4 * 9 * 16), "hipMalloc(gauge copy)");
+  chk(hipMalloc((void **)&op->err, sizeof(int)), "hipMalloc(flag)");
+  op->npartials = std::max(wck::kRedBlocks, wck::site_blocks(op->g.Vh));
+  chk(hipMalloc((void **)&op->partials, sizeof(double) * op->npartials), "hipMalloc(partials)");
+  chk(hipMalloc((void **)&op->st, sizeof(double) * 16), "hipMalloc(scalars)");
+  chk(hipHostMalloc((void **)&op->host, sizeof(double) * 16, hipHostMallocDefault), "hipHostMalloc");
+  if (rc != WC_OK) {
+    free_op(op);
+    return rc;
+  }
+  if (hipMemcpyAsync(op->Unat, U, (size_t)op->g.V * 4 * 9 * 16, hipMemcpyDefault, s) != hipSuccess) {
+    free_op(op);
+    WC_FAIL(WC_ERR_HIP, "wc_create: copying the gauge field failed");
+  }
+  rc = build_all(op, s);
+  if (rc != WC_OK) {
+    std::string keep = g_err;
+    free_op(op);
+    g_err = keep;
+    return rc;
+  }
+  *out = op;
+  return WC_OK;
+}
+
+int wc_update(wc_operator *op, const void *U, void *stream) {
+  if (!op || !U) WC_FAIL(WC_ERR_ARG, "wc_update: null argument");
+  hipStream_t s = (hipStream_t)stream;
+  WC_HIP(hipMemcpyAsync(op->Unat, U, (size_t)op->g.V * 4 * 9 * 16, hipMemcpyDefault, s));
+  return build_all(op, s);
+}
+
+int wc_set_mass(wc_operator *op, double mass, void *stream) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_set_mass: null operator");
+  set_mass_values(op, mass);
+  if (!op->has_clover) return WC_OK;
+  return build_all(op, (hipStream_t)stream);
+}
+
+int wc_destroy(wc_operator *op) {
+  free_op(op);
+  return WC_OK;
+}
+
+long wc_half_volume(const wc_operator *op) { return op ? op->g.Vh : 0; }
+int wc_precision(const wc_operator *op) { return op ? op->prec : -1; }
+
+}  // extern "C"
+
+// ------------------------------------------------------------ compositions
+template <typename R>
+static HopLaunch hop_args(const wc_operator *op, int pout, int dag, int epi, const void *in, void *out) {
+  HopLaunch a{};
+  a.g = op->g;
+  a.pout = pout;
+  a.dag = dag;
+  a.nl = op->nl;
+  a.epi = epi;
+  a.in = in;
+  a.out = out;
+  a.W = op->W[pout];
+  a.cs = op->cs;
+  a.ct = op->ct;
+  for (int mu = 0; mu < 4; ++mu) {
+    a.ph[mu][0] = op->p.boundary_phases[mu][0];
+    a.ph[mu][1] = op->p.boundary_phases[mu][1];
+  }
+  a.open_bc = op->open_bc ? 1 : 0;
+  a.diag = op->diag;
+  a.clov_scalar = 1.0 / op->diag;
+  return a;
+}
+
+// out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
+template <typename R>
+static void site_op(const wc_operator *op, int p, int inv, const void *in, void *out, const void *y,
+                    hipStream_t s) {
+  const void *B = op->has_clover ? (inv ? op->Ai[p] : op->A[p]) : nullptr;
+  double sc = inv ? 1.0 / op->diag : op->diag;
+  wck::site_clover<R>(op->g, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s);
+}
+
+// Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd; dotv (optional)
+// makes the last hop write per-block partials of Re<dotv, out>
+template <typename R>
+static int mpc_impl(wc_operator *op, int pd, int which, void *out, const void *in, const void *dotv,
+                    hipStream_t s) {
+  const int pc = 1 - pd;
+  int rc = ensure_tmp(op, 3);
+  if (rc) return rc;
+  void *tD = op->tmp[0], *tC = op->tmp[1], *tY = op->tmp[2];
+  const void *clovC = op->has_clover ? op->Ai[pc] : nullptr;
+  const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
+  const void *src = in;
+  void *dst_mpc = (which == 2) ? tY : out;
+  if (which == 0 || which == 2) {
+    site_op<R>(op, pd, 1, src, tD, nullptr, s);
+    HopLaunch a = hop_args<R>(op, pc, 0, wck::EPI_CLOV, tD, tC);
+    a.clov = clovC;
+    wck::hop<R>(a, s);
+    HopLaunch b = hop_args<R>(op, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC, dst_mpc);
+    b.y = src;
+    b.dotv = dotv;
+    b.partial = op->partials;
+    wck::hop<R>(b, s);
+    src = dst_mpc;
+  }
+  if (which == 1 || which == 2) {
+    HopLaunch a = hop_args<R>(op, pc, 1, wck::EPI_CLOV, src, tC);
+    a.clov = clovC;
+    wck::hop<R>(a, s);
+    HopLaunch b = hop_args<R>(op, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC, out);
+    b.clov = clovD;
+    b.y = src;
+    b.dotv = dotv;
+    b.partial = op->partials;
+    wck::hop<R>(b, s);
+  }
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+template <typename R>
+static int apply_impl(wc_operator *op, int opcode, void *dst, const void *src, int cb, hipStream_t s) {
+  const size_t hb = half_bytes(op);
+  const long Vh = op->g.Vh;
+  auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
+  switch (opcode) {
+    case 2001:
+    case 2002:
+    case 3001:
+    case 4001: {  // M, Mdag, Dhop, DhopDag on full fields
+      const int dag = (opcode == 2002 || opcode == 4001);
+      const bool withdiag = (opcode == 2001 || opcode == 2002);
+      for (int p = 0; p < 2; ++p) {
+        HopLaunch a = hop_args<R>(op, p, dag, withdiag ? wck::EPI_DIAG : 0, half(src, 1 - p), half(dst, p));
+        a.diag_clov = op->has_clover ? op->A[p] : nullptr;
+        a.y = half(src, p);
+        wck::hop<R>(a, s);
+      }
+      break;
+    }
+    case 2009:  // Mdiag
+      for (int p = 0; p < 2; ++p) site_op<R>(op, p, 0, half(src, p), half(dst, p), nullptr, s);
+      break;
+    case 2010:
+    case 2011:
+    case 2012:
+    case 2013:
+    case 2014:
+    case 2015:  // Dminus(Dag), Import/Export: identity for Wilson-clover
+      WC_HIP(hipMemcpyAsync(dst, src, 2 * hb, hipMemcpyDeviceToDevice, s));
+      break;
+    case 2003:
+    case 2004:
+    case 3002:
+    case 4002: {
+      if (cb != 0 && cb != 1) WC_FAIL(WC_ERR_ARG, "wc_apply: opcode %d needs src_cb 0 or 1", opcode);
+      if ((opcode == 3002 || opcode == 4002) && cb != 1)
+        WC_FAIL(WC_ERR_ARG, "wc_apply: DhopEO maps odd to even (src_cb must be 1)");
+      const int dag = (opcode == 2004 || opcode == 4002);
+      HopLaunch a = hop_args<R>(op, 1 - cb, dag, 0, src, dst);
+      wck::hop<R>(a, s);
+      break;
+    }
+    case 2005:
+    case 2006:
+    case 2007:
+    case 2008:
+      if (cb != 0 && cb != 1) WC_FAIL(WC_ERR_ARG, "wc_apply: opcode %d needs src_cb 0 or 1", opcode);
+      site_op<R>(op, cb, opcode >= 2007, src, dst, nullptr, s);
+      break;
+    default:
+      WC_FAIL(WC_ERR_ARG, "Unknown opcode %d", opcode);
+  }
+  (void)Vh;
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+// host read of a device double (synchronising the stream)
+static int read_scalar(wc_operator *op, const double *dev, double *h, hipStream_t s) {
+  WC_HIP(hipMemcpyAsync(op->host, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  *h = op->host[0];
+  return WC_OK;
+}
+
+template <typename R>
+static int norm2_dev(wc_operator *op, long n, const void *x, double *dev_out, hipStream_t s) {
+  wck::norm2_partials<R>(n, x, op->partials, s);
+  wck::reduce_final(op->partials, wck::kRedBlocks, dev_out, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+// CG of lib/gpt/algorithms/inverter/cg.py:48-112 on Mpc^dag Mpc
+template <typename R>
+static int cg_impl(wc_operator *op, int pd, void *x, const void *b, double eps, long maxiter, double *history,
+                   wc_solver_stats *stats, hipStream_t s) {
+  int rc = ensure_tmp(op, 6);
+  if (rc) return rc;
+  void *r = op->tmp[3], *p = op->tmp[4], *mmp = op->tmp[5];
+  const long n = op->g.Vh * 12;
+  const int hop_blocks = wck::site_blocks(op->g.Vh);
+  double *st = op->st;
+  // mmp = A x ; r = b - mmp ; p = r ; cp = |p|^2 ; ssq = |b|^2
+  if ((rc = mpc_impl<R>(op, pd, 2, mmp, x, nullptr, s))) return rc;
+  wck::axpy<R>(n, -1.0, 0.0, mmp, b, r, s);
+  WC_HIP(hipMemcpyAsync(p, r, half_bytes(op), hipMemcpyDeviceToDevice, s));
+  if ((rc = norm2_dev<R>(op, n, p, st + 0, s))) return rc;  // st[0] = c = cp
+  if ((rc = norm2_dev<R>(op, n, b, st + 8, s))) return rc;
+  double ssq = 0, cp = 0;
+  if ((rc = read_scalar(op, st + 8, &ssq, s))) return rc;
+  if ((rc = read_scalar(op, st + 0, &cp, s))) return rc;
+  if (stats) {
+    stats->iterations = 0;
+    stats->converged = 0;
+    stats->r2 = cp;
+    stats->b2 = ssq;
+  }
+  if (ssq == 0.0) {
+    WC_HIP(hipMemsetAsync(x, 0, half_bytes(op), s));
+    if (stats) stats->converged = 1;
+    return WC_OK;
+  }
+  const double rsq = eps * eps * ssq;
+  long k = 0;
+  for (; k < maxiter; ++k) {
+    // mmp = A p with the partials of Re<p, mmp>
+    if ((rc = mpc_impl<R>(op, pd, 2, mmp, p, p, s))) return rc;
+    wck::cg_alpha(op->partials, hop_blocks, st, s);
+    wck::cg_update1<R>(n, st, r, mmp, x, p, op->partials, s);
+    wck::cg_beta(op->partials, wck::kRedBlocks, st, s);
+    wck::cg_update2<R>(n, st, r, p, s);
+    WC_LAUNCHED();
+    if ((rc = read_scalar(op, st + 4, &cp, s))) return rc;
+    if (history) history[k] = cp;
+    if (!(std::isfinite(cp))) {
+      if (stats) { stats->iterations = k + 1; stats->r2 = cp; }
+      WC_FAIL(WC_ERR_NUMERIC, "CG: residual became %g at iteration %ld", cp, k + 1);
+    }
+    if (std::fabs(cp) <= rsq) {
+      ++k;
+      if (stats) {
+        stats->iterations = k;
+        stats->converged = 1;
+        stats->r2 = cp;
+      }
+      return WC_OK;
+    }
+  }
+  if (stats) {
+    stats->iterations = k;
+    stats->converged = 0;
+    stats->r2 = cp;
+  }
+  return WC_OK;
+}
+
+template <typename R>
+static int solve_impl(wc_operator *op, int pd, void *dst, const void *src, double eps, long maxiter,
+                      double *history, wc_solver_stats *stats, hipStream_t s) {
+  const int pc = 1 - pd;
+  int rc = ensure_tmp(op, 9);
+  if (rc) return rc;
+  const size_t hb = half_bytes(op);
+  auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
+  void *tR = op->tmp[6], *pb = op->tmp[7], *px = op->tmp[8];
+  void *tC = op->tmp[1];
+  // R src = src_D - Meooe A_C^-1 src_C
+  site_op<R>(op, pc, 1, half(src, pc), tC, nullptr, s);
+  {
+    HopLaunch a = hop_args<R>(op, pd, 0, wck::EPI_AXPY, tC, tR);
+    a.y = half(src, pd);
+    wck::hop<R>(a, s);
+  }
+  // R_ne src = Mpc^dag R src
+  if ((rc = mpc_impl<R>(op, pd, 1, pb, tR, nullptr, s))) return rc;
+  // guess L^-1 dst = A_D dst_D
+  site_op<R>(op, pd, 0, half(dst, pd), px, nullptr, s);
+  WC_LAUNCHED();
+  if ((rc = cg_impl<R>(op, pd, px, pb, eps, maxiter, history, stats, s))) return rc;
+  // dst = L x + S src: dst_D = A_D^-1 x ; dst_C = A_C^-1 src_C - A_C^-1 Meooe dst_D
+  site_op<R>(op, pd, 1, px, half(dst, pd), nullptr, s);
+  site_op<R>(op, pc, 1, half(src, pc), tR, nullptr, s);
+  {
+    HopLaunch a = hop_args<R>(op, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
+    a.clov = op->has_clover ? op->Ai[pc] : nullptr;
+    a.y = tR;
+    wck::hop<R>(a, s);
+  }
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+extern "C" {
+
+int wc_apply(wc_operator *op, int opcode, void *dst, const void *src, int src_cb, void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply: dst must not alias src");
+  hipStream_t s = (hipStream_t)stream;
+  return op->prec == WC_DOUBLE ? apply_impl<double>(op, opcode, dst, src, src_cb, s)
+                               : apply_impl<float>(op, opcode, dst, src, src_cb, s);
+}
+
+int wc_apply_mpc(wc_operator *op, int parity, int which, void *dst, const void *src, void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: dst must not alias src");
+  if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: parity must be 0 or 1");
+  if (which < 0 || which > 2) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: which must be 0, 1 or 2");
+  hipStream_t s = (hipStream_t)stream;
+  return op->prec == WC_DOUBLE ? mpc_impl<double>(op, parity, which, dst, src, nullptr, s)
+                               : mpc_impl<float>(op, parity, which, dst, src, nullptr, s);
+}
+
+int wc_cg_eo2_ne(wc_operator *op, int parity, void *x, const void *b, double eps, long maxiter,
+                 double *history, wc_solver_stats *stats, void *stream) {
+  if (!op || !x || !b) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: null argument");
+  if (x == b) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: x must not alias b");
+  if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: parity must be 0 or 1");
+  if (maxiter < 0) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: maxiter must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  return op->prec == WC_DOUBLE ? cg_impl<double>(op, parity, x, b, eps, maxiter, history, stats, s)
+                               : cg_impl<float>(op, parity, x, b, eps, maxiter, history, stats, s);
+}
+
+int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,
+                    double *history, wc_solver_stats *stats, void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: dst must not alias src");
+  if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: parity must be 0 or 1");
+  if (maxiter < 0) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: maxiter must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  return op->prec == WC_DOUBLE ? solve_impl<double>(op, parity, dst, src, eps, maxiter, history, stats, s)
+                               : solve_impl<float>(op, parity, dst, src, eps, maxiter, history, stats, s);
+}
+
+// ------------------------------------------------------------------ BLAS
+static thread_local double *g_red = nullptr;      // device partials + result
+static thread_local double *g_red_host = nullptr;  // pinned host copy
+
+static int red_buffers() {
+  if (!g_red) {
+    WC_HIP(hipMalloc((void **)&g_red, sizeof(double) * (2 * wck::kRedBlocks + 4)));
+    WC_HIP(hipHostMalloc((void **)&g_red_host, sizeof(double) * 4, hipHostMallocDefault));
+  }
+  return WC_OK;
+}
+
+#define WC_PREC_CHECK(prec)                                                   \
+  if ((prec) != WC_DOUBLE && (prec) != WC_SINGLE) WC_FAIL(WC_ERR_ARG, "unknown precision %d", (int)(prec))
+
+int wc_axpy(int prec, long n, double a_re, double a_im, const void *x, const void *y, void *z, void *stream) {
+  WC_PREC_CHECK(prec);
+  if (n < 0 || (n > 0 && (!x || !y || !z))) WC_FAIL(WC_ERR_ARG, "wc_axpy: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == WC_DOUBLE) wck::axpy<double>(n, a_re, a_im, x, y, z, s);
+  else wck::axpy<float>(n, a_re, a_im, x, y, z, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+int wc_scale(int prec, long n, double a_re, double a_im, const void *x, void *z, void *stream) {
+  WC_PREC_CHECK(prec);
+  if (n < 0 || (n > 0 && (!x || !z))) WC_FAIL(WC_ERR_ARG, "wc_scale: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == WC_DOUBLE) wck::scale<double>(n, a_re, a_im, x, z, s);
+  else wck::scale<float>(n, a_re, a_im, x, z, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+int wc_inner_product(int prec, long n, const void *a, const void *b, double *result, void *stream) {
+  WC_PREC_CHECK(prec);
+  if (n < 0 || !result || (n > 0 && (!a || !b))) WC_FAIL(WC_ERR_ARG, "wc_inner_product: bad arguments");
+  int rc = red_buffers();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  double *pre = g_red, *pim = g_red + wck::kRedBlocks, *res = g_red + 2 * wck::kRedBlocks;
+  if (prec == WC_DOUBLE) wck::dot_partials<double>(n, a, b, pre, pim, s);
+  else wck::dot_partials<float>(n, a, b, pre, pim, s);
+  wck::reduce_final(pre, wck::kRedBlocks, res, s);
+  wck::reduce_final(pim, wck::kRedBlocks, res + 1, s);
+  WC_LAUNCHED();
+  WC_HIP(hipMemcpyAsync(g_red_host, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  result[0] = g_red_host[0];
+  result[1] = g_red_host[1];
+  return WC_OK;
+}
+
+int wc_norm2(int prec, long n, const void *x, double *result, void *stream) {
+  WC_PREC_CHECK(prec);
+  if (n < 0 || !result || (n > 0 && !x)) WC_FAIL(WC_ERR_ARG, "wc_norm2: bad arguments");
+  int rc = red_buffers();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  double *pre = g_red, *res = g_red + 2 * wck::kRedBlocks;
+  if (prec == WC_DOUBLE) wck::norm2_partials<double>(n, x, pre, s);
+  else wck::norm2_partials<float>(n, x, pre, s);
+  wck::reduce_final(pre, wck::kRedBlocks, res, s);
+  WC_LAUNCHED();
+  WC_HIP(hipMemcpyAsync(g_red_host, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  *result = g_red_host[0];
+  return WC_OK;
+}
+
+int wc_axpy_norm2(int prec, long n, double a_re, double a_im, const void *x, const void *y, void *z,
+                  double *result, void *stream) {
+  WC_PREC_CHECK(prec);
+  if (n < 0 || !result || (n > 0 && (!x || !y || !z))) WC_FAIL(WC_ERR_ARG, "wc_axpy_norm2: bad arguments");
+  int rc = red_buffers();
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  double *pre = g_red, *res = g_red + 2 * wck::kRedBlocks;
+  if (prec == WC_DOUBLE) wck::axpy_norm2_partials<double>(n, a_re, a_im, x, y, z, pre, s);
+  else wck::axpy_norm2_partials<float>(n, a_re, a_im, x, y, z, pre, s);
+  wck::reduce_final(pre, wck::kRedBlocks, res, s);
+  WC_LAUNCHED();
+  WC_HIP(hipMemcpyAsync(g_red_host, res, sizeof(double), hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  *result = g_red_host[0];
+  return WC_OK;
+}
+
+int wc_convert(int dst_prec, void *dst, int src_prec, const void *src, long n, void *stream) {
+  WC_PREC_CHECK(dst_prec);
+  WC_PREC_CHECK(src_prec);
+  if (n < 0 || (n > 0 && (!dst || !src))) WC_FAIL(WC_ERR_ARG, "wc_convert: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  if (dst_prec == src_prec) {
+    if (dst != src) WC_HIP(hipMemcpyAsync(dst, src, (size_t)n * (dst_prec == WC_DOUBLE ? 16 : 8),
+                                          hipMemcpyDeviceToDevice, s));
+    return WC_OK;
+  }
+  if (dst_prec == WC_SINGLE) wck::convert_d2f(n, src, dst, s);
+  else wck::convert_f2d(n, src, dst, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+static int geom_of(const int fdims[4], Geom *g) {
+  if (!fdims) WC_FAIL(WC_ERR_ARG, "null fdims");
+  for (int mu = 0; mu < 4; ++mu) {
+    if (fdims[mu] < 2 || (fdims[mu] & 1)) WC_FAIL(WC_ERR_ARG, "fdims[%d] = %d must be even and >= 2", mu, fdims[mu]);
+    g->L[mu] = fdims[mu];
+  }
+  g->Lxh = fdims[0] / 2;
+  g->V = (long)fdims[0] * fdims[1] * fdims[2] * fdims[3];
+  g->Vh = g->V / 2;
+  return WC_OK;
+}
+
+int wc_field_import(const int fdims[4], int prec, const void *natural, void *eo, void *stream) {
+  WC_PREC_CHECK(prec);
+  Geom g;
+  int rc = geom_of(fdims, &g);
+  if (rc) return rc;
+  if (!natural || !eo || natural == eo) WC_FAIL(WC_ERR_ARG, "wc_field_import: bad pointers");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == WC_DOUBLE) wck::import_field<double>(g, natural, eo, s);
+  else wck::import_field<float>(g, natural, eo, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+int wc_field_export(const int fdims[4], int prec, const void *eo, void *natural, void *stream) {
+  WC_PREC_CHECK(prec);
+  Geom g;
+  int rc = geom_of(fdims, &g);
+  if (rc) return rc;
+  if (!natural || !eo || natural == eo) WC_FAIL(WC_ERR_ARG, "wc_field_export: bad pointers");
+  hipStream_t s = (hipStream_t)stream;
+  if (prec == WC_DOUBLE) wck::export_field<double>(g, eo, natural, s);
+  else wck::export_field<float>(g, eo, natural, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+}  // extern "C"

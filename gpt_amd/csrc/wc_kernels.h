@@ -1,0 +1,102 @@
+// Internal launcher interface between the C-ABI (wc_api.hip) and the kernels
+// (wc_kernels.hip).  Not exported.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "wc_common.h"
+
+namespace wck {
+
+// hop epilogue flags
+enum : int {
+  EPI_CLOV = 1,  // res = A res       (A = packed blocks of the output parity)
+  EPI_DIAG = 2,  // res = res + D y   (D = packed blocks or the scalar diag)
+  EPI_AXPY = 4,  // res = y - res
+  EPI_DOT = 8,   // partial[block] = Re <dotv, res>
+};
+
+struct HopLaunch {
+  Geom g;
+  int pout;          // output parity
+  int dag;           // adjoint hop (gamma_mu -> -gamma_mu)
+  int nl;            // 9 or 6 complex per link
+  int epi;           // EPI_* flags
+  const void *in;    // half field of parity 1 - pout
+  void *out;         // half field of parity pout
+  const void *W;     // links of output parity [8][nl][Vh]
+  const void *clov;  // EPI_CLOV blocks [36][Vh] (parity pout), or null
+  double clov_scalar;  // EPI_CLOV factor when clov is null (no clover term)
+  const void *diag_clov;  // EPI_DIAG blocks, or null for the scalar diag
+  double diag;       // scalar diagonal when diag_clov is null
+  const void *y;     // EPI_DIAG / EPI_AXPY operand (parity pout)
+  const void *dotv;  // EPI_DOT left vector (parity pout)
+  double *partial;   // EPI_DOT per-block partial sums
+  double cs;         // spatial hop coefficient -c_s/2
+  double ct;         // temporal hop coefficient -1/2
+  double ph[4][2];   // boundary phases
+  int open_bc;
+};
+
+// number of blocks a hop / site launch uses for Vh sites
+int site_blocks(long Vh);
+constexpr int kSiteThreads = 256;
+constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
+
+template <typename R> void hop(const HopLaunch &a, hipStream_t s);
+
+// out = A in (blocks != null) or diag * in on parity p; zeroes the open
+// boundary; optional axpy (out = y - A in)
+template <typename R>
+void site_clover(const Geom &g, int p, const void *blocks, double diag, const void *in, void *out,
+                 const void *y, int open_bc, hipStream_t s);
+
+// BLAS-1 on n complex entries (C2 of R)
+template <typename R>
+void axpy(long n, double ar, double ai, const void *x, const void *y, void *z, hipStream_t s);
+template <typename R> void scale(long n, double ar, double ai, const void *x, void *z, hipStream_t s);
+// partial sums (kRedBlocks entries) of Re/Im <a,b> (2 arrays) or |a|^2 (re only)
+template <typename R>
+void dot_partials(long n, const void *a, const void *b, double *pre, double *pim, hipStream_t s);
+template <typename R> void norm2_partials(long n, const void *a, double *pre, hipStream_t s);
+template <typename R>
+void axpy_norm2_partials(long n, double ar, double ai, const void *x, const void *y, void *z,
+                         double *pre, hipStream_t s);
+// out[0] = sum of partials (fixed order, one block)
+void reduce_final(const double *partials, int n, double *out, hipStream_t s);
+
+// precision conversion of n complex entries
+void convert_d2f(long n, const void *src, void *dst, hipStream_t s);
+void convert_f2d(long n, const void *src, void *dst, hipStream_t s);
+
+// natural [V][12] <-> even-odd [2][12][Vh] (both in R)
+template <typename R> void import_field(const Geom &g, const void *nat, void *eo, hipStream_t s);
+template <typename R> void export_field(const Geom &g, const void *eo, void *nat, hipStream_t s);
+
+// operator set-up from natural FP64 links Unat[4][V][9]
+template <typename R>
+void build_links(const Geom &g, const void *Unat, int nl, void *W0, void *W1, hipStream_t s);
+
+struct CloverParams {
+  double coef[6];          // -csw_{mu nu}/2 for the planes (01,02,03,12,13,23)
+  double sig[6][2][4][2];  // chiral 2x2 blocks of sigma_{mu nu}: [plane][chir][a*2+b][re/im]
+  double diag;             // 1/(2 kappa)
+  double csw_t;
+  double cF;
+  int open_bc;
+};
+// writes packed A and A^-1 per parity; err_flag (device int) gets 1 on a
+// near-singular block
+template <typename R>
+void build_clover(const Geom &g, const void *Unat, const CloverParams &cp, void *A0, void *A1,
+                  void *Ai0, void *Ai1, int *err_flag, hipStream_t s);
+
+// CG helpers (device scalars): st = {c, d, alpha, beta, cp}
+void cg_alpha(const double *partials, int n, double *st, hipStream_t s);
+void cg_beta(const double *partials, int n, double *st, hipStream_t s);
+template <typename R>
+void cg_update1(long n, const double *st, void *r, const void *mmp, void *x, const void *p,
+                double *partials, hipStream_t s);
+template <typename R>
+void cg_update2(long n, const double *st, const void *r, void *p, hipStream_t s);
+
+}  // namespace wck

@@ -1,0 +1,279 @@
+"""The Wilson-clover fermion operator object of gpt_amd.
+
+Mirrors gpt's operator stack for this path:
+  lib/gpt/qcd/fermion/wilson.py:63-96           wilson_clover(U, params) factory
+  lib/gpt/qcd/fermion/operator/base.py:28-306   matrix_operator attributes,
+                                                converted / updated / adj /
+                                                propagator / even_odd_sites_decomposed
+  lib/gpt/qcd/fermion/operator/interface.py:23-103  handle set-up, the operator
+                                                pool (suspend / reuse via
+                                                set_mass then update)
+  lib/gpt/qcd/fermion/register.py:2-34          opcode numbers
+The arithmetic happens in the HIP library behind include/gpt_amd_wc.h.
+"""
+import copy as _copy
+import ctypes
+
+from ... import _lib
+from ... import core
+from ...core import even, odd, none, full, redblack, lattice, matrix_operator
+from .. import gauge
+
+OPCODES = {
+    "M": 2001, "Mdag": 2002, "Meooe": 2003, "MeooeDag": 2004, "Mooee": 2005, "MooeeDag": 2006,
+    "MooeeInv": 2007, "MooeeInvDag": 2008, "Mdiag": 2009, "Dminus": 2010, "DminusDag": 2011,
+    "ImportPhysicalFermionSource": 2012, "ImportUnphysicalFermion": 2013,
+    "ExportPhysicalFermionSolution": 2014, "ExportPhysicalFermionSource": 2015,
+    "Dhop": 3001, "DhopDag": 4001, "DhopEO": 3002, "DhopEODag": 4002,
+}
+FULL_FIELD = {2001, 2002, 2009, 2010, 2011, 2012, 2013, 2014, 2015, 3001, 4001}
+
+# pool of suspended operator handles by tag (interface.py:23-24)
+operator_tag = {}
+operator_limbo = {}
+
+
+def _phases(bp):
+    out = []
+    for b in bp:
+        c = complex(b)
+        out.append((c.real, c.imag))
+    return out
+
+
+class interface:
+    """Owns one wc_operator handle (lib/gpt/qcd/fermion/operator/interface.py:29-103)."""
+
+    def __init__(self):
+        self.obj = None
+
+    def _setup(self, grid, params, U):
+        assert self.obj is None
+        lib = _lib.load()
+        tag_params = {k: params[k] for k in params if k not in ["mass"]}
+        tag = f"wilson_clover_{grid.precision.__name__}_{tuple(grid.fdimensions)}_{tag_params}"
+        Ut = gauge.stacked(U)
+        stream = core.stream_ptr()
+        if tag in operator_limbo and operator_limbo[tag]:
+            self.obj = operator_limbo[tag].pop()
+            # set mass needs to precede update for clover-type fermions (interface.py:43-45)
+            _lib.check(lib.wc_set_mass(self.obj, float(params["mass"]), stream))
+            _lib.check(lib.wc_update(self.obj, Ut.data_ptr(), stream))
+        else:
+            p = _lib.wc_params()
+            for i in range(4):
+                p.fdims[i] = grid.fdimensions[i]
+            p.mass = float(params["mass"])
+            p.csw_r = float(params["csw_r"])
+            p.csw_t = float(params["csw_t"])
+            p.cF = float(params.get("cF", 1.0))
+            p.xi_0 = float(params["xi_0"])
+            p.nu = float(params["nu"])
+            p.isAnisotropic = int(bool(params.get("isAnisotropic", False)))
+            for mu, (re, im) in enumerate(_phases(params["boundary_phases"])):
+                p.boundary_phases[mu][0] = re
+                p.boundary_phases[mu][1] = im
+            p.precision = grid.precision.code
+            p.compress_links = int(bool(params.get("compress_links", False)))
+            h = ctypes.c_void_p()
+            _lib.check(lib.wc_create(ctypes.byref(p), Ut.data_ptr(), stream, ctypes.byref(h)))
+            self.obj = h.value
+            operator_tag[self.obj] = tag
+        core.torch.cuda.current_stream().synchronize()
+
+    def setup(self, grid, params, U):
+        self.setup_arguments = (grid, params, U)
+        self._setup(grid, params, U)
+
+    def __del__(self):
+        self.suspend()
+
+    def suspend(self):
+        if self.obj is not None:
+            operator_limbo.setdefault(operator_tag[self.obj], []).append(self.obj)
+            self.obj = None
+
+    def update(self, params, U):
+        if self.obj is None:
+            self._setup(*self.setup_arguments)
+        _lib.check(_lib.load().wc_update(self.obj, gauge.stacked(U).data_ptr(), core.stream_ptr()))
+
+    def apply_unary_operator(self, opcode, dst, src):
+        assert self.obj is not None
+        if dst is src:
+            raise ValueError("dst must differ from src")
+        cb = -1 if opcode in FULL_FIELD else src.checkerboard().tag
+        _lib.check(_lib.load().wc_apply(self.obj, opcode, dst.ptr(), src.ptr(), cb, core.stream_ptr()))
+
+
+class fine_operator(matrix_operator):
+    """A gpt-style fermion operator object (operator/base.py:28-306)."""
+
+    def __init__(self, name, U, params, daggered=False, precision=None):
+        self.name = name
+        self.U = U
+        self.params_constructor = params
+        self.daggered = daggered
+        prec = precision if precision is not None else params.get("precision", core.double)
+        self.precision = prec
+        fd = U[0].grid.fdimensions
+        self.U_grid = core.grid(fd, prec, full)
+        self.F_grid = self.U_grid
+        self.U_grid_eo = self.U_grid.checkerboarded(redblack)
+        self.F_grid_eo = self.U_grid_eo
+        self.params = dict(params)
+        self.interface = interface()
+        self.interface.setup(self.U_grid, self.params, U)
+        iface = self.interface
+
+        def reg(opcode, out_cb=None):
+            def f(dst, src):
+                if src.grid.precision is not prec or dst.grid.precision is not prec:
+                    raise ValueError(f"field precision differs from the operator's ({prec.__name__})")
+                iface.apply_unary_operator(opcode, dst, src)
+                if dst.grid.cb is not full:
+                    dst.checkerboard(out_cb(src.checkerboard()))
+            return f
+
+        same = lambda cb: cb  # noqa: E731
+        flip = lambda cb: cb.inv()  # noqa: E731
+
+        def make_full(src):
+            return lattice(self.F_grid)
+
+        def make_like(src):
+            return lattice(src)
+
+        def OP(x):
+            return x if not daggered else x.adj()
+
+        M = reg(OPCODES["M"])
+        Mdag = reg(OPCODES["Mdag"])
+        super().__init__(mat=M if not daggered else Mdag, adj_mat=Mdag if not daggered else M,
+                         make_dst=make_full)
+        self.Meooe = OP(matrix_operator(reg(OPCODES["Meooe"], flip), reg(OPCODES["MeooeDag"], flip),
+                                        make_dst=make_like))
+        self.Mooee = OP(matrix_operator(reg(OPCODES["Mooee"], same), reg(OPCODES["MooeeDag"], same),
+                                        reg(OPCODES["MooeeInv"], same), reg(OPCODES["MooeeInvDag"], same),
+                                        make_dst=make_like))
+        self.DhopEO = OP(matrix_operator(reg(OPCODES["DhopEO"], flip), reg(OPCODES["DhopEODag"], flip),
+                                         make_dst=make_like))
+        self.Dhop = OP(matrix_operator(reg(OPCODES["Dhop"]), reg(OPCODES["DhopDag"]), make_dst=make_full))
+        self.Mdiag = OP(matrix_operator(reg(OPCODES["Mdiag"]), make_dst=make_full))
+        self.Dminus = OP(matrix_operator(reg(OPCODES["Dminus"]), reg(OPCODES["DminusDag"]),
+                                         make_dst=make_full))
+        for nm in ["ImportPhysicalFermionSource", "ImportUnphysicalFermion",
+                   "ExportPhysicalFermionSolution", "ExportPhysicalFermionSource"]:
+            setattr(self, nm, OP(matrix_operator(reg(OPCODES[nm]), make_dst=make_full)))
+
+        def _G5M(dst, src):
+            # gamma5 = diag(1, 1, -1, -1): negate the lower spins (components 6..11)
+            self.mat(dst, src)
+            dst.data[:, 6:12].neg_()
+
+        self.G5M = matrix_operator(_G5M, make_dst=make_full)
+
+    # native handle for the fused Schur / CG kernels
+    @property
+    def handle(self):
+        if self.interface.obj is None:
+            self.interface._setup(*self.interface.setup_arguments)
+        return self.interface.obj
+
+    def kappa(self):
+        p = self.params
+        return 1.0 / (2.0 * (p["mass"] + 1.0 + 3.0 * p["nu"] / p["xi_0"]))
+
+    def adj(self):
+        return type(self)(self.name, self.U, self.params_constructor, not self.daggered, self.precision)
+
+    def updated(self, U):
+        return type(self)(self.name, U, self.params_constructor, self.daggered, self.precision)
+
+    def modified(self, **params):
+        return type(self)(self.name, self.U, {**self.params_constructor, **params}, self.daggered,
+                          self.precision)
+
+    def converted(self, dst_precision):
+        """a second operator in another precision (operator/base.py:230-231)"""
+        return type(self)(self.name, self.U, self.params_constructor, self.daggered, dst_precision)
+
+    def update(self, U):
+        self.U = U
+        self.interface.update(self.params, U)
+
+    def suspend(self):
+        self.interface.suspend()
+
+    def arguments(self):
+        return self.U
+
+    def propagator(self, solver):
+        """exp * inv * imp (operator/base.py:270-286); Import/Export are the identity
+        for Wilson-clover."""
+        inv_matrix = solver(self)
+
+        def prop(dst, src):
+            inv_matrix(dst, src)
+
+        return matrix_operator(prop, make_dst=lambda s: lattice(self.F_grid), accept_guess=(True, False))
+
+    def even_odd_sites_decomposed(self, parity):
+        """operator/base.py:288-306: D = parity sites, C = the other parity."""
+        op = self
+
+        class even_odd_sites:
+            D_parity = parity
+            C_parity = parity.inv()
+            DD = op.Mooee
+            CC = op.Mooee
+            CD = op.Meooe
+            DC = op.Meooe
+            fermion = op
+
+        return even_odd_sites()
+
+
+def wilson_clover(U, params=None, **kwargs):
+    """g.qcd.fermion.wilson_clover (lib/gpt/qcd/fermion/wilson.py:63-96).
+
+    U: list of four link fields (gpt_amd.qcd.gauge) or natural-layout numpy
+    arrays; params as gpt: kappa or mass, csw_r, csw_t, cF, xi_0, nu,
+    isAnisotropic, boundary_phases, plus gpt_amd's ``precision``
+    (core.double / core.single) and ``compress_links`` (12-real links)."""
+    p = {"kappa": None, "mass": None, "cF": 1.0, "use_legacy": False, "boundary_phases": None,
+         "isAnisotropic": None, "csw_r": None, "csw_t": None, "nu": None, "xi_0": None, "n_rhs": 1}
+    p.update(_copy.deepcopy(params or {}))
+    p.update(kwargs)
+    if p["kappa"] is not None:
+        if p["mass"] is not None:
+            raise ValueError("give kappa or mass, not both")
+        p["mass"] = 1.0 / p["kappa"] / 2.0 - 4.0
+    del p["kappa"]
+    if p["mass"] is None:
+        raise ValueError("kappa or mass is required")
+    if p["boundary_phases"] is None:
+        raise ValueError("boundary_phases is required")
+    if complex(p["boundary_phases"][-1]) != 0.0 and p["cF"] != 1.0:
+        # wilson.py:89-90 forbids cF without an open temporal boundary
+        raise ValueError("cF != 1 needs an open temporal boundary")
+    for k, dflt in [("csw_r", 0.0), ("csw_t", 0.0), ("nu", 1.0), ("xi_0", 1.0)]:
+        if p[k] is None:
+            p[k] = dflt
+    if p["n_rhs"] != 1:
+        raise NotImplementedError("n_rhs > 1 (packed multi-RHS operator) is not available yet")
+    if not isinstance(U[0], gauge.link_field):
+        fd = list(reversed(U[0].shape[:4]))
+        U = gauge.from_numpy(core.grid(fd, core.double), U)
+    return fine_operator("wilson_clover", U, p)
+
+
+def apply_open_boundaries(field):
+    """zero the first and last time slice (lib/gpt/qcd/fermion/boundary_conditions.py:23-31)"""
+    if isinstance(field, list):
+        return [apply_open_boundaries(f) for f in field]
+    a = field.to_numpy()
+    a[0] = 0.0
+    a[-1] = 0.0
+    field.from_numpy(a)
+    return field

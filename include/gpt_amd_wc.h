@@ -1,0 +1,147 @@
+/*
+ * gpt_amd_wc.h -- C ABI of the MI355X even-odd Wilson-clover operator and
+ * its CG solve.
+ *
+ * This is the drop-in for the fermion-operator FFI that gpt's Python layer
+ * calls through cgpt (paths relative to the gpt repository root):
+ *   lib/cgpt/lib/operators.cc:34-133      create / update / set_mass /
+ *                                          delete / apply_fermion_operator
+ *   lib/cgpt/lib/operators/register.h:1-20 the opcode numbers (kept as is)
+ *   lib/cgpt/lib/transform.cc:143-246      lattice_axpy, lattice_norm2,
+ *                                          lattice_rank_inner_product
+ *   lib/cgpt/lib/foundation/transfer.h:27-75  pick / set checkerboard
+ * The caller is lib/gpt/qcd/fermion/operator/interface.py:29-103.
+ *
+ * Conventions
+ *   - All pointers to fields are DEVICE pointers owned by the caller; the
+ *     operator owns its links, clover blocks and scratch.
+ *   - A half field (one parity) is 12 x Vh complex values, component-major:
+ *     element (k, h) at index k*Vh + h, k = 3*spin + colour, half-site
+ *     h = (((t*Lz + z)*Ly + y)*Lx + x)/2.  A full field is two half fields,
+ *     even parity first.  Complex values are (re, im) pairs of double
+ *     (WC_DOUBLE) or float (WC_SINGLE).
+ *   - Links given to wc_create/wc_update are FP64 complex, natural order
+ *     [4][V][3][3], site index ((t*Lz + z)*Ly + y)*Lx + x, WITHOUT boundary
+ *     phases (the operator applies them, lib/gpt/core/covariant.py:28-37).
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Calls are
+ *     asynchronous except those that return a reduction to the host.
+ *   - Every call returns WC_OK or an error code; wc_last_error() gives the
+ *     message (the Python layer raises RuntimeError with it, mirroring
+ *     lib/cgpt/lib/exception.h:23-39).  Not thread safe per operator.
+ */
+#ifndef GPT_AMD_WC_H
+#define GPT_AMD_WC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WC_OK 0
+#define WC_ERR_ARG 1     /* bad argument: sizes, parities, aliasing, opcode */
+#define WC_ERR_HIP 2     /* a HIP runtime call failed */
+#define WC_ERR_NUMERIC 3 /* e.g. a near-singular clover block */
+
+#define WC_DOUBLE 0
+#define WC_SINGLE 1
+
+typedef struct wc_operator wc_operator;
+
+/* Parameters of g.qcd.fermion.wilson_clover (lib/gpt/qcd/fermion/wilson.py:63-96,
+ * parsed in lib/cgpt/lib/operators/wilson_clover.h:28-39).  kappa is
+ * converted to mass = 1/(2 kappa) - 4 by the caller, as wilson.py:78-80. */
+typedef struct wc_params {
+  int fdims[4];                  /* Lx, Ly, Lz, Lt: all even, >= 2 */
+  double mass;                   /* m0 */
+  double csw_r, csw_t;           /* clover coefficients (csw_r is divided by xi_0) */
+  double cF;                     /* open-boundary improvement coefficient */
+  double xi_0, nu;               /* anisotropy */
+  int isAnisotropic;
+  double boundary_phases[4][2];  /* (re, im) per direction; [3] = 0 -> open boundary */
+  int precision;                 /* WC_DOUBLE or WC_SINGLE (link/clover/field storage) */
+  int compress_links;            /* 1: store 12 reals per link, rebuild row 2 (SU(3) only) */
+} wc_params;
+
+typedef struct wc_solver_stats {
+  long iterations;  /* CG iterations (gpt's len(cg.history)) */
+  int converged;    /* 1 if |r|^2 <= eps^2 |b|^2 was reached */
+  double r2;        /* final iterated |r|^2 */
+  double b2;        /* |b|^2 */
+} wc_solver_stats;
+
+/* --- operator life cycle (cgpt operators.cc:34-94) ----------------------- */
+
+/* create_fermion_operator (operators.cc:34-57, wilson_clover.h:19-71):
+ * builds the double-stored links, the clover blocks A and A^-1. */
+int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out);
+/* update_fermion_operator (operators.cc:59-70, implementation.h:77-86) */
+int wc_update(wc_operator *op, const void *U, void *stream);
+/* set_mass_fermion_operator (operators.cc:72-83, implementation.h:21-29);
+ * rebuilds A and A^-1 for the new mass */
+int wc_set_mass(wc_operator *op, double mass, void *stream);
+/* delete_fermion_operator (operators.cc:85-94) */
+int wc_destroy(wc_operator *op);
+
+/* geometry / storage queries */
+long wc_half_volume(const wc_operator *op);
+int wc_precision(const wc_operator *op);
+
+/* apply_fermion_operator (operators.cc:96-107, operators/unary.h:36-48,
+ * register.h:1-20).  Opcodes:
+ *   full fields : 2001 M, 2002 Mdag, 2009 Mdiag, 2010/2011 Dminus(Dag) = copy,
+ *                 2012-2015 Import/Export = copy, 3001 Dhop, 4001 DhopDag
+ *   half fields : 2003 Meooe, 2004 MeooeDag, 2005 Mooee, 2006 MooeeDag,
+ *                 2007 MooeeInv, 2008 MooeeInvDag, 3002 DhopEO, 4002 DhopEODag
+ * src_cb: parity of the half-field src (0 even, 1 odd); ignored for full
+ * fields.  Meooe maps parity src_cb to 1 - src_cb; DhopEO needs src_cb = 1.
+ * dst must not alias src (reference operators assert dst != src). */
+int wc_apply(wc_operator *op, int opcode, void *dst, const void *src, int src_cb, void *stream);
+
+/* --- BLAS-1 on n complex entries (cgpt transform.cc:143-246) ------------- */
+/* lattice_axpy: z = a x + y */
+int wc_axpy(int prec, long n, double a_re, double a_im, const void *x, const void *y, void *z,
+            void *stream);
+/* z = a x */
+int wc_scale(int prec, long n, double a_re, double a_im, const void *x, void *z, void *stream);
+/* lattice_norm2: |x|^2 accumulated in FP64, result on the host */
+int wc_norm2(int prec, long n, const void *x, double *result, void *stream);
+/* lattice_rank_inner_product: <a,b> = sum conj(a) b in FP64, result[2] = (re, im) */
+int wc_inner_product(int prec, long n, const void *a, const void *b, double *result, void *stream);
+/* axpy_norm2 (lib/gpt/core/transform.py:151-153): z = a x + y, returns |z|^2 */
+int wc_axpy_norm2(int prec, long n, double a_re, double a_im, const void *x, const void *y, void *z,
+                  double *result, void *stream);
+/* g.convert between FP64 and FP32 storage (transform.cc:128-141) */
+int wc_convert(int dst_prec, void *dst, int src_prec, const void *src, long n, void *stream);
+
+/* natural [V][4][3] <-> even-odd full field (layout plumbing for I/O) */
+int wc_field_import(const int fdims[4], int prec, const void *natural, void *eo, void *stream);
+int wc_field_export(const int fdims[4], int prec, const void *eo, void *natural, void *stream);
+
+/* --- solvers ----------------------------------------------------------- */
+/* CG on the eo2_ne normal operator Mpc^dag Mpc with D = `parity`
+ * (lib/gpt/algorithms/inverter/cg.py:48-112 over
+ *  lib/gpt/algorithms/preconditioner/schur_complement_two.py:87-112 and
+ *  normal_equation.py:44-45).  x: initial guess in, solution out (half
+ * field of parity `parity`); b: right-hand side.  history (optional, maxiter
+ * doubles) receives |r|^2 per iteration.  Same recursion and stopping rule
+ * as gpt: stop when |r|^2 <= eps^2 |b|^2 on the iterated residual. */
+int wc_cg_eo2_ne(wc_operator *op, int parity, void *x, const void *b, double eps, long maxiter,
+                 double *history, wc_solver_stats *stats, void *stream);
+
+/* inv.preconditioned(pc.eo2_ne(parity), inv.cg) applied to one full-field
+ * right-hand side (lib/gpt/algorithms/inverter/preconditioned.py:32-54):
+ * dst = L CG(R_ne src, guess L^-1 dst) + S src.  dst holds the initial
+ * guess on entry (zero it for gpt's default). */
+int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,
+                    double *history, wc_solver_stats *stats, void *stream);
+
+/* the Schur pieces on half fields (schur_complement_two.py:87-148,
+ * normal_equation.py:44-48): which = 0 Mpc, 1 Mpc^dag, 2 Mpc^dag Mpc */
+int wc_apply_mpc(wc_operator *op, int parity, int which, void *dst, const void *src, void *stream);
+
+/* message of the last failed call on this thread */
+const char *wc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
