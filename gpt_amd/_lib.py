@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgpt_amd_wc.so")
+# GPT_AMD_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GPT_AMD_LIB", os.path.join(_HERE, "libgpt_amd_wc.so"))
 
 WC_OK, WC_ERR_ARG, WC_ERR_HIP, WC_ERR_NUMERIC = 0, 1, 2, 3
 WC_DOUBLE, WC_SINGLE = 0, 1

@@ -110,6 +110,8 @@ class grid:
         self.cb = cb
         self.gsites = int(np.prod(self.fdimensions))
         self.Vh = self.gsites // 2
+        if self.Vh % 64:
+            raise ValueError(f"the half volume {self.Vh} of {self.fdimensions} must be a multiple of 64")
         self.nd = 4
 
     def checkerboarded(self, cb):
@@ -135,8 +137,9 @@ class grid:
 class lattice:
     """A spin-colour vector field (gpt's vspincolor) in HBM.
 
-    Full grids hold both parities ``[2][12][Vh]`` complex; red-black grids one
-    parity ``[12][Vh]`` tagged by ``checkerboard()``."""
+    Full grids hold both parities, red-black grids one parity tagged by
+    ``checkerboard()``.  Storage is AoSoA in blocks of 64 half-sites:
+    ``data[parity][h // 64][component][h % 64][re/im]``."""
 
     def __init__(self, first, second=None):
         if isinstance(first, lattice):
@@ -147,7 +150,8 @@ class lattice:
             cb = even if g.cb is redblack else none
         self.grid = g
         self._cb = cb
-        shape = (2, 12, g.Vh, 2) if g.cb is full else (12, g.Vh, 2)
+        blocks = (g.Vh // 64, 12, 64, 2)
+        shape = (2,) + blocks if g.cb is full else blocks
         self.data = torch.empty(shape, dtype=g.precision.real_dtype, device=device())
 
     # gpt API

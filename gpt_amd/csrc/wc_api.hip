@@ -167,6 +167,14 @@ int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out
   if (p->precision != WC_DOUBLE && p->precision != WC_SINGLE)
     WC_FAIL(WC_ERR_ARG, "wc_create: unknown precision %d", p->precision);
   if (!(p->xi_0 > 0.0)) WC_FAIL(WC_ERR_ARG, "wc_create: xi_0 must be positive");
+  {
+    const long V = (long)p->fdims[0] * p->fdims[1] * p->fdims[2] * p->fdims[3];
+    if ((V / 2) % 64 != 0)
+      WC_FAIL(WC_ERR_ARG, "wc_create: the half volume %ld must be a multiple of 64 (AoSoA blocks)", V / 2);
+    const double csz = p->precision == WC_DOUBLE ? 16.0 : 8.0;
+    if (12.0 * (V / 2) * csz >= 4294967296.0)
+      WC_FAIL(WC_ERR_ARG, "wc_create: a half field of %ld sites exceeds the 4 GB gather window", V / 2);
+  }
   const bool open_bc = (p->boundary_phases[3][0] == 0.0 && p->boundary_phases[3][1] == 0.0);
   if (!open_bc && p->cF != 1.0) WC_FAIL(WC_ERR_ARG, "wc_create: cF != 1 needs an open temporal boundary");
   if (open_bc && (p->xi_0 != 1.0 || p->nu != 1.0 || p->csw_r != p->csw_t))
@@ -655,6 +663,7 @@ static int geom_of(const int fdims[4], Geom *g) {
   g->Lxh = fdims[0] / 2;
   g->V = (long)fdims[0] * fdims[1] * fdims[2] * fdims[3];
   g->Vh = g->V / 2;
+  if (g->Vh % 64 != 0) WC_FAIL(WC_ERR_ARG, "the half volume %ld must be a multiple of 64", g->Vh);
   return WC_OK;
 }
 

@@ -1,16 +1,20 @@
 // Shared device helpers for the even-odd Wilson-clover kernels (gfx950).
 //
-// Field layouts in HBM (R = double or float, C2 = R pair re/im):
-//   half field  (one parity)  : C2 psi[12][Vh]        component k = 3*spin + colour
-//   full field                : C2 psi[2][12][Vh]     parity-major
-//   links, output parity p    : C2 W[p][8][NL][Vh]    dir 2*mu   : U_mu(x)
-//                                                      dir 2*mu+1 : U_mu(x-mu)^dag
-//                               NL = 9 (3x3) or 6 (first two rows; row 2 is
-//                               rebuilt as conj(row0 x row1))
-//   clover / inverse, parity p: C2 A[p][36][Vh]       18 slots per chiral block:
-//                               3 slots of packed real diagonals, then the 15
+// Field layouts in HBM (R = double or float, C2 = R pair re/im).  Every
+// per-site array is AoSoA with blocks of 64 sites: entry k of an NK-entry
+// object at half-site h lives at ((h / 64) * NK + k) * 64 + h % 64, so one
+// wave's 64 sites read each entry as 1 KB (FP64) contiguous and all of a
+// wave's entries form one contiguous run.
+//   half field  (one parity)  : NK = 12, k = 3*spin + colour
+//   full field                : two half fields, even parity first
+//   links, output parity p    : NK = 8*NL, k = dir*NL + e; dir 2*mu : U_mu(x),
+//                               dir 2*mu+1 : U_mu(x-mu)^dag; NL = 9 (3x3) or
+//                               6 (first two rows; row 2 = conj(row0 x row1))
+//   clover / inverse, parity p: NK = 36, 18 slots per chiral block: 3 slots
+//                               of packed real diagonals, then the 15
 //                               strictly-lower entries (1,0),(2,0),(2,1),(3,0)..
-// Half-site index h = (((t*Lz + z)*Ly + y)*Lx + x) / 2 (x collapsed).
+// Half-site index h = (((t*Lz + z)*Ly + y)*Lx + x) / 2 (x collapsed); the
+// half volume Vh must be a multiple of 64.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,6 +53,15 @@ struct Geom {
   long V;     // full volume
 };
 
+constexpr int kBlk = 64;  // AoSoA block (sites)
+
+// element index of entry k of an NK-entry object at half-site h
+__host__ __device__ __forceinline__ long aidx(int k, long h, int NK) {
+  return ((h >> 6) * (long)NK + k) * kBlk + (h & 63);
+}
+// same, split into the per-site part (for a VGPR offset) and the entry part
+__host__ __device__ __forceinline__ long aidx_site(long h, int NK) { return (h >> 6) * (long)NK * kBlk + (h & 63); }
+
 // coordinates of half-site h of parity p
 __device__ __forceinline__ void half_coords(const Geom &g, long h, int p, int c[4]) {
   long r = h / g.Lxh;
@@ -69,6 +82,9 @@ __device__ __forceinline__ long half_index(const Geom &g, int x, int y, int z, i
 // workgroups round-robin) processes one contiguous chunk of the lattice and
 // its L2 sees the neighbour reuse of that chunk.  Bijective for any grid size.
 __device__ __forceinline__ long xcd_tile(long b, long nb) {
+#if !WC_XCD_MAP
+  return b;
+#endif
   long q = nb >> 3, rem = nb & 7;
   long k = b & 7, i = b >> 3;
   return k * q + (k < rem ? k : rem) + i;
@@ -112,10 +128,50 @@ __device__ __forceinline__ wc_rsrc make_rsrc(const void *base, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
 }
 
-template <typename C> __device__ __forceinline__ C bload(wc_rsrc r, uint32_t voff, uint32_t soff);
-template <> __device__ __forceinline__ double2 bload<double2>(wc_rsrc r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+// AUX: cache-policy bits of the load (gfx950: bit 1 = nt, non-temporal)
+template <typename C, int AUX = 0> struct bload_t;
+template <int AUX> struct bload_t<double2, AUX> {
+  static __device__ __forceinline__ double2 go(wc_rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, AUX));
+  }
+};
+template <int AUX> struct bload_t<float2, AUX> {
+  static __device__ __forceinline__ float2 go(wc_rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
+  }
+};
+template <typename C, int AUX = 0>
+__device__ __forceinline__ C bload(wc_rsrc r, uint32_t voff, uint32_t soff) {
+  return bload_t<C, AUX>::go(r, voff, soff);
 }
-template <> __device__ __forceinline__ float2 bload<float2>(wc_rsrc r, uint32_t voff, uint32_t soff) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
-}
+
+typedef int wc_v4i __attribute__((ext_vector_type(4)));
+typedef int wc_v2i __attribute__((ext_vector_type(2)));
+template <typename C, int AUX = 0> struct bstore_t;
+template <int AUX> struct bstore_t<double2, AUX> {
+  static __device__ __forceinline__ void go(double2 v, wc_rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(wc_v4i, v), r, (int)voff, (int)soff, AUX);
+  }
+};
+template <int AUX> struct bstore_t<float2, AUX> {
+  static __device__ __forceinline__ void go(float2 v, wc_rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wc_v2i, v), r, (int)voff, (int)soff, AUX);
+  }
+};
+
+// compile-time tuning knobs of the hop kernel (defaults are the tuned values)
+#ifndef WC_LINK_AUX
+#define WC_LINK_AUX 2  // links are read once per hop: non-temporal, keeps L2 for the spinors
+#endif
+#ifndef WC_SPINOR_AUX
+#define WC_SPINOR_AUX 0  // cache policy of the neighbour-spinor gathers
+#endif
+#ifndef WC_NT_STORE
+#define WC_NT_STORE 1  // the hop output is not re-read by the hop: non-temporal stores
+#endif
+#ifndef WC_HOP_MINWAVES
+#define WC_HOP_MINWAVES 1  // occupancy hint (waves per SIMD) of the hop kernel
+#endif
+#ifndef WC_XCD_MAP
+#define WC_XCD_MAP 1  // XCD-contiguous block -> tile mapping
+#endif

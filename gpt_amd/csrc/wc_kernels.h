@@ -39,7 +39,10 @@ struct HopLaunch {
 
 // number of blocks a hop / site launch uses for Vh sites
 int site_blocks(long Vh);
-constexpr int kSiteThreads = 256;
+#ifndef WC_SITE_THREADS
+#define WC_SITE_THREADS 64
+#endif
+constexpr int kSiteThreads = WC_SITE_THREADS;
 constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
 
 template <typename R> void hop(const HopLaunch &a, hipStream_t s);

@@ -24,17 +24,17 @@ template <typename R, typename C>
 __device__ __forceinline__ void clov_apply(const C *A, long Vh, long h, const C v[12], C o[12]) {
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
-    const C *Ab = A + (long)(b * 18) * Vh + h;
+    const C *Ab = A + aidx(b * 18, h, 36);
     R d[6];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      C dd = Ab[(long)k * Vh];
+      C dd = Ab[(long)k * kBlk];
       d[2 * k] = dd.x;
       d[2 * k + 1] = dd.y;
     }
     C L[15];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) L[k] = Ab[(long)(3 + k) * Vh];
+    for (int k = 0; k < 15; ++k) L[k] = Ab[(long)(3 + k) * kBlk];
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       C acc = cscale(v[6 * b + i], d[i]);
@@ -50,13 +50,27 @@ __device__ __forceinline__ void clov_apply(const C *A, long Vh, long h, const C 
 
 template <typename C>
 __device__ __forceinline__ void load12(const C *f, long Vh, long h, C v[12]) {
+  const C *p = f + aidx_site(h, 12);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = f[(long)k * Vh + h];
+  for (int k = 0; k < 12; ++k) v[k] = p[(long)k * kBlk];
 }
 template <typename C>
 __device__ __forceinline__ void store12(C *f, long Vh, long h, const C v[12]) {
+  C *p = f + aidx_site(h, 12);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) f[(long)k * Vh + h] = v[k];
+  for (int k = 0; k < 12; ++k) p[(long)k * kBlk] = v[k];
+}
+// output of a hop: written once, not re-read by this kernel
+template <typename C>
+__device__ __forceinline__ void store12_stream(C *f, long Vh, long h, const C v[12]) {
+#if WC_NT_STORE
+  const wc_rsrc r = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)sizeof(C)));
+  const uint32_t vo = (uint32_t)aidx_site(h, 12) * (uint32_t)sizeof(C);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) bstore_t<C, 2>::go(v[k], r, vo, (uint32_t)(k * kBlk * sizeof(C)));
+#else
+  store12(f, Vh, h, v);
+#endif
 }
 
 // ---------------------------------------------------------------- the hop
@@ -175,6 +189,11 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R
   const uint32_t Vh = (uint32_t)g.Vh;
   const uint32_t hh = (uint32_t)h;
   const wc_rsrc in = make_rsrc(inp, 12u * Vh * (uint32_t)sizeof(C));
+  // a wave covers one aligned 64-site block: its links are one contiguous
+  // 8*NL*64-entry run, addressed from a wave-uniform base
+  const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
+  const wc_rsrc rw = make_rsrc(W + blk * (8 * NL * kBlk), 8u * NL * kBlk * (uint32_t)sizeof(C));
+  const uint32_t vw = (uint32_t)(h & 63) * (uint32_t)sizeof(C);
   const uint32_t xh = (uint32_t)c[0] >> 1;
   const uint32_t st1 = (uint32_t)g.Lxh, st2 = st1 * (uint32_t)g.L[1], st3 = st2 * (uint32_t)g.L[2];
 #pragma unroll 1
@@ -196,11 +215,13 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R
         nb = (cm == 0) ? hh + wrap : hh - stride;
     }
     DirData<C> d;
+    const uint32_t vs = (uint32_t)aidx_site(nb, 12) * (uint32_t)sizeof(C);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) d.psi[k] = bload<C>(in, nb * (uint32_t)sizeof(C), (uint32_t)k * Vh * (uint32_t)sizeof(C));
-    const wc_rsrc rw = make_rsrc(W + (long)(dir * NL) * g.Vh, (uint32_t)NL * Vh * (uint32_t)sizeof(C));
+    for (int k = 0; k < 12; ++k) d.psi[k] = bload<C, WC_SPINOR_AUX>(in, vs, (uint32_t)(k * kBlk * sizeof(C)));
+    const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
 #pragma unroll
-    for (int e = 0; e < NL; ++e) d.u[e] = bload<C>(rw, hh * (uint32_t)sizeof(C), (uint32_t)e * Vh * (uint32_t)sizeof(C));
+    for (int e = 0; e < NL; ++e)
+      d.u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
     switch (dir) {
       case 0: compute_dir<R, NL, DAG, 0, 0>(a, c, d, acc); break;
       case 1: compute_dir<R, NL, DAG, 0, 1>(a, c, d, acc); break;
@@ -215,7 +236,7 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R
 }
 
 template <typename R, int NL, int DAG, int EPI>
-__global__ __launch_bounds__(kSiteThreads) void k_hop(const HopLaunch a) {
+__global__ __launch_bounds__(kSiteThreads, WC_HOP_MINWAVES) void k_hop(const HopLaunch a) {
   using C = typename c2<R>::t;
   __shared__ double red[kSiteThreads / 64 + 1];
   const Geom &g = a.g;
@@ -274,7 +295,7 @@ __global__ __launch_bounds__(kSiteThreads) void k_hop(const HopLaunch a) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) dot += (double)v[k].x * (double)res[k].x + (double)v[k].y * (double)res[k].y;
     }
-    store12((C *)a.out, Vh, h, res);
+    store12_stream((C *)a.out, Vh, h, res);
   }
   if (EPI & EPI_DOT) {
     double s = block_sum<kSiteThreads>(dot, red);
@@ -493,7 +514,7 @@ __global__ void k_import(Geom g, const typename c2<R>::t *nat, typename c2<R>::t
   int p = (x + y + z + t) & 1;
   long h = i >> 1;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) eo[((long)p * 12 + k) * g.Vh + h] = nat[i * 12 + k];
+  for (int k = 0; k < 12; ++k) eo[(long)p * 12 * g.Vh + aidx(k, h, 12)] = nat[i * 12 + k];
 }
 template <typename R>
 __global__ void k_export(Geom g, const typename c2<R>::t *eo, typename c2<R>::t *nat) {
@@ -509,7 +530,7 @@ __global__ void k_export(Geom g, const typename c2<R>::t *eo, typename c2<R>::t 
   int p = (x + y + z + t) & 1;
   long h = i >> 1;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) nat[i * 12 + k] = eo[((long)p * 12 + k) * g.Vh + h];
+  for (int k = 0; k < 12; ++k) nat[i * 12 + k] = eo[(long)p * 12 * g.Vh + aidx(k, h, 12)];
 }
 template <typename R> void import_field(const Geom &g, const void *nat, void *eo, hipStream_t s) {
   using C = typename c2<R>::t;
@@ -550,8 +571,8 @@ __global__ void k_build_links(Geom g, const double2 *U, int nl, typename c2<R>::
       const int r = e / 3, col = e % 3;
       double2 f = uf[e];
       double2 b = ub[col * 3 + r];  // (U^dag)_{r col} = conj(U_{col r})
-      W[((long)(2 * mu) * nl + e) * g.Vh + h] = cmk<C>((R)f.x, (R)f.y);
-      W[((long)(2 * mu + 1) * nl + e) * g.Vh + h] = cmk<C>((R)b.x, (R)(-b.y));
+      W[aidx(2 * mu * nl + e, h, 8 * nl)] = cmk<C>((R)f.x, (R)f.y);
+      W[aidx((2 * mu + 1) * nl + e, h, 8 * nl)] = cmk<C>((R)b.x, (R)(-b.y));
     }
   }
 }
@@ -708,13 +729,13 @@ __global__ __launch_bounds__(128) void k_build_clover(Geom g, const double2 *U, 
     // store A (packed)
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      A[(long)(ch * 18 + k) * Vh + h] = cmk<C>((R)B[ch][(2 * k) * 7].x, (R)B[ch][(2 * k + 1) * 7].x);
+      A[aidx(ch * 18 + k, h, 36)] = cmk<C>((R)B[ch][(2 * k) * 7].x, (R)B[ch][(2 * k + 1) * 7].x);
 #pragma unroll
     for (int ii = 1; ii < 6; ++ii)
 #pragma unroll
       for (int jj = 0; jj < ii; ++jj) {
         double2 v = B[ch][ii * 6 + jj];
-        A[(long)(ch * 18 + tri(ii, jj)) * Vh + h] = cmk<C>((R)v.x, (R)v.y);
+        A[aidx(ch * 18 + tri(ii, jj), h, 36)] = cmk<C>((R)v.x, (R)v.y);
       }
     // LDL^H: B = L D L^H (L unit lower)
     double d[6];
@@ -771,12 +792,12 @@ __global__ __launch_bounds__(128) void k_build_clover(Geom g, const double2 *U, 
         if (ii == jj) {
           B[ch][ii * 7] = s;  // B's diagonal is dead: keep the inverse's diagonal there
         } else {
-          Ai[(long)(ch * 18 + tri(ii, jj)) * Vh + h] = cmk<C>((R)s.x, (R)s.y);
+          Ai[aidx(ch * 18 + tri(ii, jj), h, 36)] = cmk<C>((R)s.x, (R)s.y);
         }
       }
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      Ai[(long)(ch * 18 + k) * Vh + h] = cmk<C>((R)B[ch][(2 * k) * 7].x, (R)B[ch][(2 * k + 1) * 7].x);
+      Ai[aidx(ch * 18 + k, h, 36)] = cmk<C>((R)B[ch][(2 * k) * 7].x, (R)B[ch][(2 * k + 1) * 7].x);
   }
 }
 

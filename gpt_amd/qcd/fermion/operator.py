@@ -169,7 +169,7 @@ class fine_operator(matrix_operator):
         def _G5M(dst, src):
             # gamma5 = diag(1, 1, -1, -1): negate the lower spins (components 6..11)
             self.mat(dst, src)
-            dst.data[:, 6:12].neg_()
+            dst.data[..., 6:12, :, :].neg_()
 
         self.G5M = matrix_operator(_G5M, make_dst=make_full)
 

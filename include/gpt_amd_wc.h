@@ -15,11 +15,13 @@
  * Conventions
  *   - All pointers to fields are DEVICE pointers owned by the caller; the
  *     operator owns its links, clover blocks and scratch.
- *   - A half field (one parity) is 12 x Vh complex values, component-major:
- *     element (k, h) at index k*Vh + h, k = 3*spin + colour, half-site
- *     h = (((t*Lz + z)*Ly + y)*Lx + x)/2.  A full field is two half fields,
- *     even parity first.  Complex values are (re, im) pairs of double
- *     (WC_DOUBLE) or float (WC_SINGLE).
+ *   - A half field (one parity) is 12 x Vh complex values in blocks of 64
+ *     sites: element (k, h) at index ((h/64)*12 + k)*64 + h%64, with
+ *     k = 3*spin + colour and half-site h = (((t*Lz + z)*Ly + y)*Lx + x)/2.
+ *     A full field is two half fields, even parity first.  Complex values
+ *     are (re, im) pairs of double (WC_DOUBLE) or float (WC_SINGLE).  The
+ *     half volume Vh must be a multiple of 64 (V a multiple of 128), and a
+ *     half field must fit in 4 GB.
  *   - Links given to wc_create/wc_update are FP64 complex, natural order
  *     [4][V][3][3], site index ((t*Lz + z)*Ly + y)*Lx + x, WITHOUT boundary
  *     phases (the operator applies them, lib/gpt/core/covariant.py:28-37).
