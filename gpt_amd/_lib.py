@@ -61,6 +61,10 @@ SIGNATURES = {
     "wc_cg_eo2_ne": (_I, [_P, _I, _P, _P, _D, _L, _P, ctypes.POINTER(wc_solver_stats), _P]),
     "wc_solve_eo2_ne": (_I, [_P, _I, _P, _P, _D, _L, _P, ctypes.POINTER(wc_solver_stats), _P]),
     "wc_apply_mpc": (_I, [_P, _I, _I, _P, _P, _P]),
+    "wc_apply_block": (_I, [_P, _I, _I, _P, _P, _I, _P]),
+    "wc_apply_mpc_block": (_I, [_P, _I, _I, _I, _P, _P, _P]),
+    "wc_cg_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
+    "wc_solve_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
     "wc_last_error": (ctypes.c_char_p, []),
 }
 

@@ -46,19 +46,31 @@ class cg:
         if not conv and self.fail_if_not_converged:
             raise ValueError("FATAL error: CG not converged")
 
+    def _native_result(self, N, hist, iters, conv):
+        """history: |r|^2 per iteration (of the slowest right-hand side when
+        there are several); iterations: per right-hand side"""
+        it = [int(x) for x in iters]
+        self.iterations = it
+        kmax = max(it) if it else 0
+        worst = int(np.argmax(it)) if it else 0
+        h = hist[: kmax * N].reshape(kmax, N)[:, worst] if kmax else hist[:0]
+        self.history = list(h)
+        self._finish(kmax, bool(all(conv)), h[-1] if kmax else 0.0, 0.0)
+
     def __call__(self, mat):
         if self._native_ok(mat):
             pc = mat.eo2_ne
 
             def inv_native(psi, src):
-                hist = np.zeros(max(self.maxiter, 1))
-                st = _lib.wc_solver_stats()
-                _lib.check(_lib.load().wc_cg_eo2_ne(
-                    pc.op.handle, pc.D.tag, psi.ptr(), src.ptr(), float(self.eps), int(self.maxiter),
-                    hist.ctypes.data, ctypes.byref(st), core.stream_ptr()))
+                N = src.grid.nrhs
+                hist = np.zeros(max(self.maxiter, 1) * N)
+                iters = np.zeros(N, dtype=np.int64)
+                conv = np.zeros(N, dtype=np.int32)
+                _lib.check(_lib.load().wc_cg_eo2_ne_block(
+                    pc.op.handle, pc.D.tag, N, psi.ptr(), src.ptr(), float(self.eps), int(self.maxiter),
+                    hist.ctypes.data, iters.ctypes.data, conv.ctypes.data, None, None, core.stream_ptr()))
                 psi.checkerboard(pc.D)
-                self.history = list(hist[: st.iterations])
-                self._finish(st.iterations, bool(st.converged), st.r2, self.eps**2 * st.b2)
+                self._native_result(N, hist, iters, conv)
 
             return matrix_operator(inv_native, inv_mat=mat.mat, make_dst=mat.make_dst,
                                    accept_guess=(True, False))
@@ -117,13 +129,15 @@ class preconditioned:
             if native:
                 # one fused call: R_ne, the guess L^-1 dst, CG, L x + S src
                 it = self.inverter
-                hist = np.zeros(max(it.maxiter, 1))
-                st = _lib.wc_solver_stats()
-                _lib.check(_lib.load().wc_solve_eo2_ne(
-                    matrix.op.handle, matrix.D.tag, dst.ptr(), src.ptr(), float(it.eps),
-                    int(it.maxiter), hist.ctypes.data, ctypes.byref(st), core.stream_ptr()))
-                it.history = list(hist[: st.iterations])
-                it._finish(st.iterations, bool(st.converged), st.r2, it.eps**2 * st.b2)
+                N = src.grid.nrhs
+                hist = np.zeros(max(it.maxiter, 1) * N)
+                iters = np.zeros(N, dtype=np.int64)
+                conv = np.zeros(N, dtype=np.int32)
+                _lib.check(_lib.load().wc_solve_eo2_ne_block(
+                    matrix.op.handle, matrix.D.tag, N, dst.ptr(), src.ptr(), float(it.eps),
+                    int(it.maxiter), hist.ctypes.data, iters.ctypes.data, conv.ctypes.data, None, None,
+                    core.stream_ptr()))
+                it._native_result(N, hist, iters, conv)
                 return
             pc_src = matrix.R(src)
             pc_dst = lattice(pc_src)

@@ -100,8 +100,11 @@ class grid:
     """A 4-d lattice geometry (lib/gpt/core/grid.py:98-284): ``fdimensions``
     [Lx, Ly, Lz, Lt], a precision, and full or red-black (even-odd) layout."""
 
-    def __init__(self, fdimensions, precision, cb=full):
+    def __init__(self, fdimensions, precision, cb=full, nrhs=1):
         self.fdimensions = [int(x) for x in fdimensions]
+        self.nrhs = int(nrhs)  # right-hand sides per site (gpt's 5-d n_rhs grid)
+        if not 1 <= self.nrhs <= 64:
+            raise ValueError("nrhs must be in [1, 64]")
         assert len(self.fdimensions) == 4, "gpt_amd supports 4-d grids"
         for L in self.fdimensions:
             if L < 2 or L % 2:
@@ -115,20 +118,28 @@ class grid:
         self.nd = 4
 
     def checkerboarded(self, cb):
-        return grid(self.fdimensions, self.precision, cb)
+        return grid(self.fdimensions, self.precision, cb, self.nrhs)
 
     def converted(self, prec):
-        return grid(self.fdimensions, prec, self.cb)
+        return grid(self.fdimensions, prec, self.cb, self.nrhs)
+
+    def inserted_dimension(self, n_rhs):
+        """the n_rhs grid of this geometry (lib/gpt/core/grid.py inserted_dimension)"""
+        return grid(self.fdimensions, self.precision, self.cb, n_rhs)
+
+    def single_rhs(self):
+        return grid(self.fdimensions, self.precision, self.cb, 1)
 
     def __eq__(self, o):
-        return (isinstance(o, grid) and self.fdimensions == o.fdimensions
+        return (isinstance(o, grid) and self.fdimensions == o.fdimensions and self.nrhs == o.nrhs
                 and self.precision is o.precision and self.cb is o.cb)
 
     def __hash__(self):
-        return hash((tuple(self.fdimensions), self.precision.__name__, self.cb.__name__))
+        return hash((tuple(self.fdimensions), self.nrhs, self.precision.__name__, self.cb.__name__))
 
     def __repr__(self):
-        return f"grid({self.fdimensions}, {self.precision.__name__}, {self.cb.__name__})"
+        r = f", nrhs={self.nrhs}" if self.nrhs > 1 else ""
+        return f"grid({self.fdimensions}, {self.precision.__name__}, {self.cb.__name__}{r})"
 
 
 # ----------------------------------------------------------------- lattices
@@ -138,8 +149,9 @@ class lattice:
     """A spin-colour vector field (gpt's vspincolor) in HBM.
 
     Full grids hold both parities, red-black grids one parity tagged by
-    ``checkerboard()``.  Storage is AoSoA in blocks of 64 half-sites:
-    ``data[parity][h // 64][component][h % 64][re/im]``."""
+    ``checkerboard()``.  Storage is AoSoA in blocks of 64 half-sites with the
+    right-hand side innermost:
+    ``data[parity][h // 64][component][h % 64][rhs][re/im]``."""
 
     def __init__(self, first, second=None):
         if isinstance(first, lattice):
@@ -150,7 +162,7 @@ class lattice:
             cb = even if g.cb is redblack else none
         self.grid = g
         self._cb = cb
-        blocks = (g.Vh // 64, 12, 64, 2)
+        blocks = (g.Vh // 64, 12, 64, g.nrhs, 2)
         shape = (2,) + blocks if g.cb is full else blocks
         self.data = torch.empty(shape, dtype=g.precision.real_dtype, device=device())
 
@@ -217,9 +229,29 @@ class lattice:
     def __truediv__(self, s):
         return expression.of(self) * (1.0 / s)
 
+    @property
+    def nrhs(self):
+        return self.grid.nrhs
+
+    def rhs(self, r):
+        """a single-RHS copy of right-hand side r"""
+        out = lattice(self.grid.single_rhs())
+        out._cb = self._cb
+        out.data.copy_(self.data[..., r:r + 1, :])
+        return out
+
+    def set_rhs(self, r, field):
+        if field.grid.nrhs != 1 or field.grid.Vh != self.grid.Vh or field.grid.cb is not self.grid.cb:
+            raise ValueError("set_rhs needs a single-RHS field of the same geometry")
+        self.data[..., r:r + 1, :].copy_(field.data)
+        return self
+
     # host transfer (layout plumbing): natural numpy [t, z, y, x, 4, 3]
+    # (a leading [nrhs] axis for multi-RHS fields)
     def to_numpy(self):
         g = self.grid
+        if g.nrhs > 1:
+            return np.stack([self.rhs(r).to_numpy() for r in range(g.nrhs)])
         if g.cb is full:
             nat = torch.empty((g.gsites, 12, 2), dtype=self.data.dtype, device=self.data.device)
             fd = (ctypes_int4)(*g.fdimensions)
@@ -237,6 +269,15 @@ class lattice:
     def from_numpy(self, arr):
         g = self.grid
         arr = np.asarray(arr)
+        if g.nrhs > 1:
+            if arr.shape[0] != g.nrhs:
+                raise ValueError(f"expected a leading axis of {g.nrhs} right-hand sides")
+            for r in range(g.nrhs):
+                one = lattice(g.single_rhs())
+                one._cb = self._cb
+                one.from_numpy(arr[r])
+                self.set_rhs(r, one)
+            return self
         shape = tuple(reversed(g.fdimensions)) + (4, 3)
         if arr.shape != shape:
             raise ValueError(f"expected natural layout {shape}, got {arr.shape}")
@@ -301,11 +342,14 @@ def scale(z, a, x):
 
 
 def norm2(x):
-    """|x|^2 accumulated in FP64 (lib/gpt/core/transform.py:109-114)"""
+    """|x|^2 accumulated in FP64 (lib/gpt/core/transform.py:109-114); a list
+    (one per right-hand side) for multi-RHS fields"""
     if isinstance(x, (list, tuple)):
         return [norm2(y) for y in x]
     if isinstance(x, expression):
         x = eval(x)
+    if x.nrhs > 1:
+        return [norm2(x.rhs(r)) for r in range(x.nrhs)]
     r = _ct.c_double()
     _lib.check(_lib.load().wc_norm2(x.grid.precision.code, x.nentries, x.ptr(), _ct.byref(r),
                                     stream_ptr()))
@@ -457,13 +501,15 @@ class matrix_operator:
     place, ``op.mat(dst, src)`` is the raw application."""
 
     def __init__(self, mat, adj_mat=None, inv_mat=None, adj_inv_mat=None, make_dst=None,
-                 accept_guess=(False, False)):
+                 accept_guess=(False, False), accept_list=False):
         self.mat = mat
         self.adj_mat = adj_mat
         self.inv_mat = inv_mat
         self.adj_inv_mat = adj_inv_mat
         self.make_dst = make_dst
         self.accept_guess = accept_guess
+        # accept_list: mat(dst_list, src_list) takes whole lists (matrix_operator.py:288-295)
+        self.accept_list = accept_list
 
     def adj(self):
         return matrix_operator(self.adj_mat, self.mat, self.adj_inv_mat, self.inv_mat, self.make_dst)
@@ -485,9 +531,16 @@ class matrix_operator:
     def __call__(self, first, second=None):
         if second is None:
             if isinstance(first, list):
+                if self.accept_list:
+                    dst = [self._new_dst(x) for x in first]
+                    self.mat(dst, first)
+                    return dst
                 return [self(x) for x in first]
             return self._apply(eval(first) if isinstance(first, expression) else first)
         if isinstance(first, list):
+            if self.accept_list:
+                self.mat(first, second)
+                return first
             for d, s in zip(first, second):
                 self.mat(d, s)
             return first

@@ -49,20 +49,51 @@ struct wc_operator {
   void *Ai[2] = {nullptr, nullptr};
   void *Unat = nullptr;  // FP64 links [4][V][9], kept to rebuild the clover on set_mass
   int *err = nullptr;    // device flag of the clover build
-  std::vector<void *> tmp;  // half-field scratch
-  double *partials = nullptr;
-  int npartials = 0;
-  double *st = nullptr;     // device scalars: CG state [0..7], reductions [8..15]
-  double *host = nullptr;   // pinned host mirror
+  std::vector<void *> tmp;      // half-field scratch, tmp_bytes each
+  size_t tmp_bytes = 0;
+  double *partials = nullptr;   // per-block reduction partials
+  size_t npartials = 0;
+  double *st = nullptr;         // device scalars (CG state), nst doubles
+  double *host = nullptr;       // pinned host mirror, nst doubles
+  size_t nst = 0;
 };
 
 static size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
 
-static int ensure_tmp(wc_operator *op, size_t n) {
+// n scratch half fields of nrhs right-hand sides each
+static int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
+  const size_t bytes = half_bytes(op) * (size_t)nrhs;
+  if (bytes > op->tmp_bytes) {
+    for (void *q : op->tmp) (void)hipFree(q);
+    op->tmp.clear();
+    op->tmp_bytes = bytes;
+  }
   while (op->tmp.size() < n) {
     void *q = nullptr;
-    WC_HIP(hipMalloc(&q, half_bytes(op)));
+    WC_HIP(hipMalloc(&q, op->tmp_bytes));
     op->tmp.push_back(q);
+  }
+  return WC_OK;
+}
+
+// reduction partials and CG scalars for nrhs right-hand sides
+static int ensure_red(wc_operator *op, int nrhs) {
+  const size_t np = (size_t)std::max(wck::kRedBlocks, wck::site_blocks(op->g.Vh * nrhs)) * nrhs;
+  if (np > op->npartials) {
+    (void)hipFree(op->partials);
+    op->partials = nullptr;
+    WC_HIP(hipMalloc((void **)&op->partials, sizeof(double) * np));
+    op->npartials = np;
+  }
+  const size_t ns = (size_t)16 * nrhs;
+  if (ns > op->nst) {
+    (void)hipFree(op->st);
+    if (op->host) (void)hipHostFree(op->host);
+    op->st = nullptr;
+    op->host = nullptr;
+    WC_HIP(hipMalloc((void **)&op->st, sizeof(double) * ns));
+    WC_HIP(hipHostMalloc((void **)&op->host, sizeof(double) * ns, hipHostMallocDefault));
+    op->nst = ns;
   }
   return WC_OK;
 }
@@ -213,10 +244,7 @@ int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out
   }
   chk(hipMalloc(&op->Unat, (size_t)op->g.V * 4 * 9 * 16), "hipMalloc(gauge copy)");
   chk(hipMalloc((void **)&op->err, sizeof(int)), "hipMalloc(flag)");
-  op->npartials = std::max(wck::kRedBlocks, wck::site_blocks(op->g.Vh));
-  chk(hipMalloc((void **)&op->partials, sizeof(double) * op->npartials), "hipMalloc(partials)");
-  chk(hipMalloc((void **)&op->st, sizeof(double) * 16), "hipMalloc(scalars)");
-  chk(hipHostMalloc((void **)&op->host, sizeof(double) * 16, hipHostMallocDefault), "hipHostMalloc");
+  if (rc == WC_OK && ensure_red(op, 1) != WC_OK) rc = WC_ERR_HIP;
   if (rc != WC_OK) {
     free_op(op);
     return rc;
@@ -262,9 +290,10 @@ int wc_precision(const wc_operator *op) { return op ? op->prec : -1; }
 
 // ------------------------------------------------------------ compositions
 template <typename R>
-static HopLaunch hop_args(const wc_operator *op, int pout, int dag, int epi, const void *in, void *out) {
+static HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, int epi, const void *in, void *out) {
   HopLaunch a{};
   a.g = op->g;
+  a.nrhs = nrhs;
   a.pout = pout;
   a.dag = dag;
   a.nl = op->nl;
@@ -286,32 +315,35 @@ static HopLaunch hop_args(const wc_operator *op, int pout, int dag, int epi, con
 
 // out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
 template <typename R>
-static void site_op(const wc_operator *op, int p, int inv, const void *in, void *out, const void *y,
+static void site_op(const wc_operator *op, int nrhs, int p, int inv, const void *in, void *out, const void *y,
                     hipStream_t s) {
   const void *B = op->has_clover ? (inv ? op->Ai[p] : op->A[p]) : nullptr;
   double sc = inv ? 1.0 / op->diag : op->diag;
-  wck::site_clover<R>(op->g, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s);
+  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s);
 }
 
-// Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd; dotv (optional)
-// makes the last hop write per-block partials of Re<dotv, out>
+// Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd on nrhs
+// right-hand sides; dotv (optional) makes the last hop write per-block,
+// per-RHS partials of Re<dotv, out>.  Scratch: tmp[0..2].
 template <typename R>
-static int mpc_impl(wc_operator *op, int pd, int which, void *out, const void *in, const void *dotv,
+static int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
                     hipStream_t s) {
   const int pc = 1 - pd;
-  int rc = ensure_tmp(op, 3);
+  int rc = ensure_tmp(op, 3, nrhs);
   if (rc) return rc;
+  if ((rc = ensure_red(op, nrhs))) return rc;
   void *tD = op->tmp[0], *tC = op->tmp[1], *tY = op->tmp[2];
   const void *clovC = op->has_clover ? op->Ai[pc] : nullptr;
   const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
   const void *src = in;
   void *dst_mpc = (which == 2) ? tY : out;
   if (which == 0 || which == 2) {
-    site_op<R>(op, pd, 1, src, tD, nullptr, s);
-    HopLaunch a = hop_args<R>(op, pc, 0, wck::EPI_CLOV, tD, tC);
+    site_op<R>(op, nrhs, pd, 1, src, tD, nullptr, s);
+    HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV, tD, tC);
     a.clov = clovC;
     wck::hop<R>(a, s);
-    HopLaunch b = hop_args<R>(op, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC, dst_mpc);
+    HopLaunch b = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC,
+                              dst_mpc);
     b.y = src;
     b.dotv = dotv;
     b.partial = op->partials;
@@ -319,10 +351,11 @@ static int mpc_impl(wc_operator *op, int pd, int which, void *out, const void *i
     src = dst_mpc;
   }
   if (which == 1 || which == 2) {
-    HopLaunch a = hop_args<R>(op, pc, 1, wck::EPI_CLOV, src, tC);
+    HopLaunch a = hop_args<R>(op, nrhs, pc, 1, wck::EPI_CLOV, src, tC);
     a.clov = clovC;
     wck::hop<R>(a, s);
-    HopLaunch b = hop_args<R>(op, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC, out);
+    HopLaunch b = hop_args<R>(op, nrhs, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC,
+                              out);
     b.clov = clovD;
     b.y = src;
     b.dotv = dotv;
@@ -334,9 +367,8 @@ static int mpc_impl(wc_operator *op, int pd, int which, void *out, const void *i
 }
 
 template <typename R>
-static int apply_impl(wc_operator *op, int opcode, void *dst, const void *src, int cb, hipStream_t s) {
-  const size_t hb = half_bytes(op);
-  const long Vh = op->g.Vh;
+static int apply_impl(wc_operator *op, int nrhs, int opcode, void *dst, const void *src, int cb, hipStream_t s) {
+  const size_t hb = half_bytes(op) * (size_t)nrhs;
   auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
   switch (opcode) {
     case 2001:
@@ -346,7 +378,7 @@ static int apply_impl(wc_operator *op, int opcode, void *dst, const void *src, i
       const int dag = (opcode == 2002 || opcode == 4001);
       const bool withdiag = (opcode == 2001 || opcode == 2002);
       for (int p = 0; p < 2; ++p) {
-        HopLaunch a = hop_args<R>(op, p, dag, withdiag ? wck::EPI_DIAG : 0, half(src, 1 - p), half(dst, p));
+        HopLaunch a = hop_args<R>(op, nrhs, p, dag, withdiag ? wck::EPI_DIAG : 0, half(src, 1 - p), half(dst, p));
         a.diag_clov = op->has_clover ? op->A[p] : nullptr;
         a.y = half(src, p);
         wck::hop<R>(a, s);
@@ -354,7 +386,7 @@ static int apply_impl(wc_operator *op, int opcode, void *dst, const void *src, i
       break;
     }
     case 2009:  // Mdiag
-      for (int p = 0; p < 2; ++p) site_op<R>(op, p, 0, half(src, p), half(dst, p), nullptr, s);
+      for (int p = 0; p < 2; ++p) site_op<R>(op, nrhs, p, 0, half(src, p), half(dst, p), nullptr, s);
       break;
     case 2010:
     case 2011:
@@ -372,7 +404,7 @@ static int apply_impl(wc_operator *op, int opcode, void *dst, const void *src, i
       if ((opcode == 3002 || opcode == 4002) && cb != 1)
         WC_FAIL(WC_ERR_ARG, "wc_apply: DhopEO maps odd to even (src_cb must be 1)");
       const int dag = (opcode == 2004 || opcode == 4002);
-      HopLaunch a = hop_args<R>(op, 1 - cb, dag, 0, src, dst);
+      HopLaunch a = hop_args<R>(op, nrhs, 1 - cb, dag, 0, src, dst);
       wck::hop<R>(a, s);
       break;
     }
@@ -381,124 +413,142 @@ static int apply_impl(wc_operator *op, int opcode, void *dst, const void *src, i
     case 2007:
     case 2008:
       if (cb != 0 && cb != 1) WC_FAIL(WC_ERR_ARG, "wc_apply: opcode %d needs src_cb 0 or 1", opcode);
-      site_op<R>(op, cb, opcode >= 2007, src, dst, nullptr, s);
+      site_op<R>(op, nrhs, cb, opcode >= 2007, src, dst, nullptr, s);
       break;
     default:
       WC_FAIL(WC_ERR_ARG, "Unknown opcode %d", opcode);
   }
-  (void)Vh;
   WC_LAUNCHED();
   return WC_OK;
 }
 
-// host read of a device double (synchronising the stream)
-static int read_scalar(wc_operator *op, const double *dev, double *h, hipStream_t s) {
-  WC_HIP(hipMemcpyAsync(op->host, dev, sizeof(double), hipMemcpyDeviceToHost, s));
-  WC_HIP(hipStreamSynchronize(s));
-  *h = op->host[0];
-  return WC_OK;
-}
-
+// per-RHS |x|^2 into dev_out[0..nrhs)
 template <typename R>
-static int norm2_dev(wc_operator *op, long n, const void *x, double *dev_out, hipStream_t s) {
-  wck::norm2_partials<R>(n, x, op->partials, s);
-  wck::reduce_final(op->partials, wck::kRedBlocks, dev_out, s);
-  WC_LAUNCHED();
-  return WC_OK;
-}
-
-// CG of lib/gpt/algorithms/inverter/cg.py:48-112 on Mpc^dag Mpc
-template <typename R>
-static int cg_impl(wc_operator *op, int pd, void *x, const void *b, double eps, long maxiter, double *history,
-                   wc_solver_stats *stats, hipStream_t s) {
-  int rc = ensure_tmp(op, 6);
+static int norm2_rhs(wc_operator *op, int nrhs, const void *x, double *dev_out, hipStream_t s) {
+  int rc = ensure_red(op, nrhs);
   if (rc) return rc;
+  const long n = op->g.Vh * 12 * nrhs;
+  wck::dot_partials_rhs<R>(n, nrhs, x, x, op->partials, nullptr, s);
+  wck::reduce_final_rhs(op->partials, wck::kRedBlocks, nrhs, dev_out, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+// CG of lib/gpt/algorithms/inverter/cg.py:48-112 on Mpc^dag Mpc, run for nrhs
+// independent right-hand sides at once (each its own scalars, the operator
+// applied to all of them in one pass).  A right-hand side that converges is
+// frozen (alpha = beta = 0 from then on).  iters/conv/r2/b2: nrhs entries
+// each (may be null); history: maxiter * nrhs doubles (may be null).
+template <typename R>
+static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, double eps, long maxiter,
+                   double *history, long *iters, int *conv, double *r2, double *b2, hipStream_t s) {
+  int rc = ensure_tmp(op, 6, nrhs);
+  if (rc) return rc;
+  if ((rc = ensure_red(op, nrhs))) return rc;
+  const int N = nrhs;
   void *r = op->tmp[3], *p = op->tmp[4], *mmp = op->tmp[5];
-  const long n = op->g.Vh * 12;
-  const int hop_blocks = wck::site_blocks(op->g.Vh);
-  double *st = op->st;
-  // mmp = A x ; r = b - mmp ; p = r ; cp = |p|^2 ; ssq = |b|^2
-  if ((rc = mpc_impl<R>(op, pd, 2, mmp, x, nullptr, s))) return rc;
+  const long n = op->g.Vh * 12 * N;
+  const int hop_blocks = wck::site_blocks(op->g.Vh * N);
+  double *st = op->st, *hs = op->host;
+  // mmp = A x ; r = b - mmp ; p = r ; c = |p|^2 ; ssq = |b|^2
+  if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, x, nullptr, s))) return rc;
   wck::axpy<R>(n, -1.0, 0.0, mmp, b, r, s);
-  WC_HIP(hipMemcpyAsync(p, r, half_bytes(op), hipMemcpyDeviceToDevice, s));
-  if ((rc = norm2_dev<R>(op, n, p, st + 0, s))) return rc;  // st[0] = c = cp
-  if ((rc = norm2_dev<R>(op, n, b, st + 8, s))) return rc;
-  double ssq = 0, cp = 0;
-  if ((rc = read_scalar(op, st + 8, &ssq, s))) return rc;
-  if ((rc = read_scalar(op, st + 0, &cp, s))) return rc;
-  if (stats) {
-    stats->iterations = 0;
-    stats->converged = 0;
-    stats->r2 = cp;
-    stats->b2 = ssq;
+  WC_HIP(hipMemcpyAsync(p, r, half_bytes(op) * N, hipMemcpyDeviceToDevice, s));
+  if ((rc = norm2_rhs<R>(op, N, p, st + 0 * N, s))) return rc;  // c
+  if ((rc = norm2_rhs<R>(op, N, b, st + 6 * N, s))) return rc;  // |b|^2
+  WC_HIP(hipMemcpyAsync(hs, st, sizeof(double) * 7 * N, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  std::vector<double> rsq(N), active(N, 1.0);
+  int nactive = 0;
+  for (int i = 0; i < N; ++i) {
+    const double ssq = hs[6 * N + i];
+    rsq[i] = eps * eps * ssq;
+    if (iters) iters[i] = 0;
+    if (conv) conv[i] = 0;
+    if (r2) r2[i] = hs[i];
+    if (b2) b2[i] = ssq;
+    if (ssq == 0.0) {  // zero source: zero solution, nothing to iterate (cg.py:69-72)
+      active[i] = 0.0;
+      if (conv) conv[i] = 1;
+    } else {
+      ++nactive;
+    }
   }
-  if (ssq == 0.0) {
-    WC_HIP(hipMemsetAsync(x, 0, half_bytes(op), s));
-    if (stats) stats->converged = 1;
-    return WC_OK;
+  if (nactive < N) {
+    for (int i = 0; i < N; ++i) hs[7 * N + i] = active[i];
+    WC_HIP(hipMemcpyAsync(st + 7 * N, hs + 7 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    wck::scale_rhs<R>(n, N, st + 7 * N, x, s);
   }
-  const double rsq = eps * eps * ssq;
+  for (int i = 0; i < N; ++i) hs[i] = active[i];
+  WC_HIP(hipMemcpyAsync(st + 5 * N, hs, sizeof(double) * N, hipMemcpyHostToDevice, s));
   long k = 0;
-  for (; k < maxiter; ++k) {
-    // mmp = A p with the partials of Re<p, mmp>
-    if ((rc = mpc_impl<R>(op, pd, 2, mmp, p, p, s))) return rc;
-    wck::cg_alpha(op->partials, hop_blocks, st, s);
-    wck::cg_update1<R>(n, st, r, mmp, x, p, op->partials, s);
-    wck::cg_beta(op->partials, wck::kRedBlocks, st, s);
-    wck::cg_update2<R>(n, st, r, p, s);
+  for (; k < maxiter && nactive > 0; ++k) {
+    // mmp = A p with per-RHS partials of Re<p, mmp>
+    if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, p, p, s))) return rc;
+    wck::cg_alpha(op->partials, hop_blocks, N, st, s);
+    wck::cg_update1<R>(n, N, st, r, mmp, x, p, op->partials, s);
+    wck::cg_beta(op->partials, wck::kRedBlocks, N, st, s);
+    wck::cg_update2<R>(n, N, st, r, p, s);
     WC_LAUNCHED();
-    if ((rc = read_scalar(op, st + 4, &cp, s))) return rc;
-    if (history) history[k] = cp;
-    if (!(std::isfinite(cp))) {
-      if (stats) { stats->iterations = k + 1; stats->r2 = cp; }
-      WC_FAIL(WC_ERR_NUMERIC, "CG: residual became %g at iteration %ld", cp, k + 1);
-    }
-    if (std::fabs(cp) <= rsq) {
-      ++k;
-      if (stats) {
-        stats->iterations = k;
-        stats->converged = 1;
-        stats->r2 = cp;
+    WC_HIP(hipMemcpyAsync(hs + 4 * N, st + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    WC_HIP(hipStreamSynchronize(s));
+    bool changed = false;
+    for (int i = 0; i < N; ++i) {
+      if (active[i] == 0.0) continue;
+      const double cp = hs[4 * N + i];
+      if (history) history[k * N + i] = cp;
+      if (r2) r2[i] = cp;
+      if (!std::isfinite(cp)) {
+        if (iters) iters[i] = k + 1;
+        WC_FAIL(WC_ERR_NUMERIC, "CG: residual of right-hand side %d became %g at iteration %ld", i, cp, k + 1);
       }
-      return WC_OK;
+      if (std::fabs(cp) <= rsq[i]) {
+        active[i] = 0.0;
+        --nactive;
+        changed = true;
+        if (iters) iters[i] = k + 1;
+        if (conv) conv[i] = 1;
+      }
+    }
+    if (changed && nactive > 0) {
+      for (int i = 0; i < N; ++i) hs[5 * N + i] = active[i];
+      WC_HIP(hipMemcpyAsync(st + 5 * N, hs + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
     }
   }
-  if (stats) {
-    stats->iterations = k;
-    stats->converged = 0;
-    stats->r2 = cp;
-  }
+  for (int i = 0; i < N; ++i)
+    if (active[i] != 0.0 && iters) iters[i] = k;
   return WC_OK;
 }
 
+// dst = L CG(R_ne src) + S src for nrhs right-hand sides (preconditioned.py:32-54)
 template <typename R>
-static int solve_impl(wc_operator *op, int pd, void *dst, const void *src, double eps, long maxiter,
-                      double *history, wc_solver_stats *stats, hipStream_t s) {
+static int solve_impl(wc_operator *op, int nrhs, int pd, void *dst, const void *src, double eps, long maxiter,
+                      double *history, long *iters, int *conv, double *r2, double *b2, hipStream_t s) {
   const int pc = 1 - pd;
-  int rc = ensure_tmp(op, 9);
+  int rc = ensure_tmp(op, 9, nrhs);
   if (rc) return rc;
-  const size_t hb = half_bytes(op);
+  const size_t hb = half_bytes(op) * (size_t)nrhs;
   auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
   void *tR = op->tmp[6], *pb = op->tmp[7], *px = op->tmp[8];
   void *tC = op->tmp[1];
   // R src = src_D - Meooe A_C^-1 src_C
-  site_op<R>(op, pc, 1, half(src, pc), tC, nullptr, s);
+  site_op<R>(op, nrhs, pc, 1, half(src, pc), tC, nullptr, s);
   {
-    HopLaunch a = hop_args<R>(op, pd, 0, wck::EPI_AXPY, tC, tR);
+    HopLaunch a = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY, tC, tR);
     a.y = half(src, pd);
     wck::hop<R>(a, s);
   }
   // R_ne src = Mpc^dag R src
-  if ((rc = mpc_impl<R>(op, pd, 1, pb, tR, nullptr, s))) return rc;
+  if ((rc = mpc_impl<R>(op, nrhs, pd, 1, pb, tR, nullptr, s))) return rc;
   // guess L^-1 dst = A_D dst_D
-  site_op<R>(op, pd, 0, half(dst, pd), px, nullptr, s);
+  site_op<R>(op, nrhs, pd, 0, half(dst, pd), px, nullptr, s);
   WC_LAUNCHED();
-  if ((rc = cg_impl<R>(op, pd, px, pb, eps, maxiter, history, stats, s))) return rc;
+  if ((rc = cg_impl<R>(op, nrhs, pd, px, pb, eps, maxiter, history, iters, conv, r2, b2, s))) return rc;
   // dst = L x + S src: dst_D = A_D^-1 x ; dst_C = A_C^-1 src_C - A_C^-1 Meooe dst_D
-  site_op<R>(op, pd, 1, px, half(dst, pd), nullptr, s);
-  site_op<R>(op, pc, 1, half(src, pc), tR, nullptr, s);
+  site_op<R>(op, nrhs, pd, 1, px, half(dst, pd), nullptr, s);
+  site_op<R>(op, nrhs, pc, 1, half(src, pc), tR, nullptr, s);
   {
-    HopLaunch a = hop_args<R>(op, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
+    HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
     a.clov = op->has_clover ? op->Ai[pc] : nullptr;
     a.y = tR;
     wck::hop<R>(a, s);
@@ -507,49 +557,112 @@ static int solve_impl(wc_operator *op, int pd, void *dst, const void *src, doubl
   return WC_OK;
 }
 
-extern "C" {
-
-int wc_apply(wc_operator *op, int opcode, void *dst, const void *src, int src_cb, void *stream) {
-  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply: null argument");
-  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply: dst must not alias src");
-  hipStream_t s = (hipStream_t)stream;
-  return op->prec == WC_DOUBLE ? apply_impl<double>(op, opcode, dst, src, src_cb, s)
-                               : apply_impl<float>(op, opcode, dst, src, src_cb, s);
+static int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
+  if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);
+  if ((double)half_bytes(op) * nrhs >= 4294967296.0)
+    WC_FAIL(WC_ERR_ARG, "%s: a half field of %d right-hand sides exceeds the 4 GB gather window", fn, nrhs);
+  return WC_OK;
 }
 
-int wc_apply_mpc(wc_operator *op, int parity, int which, void *dst, const void *src, void *stream) {
+extern "C" {
+
+int wc_apply_block(wc_operator *op, int opcode, int nrhs, void *dst, const void *src, int src_cb, void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply: dst must not alias src");
+  int rc = check_nrhs(op, nrhs, "wc_apply");
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return op->prec == WC_DOUBLE ? apply_impl<double>(op, nrhs, opcode, dst, src, src_cb, s)
+                               : apply_impl<float>(op, nrhs, opcode, dst, src, src_cb, s);
+}
+
+int wc_apply(wc_operator *op, int opcode, void *dst, const void *src, int src_cb, void *stream) {
+  return wc_apply_block(op, opcode, 1, dst, src, src_cb, stream);
+}
+
+int wc_apply_mpc_block(wc_operator *op, int parity, int which, int nrhs, void *dst, const void *src,
+                       void *stream) {
   if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: null argument");
   if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: dst must not alias src");
   if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: parity must be 0 or 1");
   if (which < 0 || which > 2) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc: which must be 0, 1 or 2");
+  int rc = check_nrhs(op, nrhs, "wc_apply_mpc");
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  return op->prec == WC_DOUBLE ? mpc_impl<double>(op, parity, which, dst, src, nullptr, s)
-                               : mpc_impl<float>(op, parity, which, dst, src, nullptr, s);
+  return op->prec == WC_DOUBLE ? mpc_impl<double>(op, nrhs, parity, which, dst, src, nullptr, s)
+                               : mpc_impl<float>(op, nrhs, parity, which, dst, src, nullptr, s);
 }
 
-int wc_cg_eo2_ne(wc_operator *op, int parity, void *x, const void *b, double eps, long maxiter,
-                 double *history, wc_solver_stats *stats, void *stream) {
+int wc_apply_mpc(wc_operator *op, int parity, int which, void *dst, const void *src, void *stream) {
+  return wc_apply_mpc_block(op, parity, which, 1, dst, src, stream);
+}
+
+int wc_cg_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *x, const void *b, double eps, long maxiter,
+                       double *history, long *iterations, int *converged, double *r2, double *b2,
+                       void *stream) {
   if (!op || !x || !b) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: null argument");
   if (x == b) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: x must not alias b");
   if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: parity must be 0 or 1");
   if (maxiter < 0) WC_FAIL(WC_ERR_ARG, "wc_cg_eo2_ne: maxiter must be >= 0");
+  int rc = check_nrhs(op, nrhs, "wc_cg_eo2_ne");
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  return op->prec == WC_DOUBLE ? cg_impl<double>(op, parity, x, b, eps, maxiter, history, stats, s)
-                               : cg_impl<float>(op, parity, x, b, eps, maxiter, history, stats, s);
+  return op->prec == WC_DOUBLE
+             ? cg_impl<double>(op, nrhs, parity, x, b, eps, maxiter, history, iterations, converged, r2, b2, s)
+             : cg_impl<float>(op, nrhs, parity, x, b, eps, maxiter, history, iterations, converged, r2, b2, s);
 }
 
-int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,
-                    double *history, wc_solver_stats *stats, void *stream) {
+int wc_cg_eo2_ne(wc_operator *op, int parity, void *x, const void *b, double eps, long maxiter,
+                 double *history, wc_solver_stats *stats, void *stream) {
+  long it = 0;
+  int cv = 0;
+  double r2 = 0, b2 = 0;
+  int rc = wc_cg_eo2_ne_block(op, parity, 1, x, b, eps, maxiter, history, &it, &cv, &r2, &b2, stream);
+  if (stats) {
+    stats->iterations = it;
+    stats->converged = cv;
+    stats->r2 = r2;
+    stats->b2 = b2;
+  }
+  return rc;
+}
+
+int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, const void *src, double eps,
+                          long maxiter, double *history, long *iterations, int *converged, double *r2,
+                          double *b2, void *stream) {
   if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: null argument");
   if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: dst must not alias src");
   if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: parity must be 0 or 1");
   if (maxiter < 0) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne: maxiter must be >= 0");
+  int rc = check_nrhs(op, nrhs, "wc_solve_eo2_ne");
+  if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
-  return op->prec == WC_DOUBLE ? solve_impl<double>(op, parity, dst, src, eps, maxiter, history, stats, s)
-                               : solve_impl<float>(op, parity, dst, src, eps, maxiter, history, stats, s);
+  return op->prec == WC_DOUBLE ? solve_impl<double>(op, nrhs, parity, dst, src, eps, maxiter, history, iterations,
+                                                    converged, r2, b2, s)
+                               : solve_impl<float>(op, nrhs, parity, dst, src, eps, maxiter, history, iterations,
+                                                   converged, r2, b2, s);
 }
 
+int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,
+                    double *history, wc_solver_stats *stats, void *stream) {
+  long it = 0;
+  int cv = 0;
+  double r2 = 0, b2 = 0;
+  int rc = wc_solve_eo2_ne_block(op, parity, 1, dst, src, eps, maxiter, history, &it, &cv, &r2, &b2, stream);
+  if (stats) {
+    stats->iterations = it;
+    stats->converged = cv;
+    stats->r2 = r2;
+    stats->b2 = b2;
+  }
+  return rc;
+}
+
+}  // extern "C"
+
 // ------------------------------------------------------------------ BLAS
+extern "C" {
+
 static thread_local double *g_red = nullptr;      // device partials + result
 static thread_local double *g_red_host = nullptr;  // pinned host copy
 

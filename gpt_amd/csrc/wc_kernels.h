@@ -17,6 +17,7 @@ enum : int {
 
 struct HopLaunch {
   Geom g;
+  int nrhs;          // right-hand sides per site (innermost index of every field)
   int pout;          // output parity
   int dag;           // adjoint hop (gamma_mu -> -gamma_mu)
   int nl;            // 9 or 6 complex per link
@@ -47,11 +48,21 @@ constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
 
 template <typename R> void hop(const HopLaunch &a, hipStream_t s);
 
-// out = A in (blocks != null) or diag * in on parity p; zeroes the open
-// boundary; optional axpy (out = y - A in)
+// out = A in (blocks != null) or diag * in on parity p, for nrhs right-hand
+// sides; zeroes the open boundary; optional axpy (out = y - A in)
 template <typename R>
-void site_clover(const Geom &g, int p, const void *blocks, double diag, const void *in, void *out,
+void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
                  const void *y, int open_bc, hipStream_t s);
+
+// Per-right-hand-side reductions over fields whose innermost index is the
+// RHS (nrhs of them): partials[b * nrhs + r] for b < kRedBlocks, then
+// out[r] = sum over b in fixed order.
+template <typename R>
+void dot_partials_rhs(long n, int nrhs, const void *a, const void *b, double *pre, double *pim, hipStream_t s);
+void reduce_final_rhs(const double *partials, int nblocks, int nrhs, double *out, hipStream_t s);
+int rhs_block_threads(int nrhs);
+// x[e] *= f[e % nrhs] for n entries (per-RHS real factors on the device)
+template <typename R> void scale_rhs(long n, int nrhs, const double *f, void *x, hipStream_t s);
 
 // BLAS-1 on n complex entries (C2 of R)
 template <typename R>
@@ -93,13 +104,14 @@ template <typename R>
 void build_clover(const Geom &g, const void *Unat, const CloverParams &cp, void *A0, void *A1,
                   void *Ai0, void *Ai1, int *err_flag, hipStream_t s);
 
-// CG helpers (device scalars): st = {c, d, alpha, beta, cp}
-void cg_alpha(const double *partials, int n, double *st, hipStream_t s);
-void cg_beta(const double *partials, int n, double *st, hipStream_t s);
+// CG on nrhs independent right-hand sides with device scalars:
+// st[j * nrhs + r], j = 0 c, 1 d, 2 alpha, 3 beta, 4 cp, 5 active (1/0)
+void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s);
+void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s);
 template <typename R>
-void cg_update1(long n, const double *st, void *r, const void *mmp, void *x, const void *p,
+void cg_update1(long n, int nrhs, const double *st, void *r, const void *mmp, void *x, const void *p,
                 double *partials, hipStream_t s);
 template <typename R>
-void cg_update2(long n, const double *st, const void *r, void *p, hipStream_t s);
+void cg_update2(long n, int nrhs, const double *st, const void *r, void *p, hipStream_t s);
 
 }  // namespace wck

@@ -48,28 +48,29 @@ __device__ __forceinline__ void clov_apply(const C *A, long Vh, long h, const C 
   }
 }
 
+// spinor of site h, right-hand side r of N: entry k at aidx(k, h, 12) * N + r
 template <typename C>
-__device__ __forceinline__ void load12(const C *f, long Vh, long h, C v[12]) {
-  const C *p = f + aidx_site(h, 12);
+__device__ __forceinline__ void load12(const C *f, long h, int r, int N, C v[12]) {
+  const C *p = f + aidx_site(h, 12) * N + r;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = p[(long)k * kBlk];
+  for (int k = 0; k < 12; ++k) v[k] = p[(long)k * kBlk * N];
 }
 template <typename C>
-__device__ __forceinline__ void store12(C *f, long Vh, long h, const C v[12]) {
-  C *p = f + aidx_site(h, 12);
+__device__ __forceinline__ void store12(C *f, long h, int r, int N, const C v[12]) {
+  C *p = f + aidx_site(h, 12) * N + r;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) p[(long)k * kBlk] = v[k];
+  for (int k = 0; k < 12; ++k) p[(long)k * kBlk * N] = v[k];
 }
 // output of a hop: written once, not re-read by this kernel
 template <typename C>
-__device__ __forceinline__ void store12_stream(C *f, long Vh, long h, const C v[12]) {
+__device__ __forceinline__ void store12_stream(C *f, long Vh, long h, int r, int N, const C v[12]) {
 #if WC_NT_STORE
-  const wc_rsrc r = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)sizeof(C)));
-  const uint32_t vo = (uint32_t)aidx_site(h, 12) * (uint32_t)sizeof(C);
+  const wc_rsrc rs = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)N * (uint32_t)sizeof(C)));
+  const uint32_t vo = (uint32_t)(aidx_site(h, 12) * N + r) * (uint32_t)sizeof(C);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) bstore_t<C, 2>::go(v[k], r, vo, (uint32_t)(k * kBlk * sizeof(C)));
+  for (int k = 0; k < 12; ++k) bstore_t<C, 2>::go(v[k], rs, vo, (uint32_t)(k * kBlk * N * sizeof(C)));
 #else
-  store12(f, Vh, h, v);
+  store12(f, h, r, N, v);
 #endif
 }
 
@@ -182,18 +183,19 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4],
 // h = ((t*Lz + z)*Ly + y)*Lxh + x/2.
 template <typename R, int NL, int DAG>
 __device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R>::t *inp,
-                                         const typename c2<R>::t *W, long h, const int c[4],
+                                         const typename c2<R>::t *W, long h, int r, int N, const int c[4],
                                          typename c2<R>::t acc[12]) {
   using C = typename c2<R>::t;
   const Geom &g = a.g;
   const uint32_t Vh = (uint32_t)g.Vh;
   const uint32_t hh = (uint32_t)h;
-  const wc_rsrc in = make_rsrc(inp, 12u * Vh * (uint32_t)sizeof(C));
-  // a wave covers one aligned 64-site block: its links are one contiguous
-  // 8*NL*64-entry run, addressed from a wave-uniform base
+  const wc_rsrc in = make_rsrc(inp, 12u * Vh * (uint32_t)N * (uint32_t)sizeof(C));
+  // a wave's sites lie in at most two aligned 64-site blocks (one when N = 1):
+  // their links are addressed from the wave's first block, a wave-uniform base
   const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
-  const wc_rsrc rw = make_rsrc(W + blk * (8 * NL * kBlk), 8u * NL * kBlk * (uint32_t)sizeof(C));
-  const uint32_t vw = (uint32_t)(h & 63) * (uint32_t)sizeof(C);
+  const wc_rsrc rw = make_rsrc(W + blk * (8 * NL * kBlk), 2u * 8u * NL * kBlk * (uint32_t)sizeof(C));
+  const uint32_t vw = (uint32_t)(aidx_site(h, 8 * NL) - blk * (8 * NL * kBlk)) * (uint32_t)sizeof(C);
+  const uint32_t kstride = (uint32_t)(kBlk * N * sizeof(C));
   const uint32_t xh = (uint32_t)c[0] >> 1;
   const uint32_t st1 = (uint32_t)g.Lxh, st2 = st1 * (uint32_t)g.L[1], st3 = st2 * (uint32_t)g.L[2];
 #pragma unroll 1
@@ -215,9 +217,9 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R
         nb = (cm == 0) ? hh + wrap : hh - stride;
     }
     DirData<C> d;
-    const uint32_t vs = (uint32_t)aidx_site(nb, 12) * (uint32_t)sizeof(C);
+    const uint32_t vs = ((uint32_t)aidx_site(nb, 12) * (uint32_t)N + (uint32_t)r) * (uint32_t)sizeof(C);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) d.psi[k] = bload<C, WC_SPINOR_AUX>(in, vs, (uint32_t)(k * kBlk * sizeof(C)));
+    for (int k = 0; k < 12; ++k) d.psi[k] = bload<C, WC_SPINOR_AUX>(in, vs, (uint32_t)k * kstride);
     const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
 #pragma unroll
     for (int e = 0; e < NL; ++e)
@@ -238,10 +240,16 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R
 template <typename R, int NL, int DAG, int EPI>
 __global__ __launch_bounds__(kSiteThreads, WC_HOP_MINWAVES) void k_hop(const HopLaunch a) {
   using C = typename c2<R>::t;
-  __shared__ double red[kSiteThreads / 64 + 1];
+  __shared__ double red[kSiteThreads];
   const Geom &g = a.g;
   const long Vh = g.Vh;
-  const long h = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
+  const int N = a.nrhs;
+  // thread t -> (site h, right-hand side r); the N threads of a site are
+  // adjacent, so they share each link load
+  const long t0 = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads;
+  const long t = t0 + threadIdx.x;
+  const long h = (N == 1) ? t : t / N;
+  const int r = (int)(t - h * N);
   double dot = 0.0;
   if (h < Vh) {
     int c[4];
@@ -251,7 +259,7 @@ __global__ __launch_bounds__(kSiteThreads, WC_HOP_MINWAVES) void k_hop(const Hop
     C acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = cmk<C>(0, 0);
-    hop_site<R, NL, DAG>(a, in, W, h, c, acc);
+    hop_site<R, NL, DAG>(a, in, W, h, r, N, c, acc);
     // epilogue
     const bool zb = a.open_bc && (c[3] == 0 || c[3] == g.L[3] - 1);
     C res[12];
@@ -271,7 +279,7 @@ __global__ __launch_bounds__(kSiteThreads, WC_HOP_MINWAVES) void k_hop(const Hop
     }
     if (EPI & (EPI_DIAG | EPI_AXPY)) {
       C y[12];
-      load12((const C *)a.y, Vh, h, y);
+      load12((const C *)a.y, h, r, N, y);
       if (EPI & EPI_DIAG) {
         if (!zb) {
           C dy[12];
@@ -291,21 +299,33 @@ __global__ __launch_bounds__(kSiteThreads, WC_HOP_MINWAVES) void k_hop(const Hop
     }
     if (EPI & EPI_DOT) {
       C v[12];
-      load12((const C *)a.dotv, Vh, h, v);
+      load12((const C *)a.dotv, h, r, N, v);
 #pragma unroll
       for (int k = 0; k < 12; ++k) dot += (double)v[k].x * (double)res[k].x + (double)v[k].y * (double)res[k].y;
     }
-    store12_stream((C *)a.out, Vh, h, res);
+    store12_stream((C *)a.out, Vh, h, r, N, res);
   }
   if (EPI & EPI_DOT) {
-    double s = block_sum<kSiteThreads>(dot, red);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+    if (N == 1) {
+      double s = block_sum<kSiteThreads>(dot, red);
+      if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+    } else {
+      // per-RHS sums of the block in fixed lane order: partial[block * N + r]
+      red[threadIdx.x] = dot;
+      __syncthreads();
+      if (threadIdx.x < N) {
+        double s = 0.0;
+        const int r0 = (int)(t0 % N);
+        for (int i = ((int)threadIdx.x - r0 + N) % N; i < kSiteThreads; i += N) s += red[i];
+        a.partial[(long)blockIdx.x * N + threadIdx.x] = s;
+      }
+    }
   }
 }
 
 template <typename R, int NL, int DAG>
 static void hop_epi(const HopLaunch &a, hipStream_t s) {
-  const int nb = site_blocks(a.g.Vh);
+  const int nb = site_blocks(a.g.Vh * a.nrhs);
   switch (a.epi) {
 #define WC_EPI(E) \
   case E: k_hop<R, NL, DAG, E><<<nb, kSiteThreads, 0, s>>>(a); break;
@@ -333,15 +353,17 @@ template void hop<float>(const HopLaunch &, hipStream_t);
 
 // ------------------------------------------------------- site-local clover
 template <typename R>
-__global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int p, const typename c2<R>::t *A,
+__global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int p, const typename c2<R>::t *A,
                                                              double diag, const typename c2<R>::t *in,
                                                              typename c2<R>::t *out,
                                                              const typename c2<R>::t *y, int open_bc) {
   using C = typename c2<R>::t;
-  const long h = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
+  const long t = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
+  const long h = (N == 1) ? t : t / N;
+  const int r = (int)(t - h * N);
   if (h >= g.Vh) return;
   C v[12], o[12];
-  load12(in, g.Vh, h, v);
+  load12(in, h, r, N, v);
   bool zb = false;
   if (open_bc) {
     int c[4];
@@ -359,23 +381,24 @@ __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int p, con
   }
   if (y) {
     C yy[12];
-    load12(y, g.Vh, h, yy);
+    load12(y, h, r, N, yy);
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = csub(yy[k], o[k]);
   }
-  store12(out, g.Vh, h, o);
+  store12(out, h, r, N, o);
 }
 
 template <typename R>
-void site_clover(const Geom &g, int p, const void *blocks, double diag, const void *in, void *out,
+void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
                  const void *y, int open_bc, hipStream_t s) {
   using C = typename c2<R>::t;
-  k_site_clover<R><<<site_blocks(g.Vh), kSiteThreads, 0, s>>>(g, p, (const C *)blocks, diag, (const C *)in,
-                                                              (C *)out, (const C *)y, open_bc);
+  k_site_clover<R><<<site_blocks(g.Vh * nrhs), kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag,
+                                                                     (const C *)in, (C *)out, (const C *)y,
+                                                                     open_bc);
 }
-template void site_clover<double>(const Geom &, int, const void *, double, const void *, void *,
+template void site_clover<double>(const Geom &, int, int, const void *, double, const void *, void *,
                                   const void *, int, hipStream_t);
-template void site_clover<float>(const Geom &, int, const void *, double, const void *, void *,
+template void site_clover<float>(const Geom &, int, int, const void *, double, const void *, void *,
                                  const void *, int, hipStream_t);
 
 // ------------------------------------------------------------------ BLAS-1
@@ -813,46 +836,122 @@ template void build_clover<double>(const Geom &, const void *, const CloverParam
 template void build_clover<float>(const Geom &, const void *, const CloverParams &, void *, void *, void *,
                                   void *, int *, hipStream_t);
 
-// ------------------------------------------------------------ CG scalars
-// st[0] = c, st[1] = d, st[2] = alpha, st[3] = beta, st[4] = cp
-__global__ __launch_bounds__(1024) void k_cg_alpha(const double *p, int n, double *st) {
-  __shared__ double red[1024 / 64 + 1];
+// ------------------------------------------------- per-RHS reductions
+// block size: the largest multiple of nrhs not above 256, so that a thread's
+// elements (strided by a multiple of nrhs) all belong to one RHS
+int rhs_block_threads(int nrhs) { return (256 / nrhs) * nrhs; }
+
+// sum of lds[0..nt) over the entries of RHS threadIdx.x (< nrhs), fixed order
+__device__ __forceinline__ double seg_sum(const double *lds, int nt, int nrhs) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) s += p[i];
-  s = block_sum<1024>(s, red);
-  if (threadIdx.x == 0) {
-    st[1] = s;
-    st[2] = st[0] / s;
-  }
-}
-__global__ __launch_bounds__(1024) void k_cg_beta(const double *p, int n, double *st) {
-  __shared__ double red[1024 / 64 + 1];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += 1024) s += p[i];
-  s = block_sum<1024>(s, red);
-  if (threadIdx.x == 0) {
-    st[4] = s;
-    st[3] = s / st[0];
-    st[0] = s;
-  }
-}
-void cg_alpha(const double *partials, int n, double *st, hipStream_t s) {
-  k_cg_alpha<<<1, 1024, 0, s>>>(partials, n, st);
-}
-void cg_beta(const double *partials, int n, double *st, hipStream_t s) {
-  k_cg_beta<<<1, 1024, 0, s>>>(partials, n, st);
+  for (int i = threadIdx.x; i < nt; i += nrhs) s += lds[i];
+  return s;
 }
 
-// r -= alpha mmp; x += alpha p; partial |r|^2
 template <typename R>
-__global__ __launch_bounds__(256) void k_cg_update1(long n, const double *st, typename c2<R>::t *r,
+__global__ __launch_bounds__(256) void k_dot_rhs(long n, int N, const typename c2<R>::t *a,
+                                                 const typename c2<R>::t *b, double *pre, double *pim) {
+  __shared__ double lr[256], li[256];
+  const long stride = (long)gridDim.x * blockDim.x;
+  double sr = 0.0, si = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double ax = a[i].x, ay = a[i].y, bx = b[i].x, by = b[i].y;
+    sr += ax * bx + ay * by;
+    si += ax * by - ay * bx;
+  }
+  lr[threadIdx.x] = sr;
+  li[threadIdx.x] = si;
+  __syncthreads();
+  if ((int)threadIdx.x < N) {
+    pre[(long)blockIdx.x * N + threadIdx.x] = seg_sum(lr, blockDim.x, N);
+    if (pim) pim[(long)blockIdx.x * N + threadIdx.x] = seg_sum(li, blockDim.x, N);
+  }
+}
+template <typename R>
+void dot_partials_rhs(long n, int nrhs, const void *a, const void *b, double *pre, double *pim, hipStream_t s) {
+  using C = typename c2<R>::t;
+  k_dot_rhs<R><<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, (const C *)a, (const C *)b, pre, pim);
+}
+template void dot_partials_rhs<double>(long, int, const void *, const void *, double *, double *, hipStream_t);
+template void dot_partials_rhs<float>(long, int, const void *, const void *, double *, double *, hipStream_t);
+
+// out[r] = sum_b partials[b * N + r]; one block per RHS, fixed order
+__global__ __launch_bounds__(256) void k_reduce_final_rhs(const double *p, int nb, int N, double *out) {
+  __shared__ double red[256 / 64 + 1];
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += p[(long)b * N + r];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) out[r] = s;
+}
+void reduce_final_rhs(const double *partials, int nblocks, int nrhs, double *out, hipStream_t s) {
+  k_reduce_final_rhs<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, out);
+}
+
+template <typename R>
+__global__ void k_scale_rhs(long n, int N, const double *f, typename c2<R>::t *x) {
+  using C = typename c2<R>::t;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const R a = (R)f[i % N];
+  C v = x[i];
+  x[i] = cmk<C>(a * v.x, a * v.y);
+}
+template <typename R> void scale_rhs(long n, int nrhs, const double *f, void *x, hipStream_t s) {
+  using C = typename c2<R>::t;
+  if (n > 0) k_scale_rhs<R><<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (C *)x);
+}
+template void scale_rhs<double>(long, int, const double *, void *, hipStream_t);
+template void scale_rhs<float>(long, int, const double *, void *, hipStream_t);
+
+// ------------------------------------------------------------ CG scalars
+// st[j * N + r]: j = 0 c, 1 d, 2 alpha, 3 beta, 4 cp, 5 active
+__global__ __launch_bounds__(256) void k_cg_alpha(const double *p, int nb, int N, double *st) {
+  __shared__ double red[256 / 64 + 1];
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += p[(long)b * N + r];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) {
+    st[1 * N + r] = s;
+    st[2 * N + r] = (st[5 * N + r] != 0.0 && s != 0.0) ? st[0 * N + r] / s : 0.0;
+  }
+}
+__global__ __launch_bounds__(256) void k_cg_beta(const double *p, int nb, int N, double *st) {
+  __shared__ double red[256 / 64 + 1];
+  const int r = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += p[(long)b * N + r];
+  s = block_sum<256>(s, red);
+  if (threadIdx.x == 0) {
+    st[4 * N + r] = s;
+    if (st[5 * N + r] != 0.0) {
+      st[3 * N + r] = s / st[0 * N + r];
+      st[0 * N + r] = s;
+    } else {
+      st[3 * N + r] = 0.0;
+    }
+  }
+}
+void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s) {
+  k_cg_alpha<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st);
+}
+void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s) {
+  k_cg_beta<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st);
+}
+
+// r -= alpha_r mmp; x += alpha_r p; partial |r|^2 per RHS
+template <typename R>
+__global__ __launch_bounds__(256) void k_cg_update1(long n, int N, const double *st, typename c2<R>::t *r,
                                                     const typename c2<R>::t *mmp, typename c2<R>::t *x,
                                                     const typename c2<R>::t *p, double *pre) {
   using C = typename c2<R>::t;
-  __shared__ double red[256 / 64 + 1];
-  const R a = (R)st[2];
+  __shared__ double lds[256];
+  const long stride = (long)gridDim.x * blockDim.x;
+  const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const R a = (R)st[2 * N + (int)(i0 % N)];
   double s = 0.0;
-  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+  for (long i = i0; i < n; i += stride) {
     C rv = r[i], mv = mmp[i], xv = x[i], pv = p[i];
     rv = cmk<C>(rv.x - a * mv.x, rv.y - a * mv.y);
     xv = cmk<C>(xv.x + a * pv.x, xv.y + a * pv.y);
@@ -860,35 +959,38 @@ __global__ __launch_bounds__(256) void k_cg_update1(long n, const double *st, ty
     x[i] = xv;
     s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
   }
-  s = block_sum<256>(s, red);
-  if (threadIdx.x == 0) pre[blockIdx.x] = s;
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  if ((int)threadIdx.x < N) pre[(long)blockIdx.x * N + threadIdx.x] = seg_sum(lds, blockDim.x, N);
 }
 template <typename R>
-void cg_update1(long n, const double *st, void *r, const void *mmp, void *x, const void *p, double *partials,
-                hipStream_t s) {
+void cg_update1(long n, int nrhs, const double *st, void *r, const void *mmp, void *x, const void *p,
+                double *partials, hipStream_t s) {
   using C = typename c2<R>::t;
-  k_cg_update1<R><<<kRedBlocks, 256, 0, s>>>(n, st, (C *)r, (const C *)mmp, (C *)x, (const C *)p, partials);
+  k_cg_update1<R><<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, st, (C *)r, (const C *)mmp, (C *)x,
+                                                                 (const C *)p, partials);
 }
-template void cg_update1<double>(long, const double *, void *, const void *, void *, const void *, double *,
-                                 hipStream_t);
-template void cg_update1<float>(long, const double *, void *, const void *, void *, const void *, double *,
-                                hipStream_t);
+template void cg_update1<double>(long, int, const double *, void *, const void *, void *, const void *,
+                                 double *, hipStream_t);
+template void cg_update1<float>(long, int, const double *, void *, const void *, void *, const void *,
+                                double *, hipStream_t);
 
-// p = beta p + r
+// p = beta_r p + r
 template <typename R>
-__global__ void k_cg_update2(long n, const double *st, const typename c2<R>::t *r, typename c2<R>::t *p) {
+__global__ void k_cg_update2(long n, int N, const double *st, const typename c2<R>::t *r,
+                             typename c2<R>::t *p) {
   using C = typename c2<R>::t;
   long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  const R b = (R)st[3];
+  const R b = (R)st[3 * N + (int)(i % N)];
   C pv = p[i], rv = r[i];
   p[i] = cmk<C>(b * pv.x + rv.x, b * pv.y + rv.y);
 }
-template <typename R> void cg_update2(long n, const double *st, const void *r, void *p, hipStream_t s) {
+template <typename R> void cg_update2(long n, int nrhs, const double *st, const void *r, void *p, hipStream_t s) {
   using C = typename c2<R>::t;
-  k_cg_update2<R><<<elem_blocks(n), 256, 0, s>>>(n, st, (const C *)r, (C *)p);
+  k_cg_update2<R><<<elem_blocks(n), 256, 0, s>>>(n, nrhs, st, (const C *)r, (C *)p);
 }
-template void cg_update2<double>(long, const double *, const void *, void *, hipStream_t);
-template void cg_update2<float>(long, const double *, const void *, void *, hipStream_t);
+template void cg_update2<double>(long, int, const double *, const void *, void *, hipStream_t);
+template void cg_update2<float>(long, int, const double *, const void *, void *, hipStream_t);
 
 }  // namespace wck

@@ -102,8 +102,11 @@ class interface:
         assert self.obj is not None
         if dst is src:
             raise ValueError("dst must differ from src")
+        if dst.grid.nrhs != src.grid.nrhs:
+            raise ValueError("dst and src hold different numbers of right-hand sides")
         cb = -1 if opcode in FULL_FIELD else src.checkerboard().tag
-        _lib.check(_lib.load().wc_apply(self.obj, opcode, dst.ptr(), src.ptr(), cb, core.stream_ptr()))
+        _lib.check(_lib.load().wc_apply_block(self.obj, opcode, src.grid.nrhs, dst.ptr(), src.ptr(), cb,
+                                              core.stream_ptr()))
 
 
 class fine_operator(matrix_operator):
@@ -117,17 +120,21 @@ class fine_operator(matrix_operator):
         prec = precision if precision is not None else params.get("precision", core.double)
         self.precision = prec
         fd = U[0].grid.fdimensions
+        self.n_rhs = int(params.get("n_rhs", 1))
         self.U_grid = core.grid(fd, prec, full)
-        self.F_grid = self.U_grid
         self.U_grid_eo = self.U_grid.checkerboarded(redblack)
-        self.F_grid_eo = self.U_grid_eo
-        self.params = dict(params)
+        # n_rhs > 1: fields carry n_rhs right-hand sides (wilson.py:82-86, base.py:47-60)
+        self.F_grid = self.U_grid.inserted_dimension(self.n_rhs)
+        self.F_grid_eo = self.F_grid.checkerboarded(redblack)
+        self.params = {k: v for k, v in params.items() if k != "n_rhs"}
         self.interface = interface()
         self.interface.setup(self.U_grid, self.params, U)
         iface = self.interface
 
         def reg(opcode, out_cb=None):
             def f(dst, src):
+                if isinstance(src, list):
+                    return packed_apply(f, dst, src)
                 if src.grid.precision is not prec or dst.grid.precision is not prec:
                     raise ValueError(f"field precision differs from the operator's ({prec.__name__})")
                 iface.apply_unary_operator(opcode, dst, src)
@@ -169,7 +176,8 @@ class fine_operator(matrix_operator):
         def _G5M(dst, src):
             # gamma5 = diag(1, 1, -1, -1): negate the lower spins (components 6..11)
             self.mat(dst, src)
-            dst.data[..., 6:12, :, :].neg_()
+            d = dst.data
+            d.view(-1, 12, d.shape[-3] * d.shape[-2] * 2)[:, 6:12].neg_()
 
         self.G5M = matrix_operator(_G5M, make_dst=make_full)
 
@@ -207,6 +215,16 @@ class fine_operator(matrix_operator):
 
     def arguments(self):
         return self.U
+
+    def packed(self):
+        """the operator on lists of n_rhs single-RHS fields, applied as one
+        n_rhs field (lib/gpt/core/operator/matrix_operator.py:200-239)"""
+        op = self
+
+        def mat(dst, src):
+            packed_apply(op.mat, dst, src)
+
+        return matrix_operator(mat, make_dst=lambda s: lattice(s), accept_list=True)
 
     def propagator(self, solver):
         """exp * inv * imp (operator/base.py:270-286); Import/Export are the identity
@@ -260,12 +278,42 @@ def wilson_clover(U, params=None, **kwargs):
     for k, dflt in [("csw_r", 0.0), ("csw_t", 0.0), ("nu", 1.0), ("xi_0", 1.0)]:
         if p[k] is None:
             p[k] = dflt
-    if p["n_rhs"] != 1:
-        raise NotImplementedError("n_rhs > 1 (packed multi-RHS operator) is not available yet")
+    if not 1 <= int(p["n_rhs"]) <= 64:
+        raise ValueError("n_rhs must be in [1, 64]")
     if not isinstance(U[0], gauge.link_field):
         fd = list(reversed(U[0].shape[:4]))
         U = gauge.from_numpy(core.grid(fd, core.double), U)
     return fine_operator("wilson_clover", U, p)
+
+
+def pack(fields, g=None):
+    """list of single-RHS fields -> one multi-RHS field (RHS innermost)"""
+    n = len(fields)
+    g = (g or fields[0].grid).inserted_dimension(n)
+    out = lattice(g)
+    out._cb = fields[0].checkerboard()
+    for r, f in enumerate(fields):
+        out.set_rhs(r, f)
+    return out
+
+
+def unpack(field, dst=None):
+    n = field.grid.nrhs
+    if dst is None:
+        return [field.rhs(r) for r in range(n)]
+    for r in range(n):
+        dst[r].data.copy_(field.data[..., r:r + 1, :])
+        if dst[r].grid.cb is not full:
+            dst[r].checkerboard(field.checkerboard())
+    return dst
+
+
+def packed_apply(mat, dst, src):
+    P = pack(src)
+    Q = lattice(P)
+    mat(Q, P)
+    unpack(Q, dst)
+    return dst
 
 
 def apply_open_boundaries(field):

@@ -36,8 +36,8 @@ class _eo2_ne:
             def f(dst, src):
                 if src.checkerboard() is not self.D:
                     raise ValueError("Mpc acts on fields of the D parity")
-                _lib.check(_lib.load().wc_apply_mpc(op.handle, self.D.tag, which, dst.ptr(), src.ptr(),
-                                                    core.stream_ptr()))
+                _lib.check(_lib.load().wc_apply_mpc_block(op.handle, self.D.tag, which, src.grid.nrhs,
+                                                          dst.ptr(), src.ptr(), core.stream_ptr()))
                 dst.checkerboard(self.D)
             return f
 

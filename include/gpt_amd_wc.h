@@ -140,6 +140,27 @@ int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, dou
  * normal_equation.py:44-48): which = 0 Mpc, 1 Mpc^dag, 2 Mpc^dag Mpc */
 int wc_apply_mpc(wc_operator *op, int parity, int which, void *dst, const void *src, void *stream);
 
+/* --- multiple right-hand sides ------------------------------------------
+ * gpt's n_rhs > 1 operator (lib/gpt/qcd/fermion/wilson.py:82-86, a 5-d grid
+ * with the RHS as the fastest index, operator/base.py:47-60) applies one
+ * operator to nrhs fields at once, reusing every link and clover block for
+ * all of them.  A field of nrhs right-hand sides keeps the half-field layout
+ * with the RHS innermost: element (k, h, r) at
+ * (((h/64)*12 + k)*64 + h%64)*nrhs + r.  1 <= nrhs <= 64.
+ * The CG runs nrhs independent recursions of cg.py (each its own scalars
+ * and stopping test, so iteration counts equal the single-RHS ones); arrays
+ * iterations / converged / r2 / b2 hold nrhs entries (each may be NULL),
+ * history maxiter * nrhs doubles (may be NULL). */
+int wc_apply_block(wc_operator *op, int opcode, int nrhs, void *dst, const void *src, int src_cb, void *stream);
+int wc_apply_mpc_block(wc_operator *op, int parity, int which, int nrhs, void *dst, const void *src,
+                       void *stream);
+int wc_cg_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *x, const void *b, double eps, long maxiter,
+                       double *history, long *iterations, int *converged, double *r2, double *b2,
+                       void *stream);
+int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, const void *src, double eps,
+                          long maxiter, double *history, long *iterations, int *converged, double *r2,
+                          double *b2, void *stream);
+
 /* message of the last failed call on this thread */
 const char *wc_last_error(void);
 

@@ -1,0 +1,102 @@
+"""Multiple right-hand sides (gpt's n_rhs operator, lib/gpt/qcd/fermion/wilson.py:82-86)
+against the single-RHS path: the packed operator equals the single-RHS one per
+RHS (tests/qcd/fermion_operators.py:97-132, eps^2 < 1e-15), and the batched
+CG gives each RHS the iteration count and solution of its own solve."""
+import numpy as np
+import pytest
+
+from oracle import gauge as G
+from oracle import rng as R
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+import gpt_amd as g  # noqa: E402
+
+inv = g.algorithms.inverter
+pc = g.qcd.fermion.preconditioner
+P = dict(mass=0.123, csw_r=0.6, csw_t=0.6, cF=1.0, xi_0=1.0, nu=1.0, isAnisotropic=False,
+         boundary_phases=[1, 1, 1, -1])
+
+
+@pytest.fixture(scope="module")
+def setup():
+    gr = R.grid([8, 8, 8, 16])
+    rng = R.random("multi_rhs")
+    U = G.random_links(gr, rng, scale=0.5)
+    srcs = [rng.vspincolor(gr) for _ in range(5)]
+    return gr, U, srcs
+
+
+def _fields(arrs, grid):
+    out = []
+    for a in arrs:
+        x = g.lattice(grid)
+        x.from_numpy(a)
+        out.append(x)
+    return out
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_packed_operator_equals_single_rhs(setup, prec):
+    gr, U, srcs = setup
+    pr = g.double if prec == "double" else g.single
+    w1 = g.qcd.fermion.wilson_clover(U, P, precision=pr)
+    w4 = g.qcd.fermion.wilson_clover(U, P, precision=pr, n_rhs=4)
+    assert w4.F_grid.nrhs == 4
+    test = _fields([s.astype(pr.complex_dtype) for s in srcs[:4]], w1.F_grid)
+    test0 = [w1(t) for t in test]
+    test1 = w4.packed()(test)
+    tol = 1e-15 if prec == "double" else 1e-12
+    for i in range(4):
+        eps2 = g.norm2(g(test0[i] - test1[i])) / g.norm2(test0[i])
+        assert eps2 < tol, (i, eps2)
+    # the even-odd pieces on an n_rhs field directly
+    from gpt_amd.qcd.fermion.operator import pack, unpack
+
+    half = [g.pick_checkerboard(g.odd, t) for t in test]
+    H = pack(half)
+    for name in ["Meooe", "Mooee"]:
+        op1, op4 = getattr(w1, name), getattr(w4, name)
+        one = [op1(h) for h in half]
+        many = unpack(op4(H))
+        for i in range(4):
+            d = np.abs(one[i].to_numpy() - many[i].to_numpy()).max()
+            assert d <= (1e-14 if prec == "double" else 1e-5) * np.abs(one[i].to_numpy()).max(), (name, i)
+    inv_one = [w1.Mooee.inv()(h) for h in half]
+    inv_many = unpack(w4.Mooee.inv()(H))
+    for i in range(4):
+        assert np.abs(inv_one[i].to_numpy() - inv_many[i].to_numpy()).max() <= 1e-14 * np.abs(
+            inv_one[i].to_numpy()).max() + (0 if prec == "double" else 1e-6)
+
+
+def test_batched_cg_matches_single_solves(setup):
+    gr, U, srcs = setup
+    N = 5
+    w1 = g.qcd.fermion.wilson_clover(U, P)
+    wN = g.qcd.fermion.wilson_clover(U, P, n_rhs=N)
+    arrs = [s.copy() for s in srcs]
+    arrs[2][:] = 0.0  # a zero source inside the batch: its solution is zero, as gpt's cg gives
+    single = _fields(arrs, w1.F_grid)
+    cg1 = inv.cg(eps=1e-10, maxiter=2000)
+    slv1 = w1.propagator(inv.preconditioned(pc.eo2_ne(), cg1))
+    x1, it1 = [], []
+    for s in single:
+        x1.append(g(slv1 * s))
+        it1.append(len(cg1.history))
+    from gpt_amd.qcd.fermion.operator import pack, unpack
+
+    cgN = inv.cg(eps=1e-10, maxiter=2000)
+    slvN = wN.propagator(inv.preconditioned(pc.eo2_ne(), cgN))
+    XN = unpack(g(slvN * pack(single)))
+    assert cgN.converged
+    for i in range(N):
+        assert abs(cgN.iterations[i] - it1[i]) <= 1, (i, cgN.iterations, it1)
+        a, b = XN[i].to_numpy(), x1[i].to_numpy()
+        if i == 2:
+            assert np.abs(a).max() == 0.0 and cgN.iterations[i] == 0
+        else:
+            assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-10

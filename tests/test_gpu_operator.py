@@ -209,18 +209,23 @@ def test_errors_raise():
 
 
 def test_operator_pool_reuse_is_exact(small):
+    import gc
+
+    from gpt_amd.qcd.fermion import operator as O
+
     gr, U, src = small
     p = dict(CASES["baseline_clover"])
     w1 = g.qcd.fermion.wilson_clover(U, p)
-    h1 = w1.handle
+    tag = O.operator_tag[w1.handle]
     del w1
-    import gc
-
     gc.collect()
-    # same tag with another mass: the pooled handle is reused (set_mass, then update)
+    parked = len(O.operator_limbo.get(tag, []))
+    assert parked >= 1
+    # same tag with another mass: a parked handle is reused (set_mass, then update)
     p2 = dict(p, mass=0.23)
     w2 = g.qcd.fermion.wilson_clover(U, p2)
-    assert w2.handle == h1
+    assert len(O.operator_limbo.get(tag, [])) == parked - 1
+    assert O.operator_tag[w2.handle] == tag
     ref = W.wilson_clover(U, **p2)
     got = w2(to_dev(src, w2.F_grid)).to_numpy()
     assert site_rel_err(got, ref.M(src)) < 1e-13
