@@ -65,6 +65,7 @@ SIGNATURES = {
     "wc_apply_mpc_block": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "wc_cg_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
     "wc_solve_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
+    "wc_solve_eo2_ne_mixed": (_I, [_P, _P, _I, _I, _P, _P, _D, _L, _D, _P, _P, _P, _P, _P]),
     "wc_last_error": (ctypes.c_char_p, []),
 }
 

@@ -9,6 +9,11 @@
 * ``preconditioned``    -- lib/gpt/algorithms/inverter/preconditioned.py:24-54
 * ``defect_correcting`` -- lib/gpt/algorithms/inverter/defect_correcting.py:74-142
 * ``mixed_precision``   -- lib/gpt/algorithms/inverter/mixed_precision.py:23-32
+* ``cg_mixed``          -- no gpt counterpart: CG with reliable updates whose
+  recursion runs in ``inner_precision`` (FP32 by default) while FP64 keeps the
+  solution and the true residuals; used as
+  ``preconditioned(eo2_ne(), cg_mixed(eps=1e-10))`` it stops on the full
+  system's true relative residual |src - M dst| / |src| <= eps.
 """
 import ctypes
 
@@ -113,6 +118,58 @@ class cg:
         return matrix_operator(inv, inv_mat=mat.mat, make_dst=mat.make_dst, accept_guess=(True, False))
 
 
+class cg_mixed:
+    """Reliable-update CG in a lower inner precision (see the module notes).
+    Only the native eo2_ne path exists: it needs the fermion operator, so use
+    it inside ``preconditioned(eo2_ne(), cg_mixed(...))``."""
+
+    def __init__(self, params=None, **kw):
+        from .. import core as _core
+
+        p = {"eps": 1e-10, "maxiter": 100000, "inner_precision": _core.single, "delta": 0.1,
+             "fail_if_not_converged": False}
+        p.update(params or {})
+        p.update(kw)
+        self.params = p
+        self.eps = p["eps"]
+        self.maxiter = p["maxiter"]
+        self.inner_precision = p["inner_precision"]
+        self.delta = p["delta"]
+        self.fail_if_not_converged = p["fail_if_not_converged"]
+        self.iterations = []
+        self.true_residual = []
+        self.reliable_updates = 0
+        self.converged = None
+        self._inner = {}
+
+    def inner_operator(self, op):
+        key = id(op)
+        if key not in self._inner:
+            self._inner[key] = (op, op.converted(self.inner_precision))
+        return self._inner[key][1]
+
+    def solve(self, op, parity, dst, src):
+        N = src.grid.nrhs
+        inner = self.inner_operator(op)
+        iters = np.zeros(N, dtype=np.int64)
+        conv = np.zeros(N, dtype=np.int32)
+        tres = np.zeros(N)
+        nupd = np.zeros(1, dtype=np.int64)
+        _lib.check(_lib.load().wc_solve_eo2_ne_mixed(
+            op.handle, inner.handle, parity.tag, N, dst.ptr(), src.ptr(), float(self.eps),
+            int(self.maxiter), float(self.delta), iters.ctypes.data, conv.ctypes.data, tres.ctypes.data,
+            nupd.ctypes.data, core.stream_ptr()))
+        self.iterations = [int(x) for x in iters]
+        self.true_residual = [float(x) for x in tres]
+        self.reliable_updates = int(nupd[0])
+        self.converged = bool(all(conv))
+        if not self.converged and self.fail_if_not_converged:
+            raise ValueError("FATAL error: mixed-precision CG not converged")
+
+    def __call__(self, mat):
+        raise TypeError("cg_mixed runs inside preconditioned(eo2_ne(), cg_mixed(...))")
+
+
 class preconditioned:
     """M^-1 = L Mpc^-1 R + S (preconditioned.py:24-54)."""
 
@@ -122,6 +179,15 @@ class preconditioned:
 
     def __call__(self, mat):
         matrix = self.preconditioner(mat)
+        if isinstance(self.inverter, cg_mixed):
+            if getattr(matrix.Mpc, "eo2_ne", None) is None:
+                raise TypeError("cg_mixed needs the eo2_ne preconditioner of a gpt_amd fermion")
+
+            def inv_mixed(dst, src):
+                self.inverter.solve(matrix.op, matrix.D, dst, src)
+
+            return matrix_operator(inv_mixed, inv_mat=mat.mat, make_dst=mat.make_dst,
+                                   accept_guess=(True, False))
         inv_mat = self.inverter(matrix.Mpc)
         native = (isinstance(self.inverter, cg) and self.inverter._native_ok(matrix.Mpc))
 

@@ -557,6 +557,154 @@ static int solve_impl(wc_operator *op, int nrhs, int pd, void *dst, const void *
   return WC_OK;
 }
 
+// Mixed-precision solve with reliable updates for nrhs right-hand sides.
+// The inner CG (cg.py's recursion on Mpc^dag Mpc) runs in the inner
+// operator's precision; FP64 holds the solution x, the source terms and the
+// true residuals.  A reliable update (x += x_inner, r = b - A x recomputed in
+// FP64, the search direction kept) happens when a right-hand side's iterated
+// |r|^2 has dropped by delta^2 since the last one.  Stopping test, at each
+// reliable update: |R src - Mpc y| <= tol |src| -- after the L, S
+// reconstruction this is exactly the full system's true residual
+// |src - M dst| / |src| (the C rows are solved exactly).
+static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst, const void *src, double tol,
+                      long maxiter, double delta, long *iters, int *conv, double *tres, long *nupd, hipStream_t s) {
+  const int pc = 1 - pd;
+  int rc;
+  if ((rc = ensure_tmp(hi, 9, N))) return rc;
+  if ((rc = ensure_tmp(lo, 7, N))) return rc;
+  if ((rc = ensure_red(hi, N))) return rc;
+  if ((rc = ensure_red(lo, N))) return rc;
+  const size_t hb = half_bytes(hi) * (size_t)N;
+  auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
+  void *bs = hi->tmp[3], *xh = hi->tmp[4], *rs = hi->tmp[5], *rh = hi->tmp[6], *tC = hi->tmp[7], *mp = hi->tmp[8];
+  void *rl = lo->tmp[3], *pl = lo->tmp[4], *ml = lo->tmp[5], *xl = lo->tmp[6];
+  const long n = hi->g.Vh * 12 * N;
+  const int hop_blocks = wck::site_blocks(hi->g.Vh * N);
+  double *sth = hi->st, *hsh = hi->host, *stl = lo->st, *hsl = lo->host;
+  const bool lo_single = lo->prec == WC_SINGLE;
+
+  // b_s = R src = src_D - Meooe A_C^-1 src_C ; |src|^2 per RHS
+  site_op<double>(hi, N, pc, 1, half(src, pc), tC, nullptr, s);
+  {
+    HopLaunch a = hop_args<double>(hi, N, pd, 0, wck::EPI_AXPY, tC, bs);
+    a.y = half(src, pd);
+    wck::hop<double>(a, s);
+  }
+  // |src|^2 over both parities (a full field is two half fields back to back)
+  wck::dot_partials_rhs<double>(2 * n, N, src, src, hi->partials, nullptr, s);
+  wck::reduce_final_rhs(hi->partials, wck::kRedBlocks, N, sth + 8 * N, s);
+  // guess: y = L^-1 dst = A_D dst_D
+  site_op<double>(hi, N, pd, 0, half(dst, pd), xh, nullptr, s);
+  WC_LAUNCHED();
+
+  std::vector<double> src2(N), maxr(N), active(N, 1.0);
+  int nactive = N;
+  long updates = 0;
+  // reliable update: r_s = b_s - Mpc y, true = |r_s|^2, r_h = Mpc^dag r_s,
+  // r_lo = r_h, c = |r_h|^2 (into the inner CG state)
+  auto reliable = [&](long k) -> int {
+    int rc2;
+    if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s))) return rc2;
+    wck::axpy<double>(n, -1.0, 0.0, mp, bs, rs, s);
+    if ((rc2 = mpc_impl<double>(hi, N, pd, 1, rh, rs, nullptr, s))) return rc2;
+    if (lo_single) wck::convert_d2f(n, rh, rl, s);
+    else WC_HIP(hipMemcpyAsync(rl, rh, hb, hipMemcpyDeviceToDevice, s));
+    if ((rc2 = norm2_rhs<double>(hi, N, rs, sth + 9 * N, s))) return rc2;
+    if ((rc2 = norm2_rhs<double>(hi, N, rh, sth + 10 * N, s))) return rc2;
+    WC_HIP(hipMemcpyAsync(hsh + 8 * N, sth + 8 * N, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
+    WC_HIP(hipStreamSynchronize(s));
+    ++updates;
+    for (int i = 0; i < N; ++i) {
+      src2[i] = hsh[8 * N + i];
+      if (active[i] == 0.0) continue;
+      const double t2 = hsh[9 * N + i], c = hsh[10 * N + i];
+      if (!std::isfinite(t2) || !std::isfinite(c))
+        WC_FAIL(WC_ERR_NUMERIC, "mixed CG: true residual of right-hand side %d became %g", i, t2);
+      if (tres) tres[i] = src2[i] > 0 ? std::sqrt(t2 / src2[i]) : 0.0;
+      if (t2 <= tol * tol * src2[i]) {
+        active[i] = 0.0;
+        --nactive;
+        if (iters) iters[i] = k;
+        if (conv) conv[i] = 1;
+      }
+      maxr[i] = c;
+    }
+    // c of the inner recursion = |r_h|^2 for the still active right-hand sides
+    for (int i = 0; i < N; ++i) hsl[i] = active[i] != 0.0 ? hsh[10 * N + i] : 0.0;
+    WC_HIP(hipMemcpyAsync(stl + 0 * N, hsl, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < N; ++i) hsl[5 * N + i] = active[i];
+    WC_HIP(hipMemcpyAsync(stl + 5 * N, hsl + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    return WC_OK;
+  };
+  for (int i = 0; i < N; ++i) {
+    if (iters) iters[i] = 0;
+    if (conv) conv[i] = 0;
+    if (tres) tres[i] = 0.0;
+  }
+  if ((rc = reliable(0))) return rc;
+  // zero sources converge at once (the true residual is 0 <= 0)
+  if (lo_single) wck::zero<float>(n, xl, s);
+  else wck::zero<double>(n, xl, s);
+  WC_HIP(hipMemcpyAsync(pl, rl, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
+  long k = 0;
+  for (; k < maxiter && nactive > 0; ++k) {
+    if (lo_single) {
+      if ((rc = mpc_impl<float>(lo, N, pd, 2, ml, pl, pl, s))) return rc;
+    } else {
+      if ((rc = mpc_impl<double>(lo, N, pd, 2, ml, pl, pl, s))) return rc;
+    }
+    wck::cg_alpha(lo->partials, hop_blocks, N, stl, s);
+    if (lo_single) {
+      wck::cg_update1<float>(n, N, stl, rl, ml, xl, pl, lo->partials, s);
+      wck::cg_beta(lo->partials, wck::kRedBlocks, N, stl, s);
+      wck::cg_update2<float>(n, N, stl, rl, pl, s);
+    } else {
+      wck::cg_update1<double>(n, N, stl, rl, ml, xl, pl, lo->partials, s);
+      wck::cg_beta(lo->partials, wck::kRedBlocks, N, stl, s);
+      wck::cg_update2<double>(n, N, stl, rl, pl, s);
+    }
+    WC_LAUNCHED();
+    WC_HIP(hipMemcpyAsync(hsl + 4 * N, stl + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    WC_HIP(hipStreamSynchronize(s));
+    bool trigger = false;
+    for (int i = 0; i < N; ++i) {
+      if (active[i] == 0.0) continue;
+      const double cp = hsl[4 * N + i];
+      if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "mixed CG: residual of right-hand side %d became %g", i, cp);
+      if (cp > maxr[i]) maxr[i] = cp;
+      if (cp < delta * delta * maxr[i]) trigger = true;
+    }
+    if (trigger) {
+      // fold the inner solution into the FP64 one, restart its accumulation
+      if (lo_single) wck::accumulate_f2d(n, xl, xh, s);
+      else wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
+      if (lo_single) wck::zero<float>(n, xl, s);
+      else wck::zero<double>(n, xl, s);
+      if ((rc = reliable(k + 1))) return rc;
+    }
+  }
+  if (nactive > 0) {
+    // not converged within maxiter: fold what the inner solve has and report
+    if (lo_single) wck::accumulate_f2d(n, xl, xh, s);
+    else wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
+    if ((rc = reliable(k))) return rc;
+    for (int i = 0; i < N; ++i)
+      if (active[i] != 0.0 && iters) iters[i] = k;
+  }
+  if (nupd) *nupd = updates;
+  // dst = L y + S src
+  site_op<double>(hi, N, pd, 1, xh, half(dst, pd), nullptr, s);
+  site_op<double>(hi, N, pc, 1, half(src, pc), tC, nullptr, s);
+  {
+    HopLaunch a = hop_args<double>(hi, N, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
+    a.clov = hi->has_clover ? hi->Ai[pc] : nullptr;
+    a.y = tC;
+    wck::hop<double>(a, s);
+  }
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
 static int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
   if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);
   if ((double)half_bytes(op) * nrhs >= 4294967296.0)
@@ -641,6 +789,23 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
                                                     converged, r2, b2, s)
                                : solve_impl<float>(op, nrhs, parity, dst, src, eps, maxiter, history, iterations,
                                                    converged, r2, b2, s);
+}
+
+int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int nrhs, void *dst, const void *src,
+                          double tol, long maxiter, double delta, long *iterations, int *converged,
+                          double *true_residual, long *reliable_updates, void *stream) {
+  if (!op || !inner || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: dst must not alias src");
+  if (op->prec != WC_DOUBLE) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: the outer operator must be FP64");
+  for (int mu = 0; mu < 4; ++mu)
+    if (op->g.L[mu] != inner->g.L[mu]) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: operators of different lattices");
+  if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: parity must be 0 or 1");
+  if (maxiter < 0 || !(tol > 0.0) || !(delta > 0.0 && delta < 1.0))
+    WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: need maxiter >= 0, tol > 0, 0 < delta < 1");
+  int rc = check_nrhs(op, nrhs, "wc_solve_eo2_ne_mixed");
+  if (rc) return rc;
+  return mixed_impl(op, inner, nrhs, parity, dst, src, tol, maxiter, delta, iterations, converged, true_residual,
+                    reliable_updates, (hipStream_t)stream);
 }
 
 int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,

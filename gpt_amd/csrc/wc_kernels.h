@@ -61,6 +61,10 @@ template <typename R>
 void dot_partials_rhs(long n, int nrhs, const void *a, const void *b, double *pre, double *pim, hipStream_t s);
 void reduce_final_rhs(const double *partials, int nblocks, int nrhs, double *out, hipStream_t s);
 int rhs_block_threads(int nrhs);
+// hi[e] += (double) lo[e] (FP32 -> FP64 accumulation of n entries)
+void accumulate_f2d(long n, const void *lo, void *hi, hipStream_t s);
+// x = 0 for n complex entries of R (a kernel, so it can sit in a captured graph)
+template <typename R> void zero(long n, void *x, hipStream_t s);
 // x[e] *= f[e % nrhs] for n entries (per-RHS real factors on the device)
 template <typename R> void scale_rhs(long n, int nrhs, const double *f, void *x, hipStream_t s);
 

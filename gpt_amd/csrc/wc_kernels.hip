@@ -888,6 +888,28 @@ void reduce_final_rhs(const double *partials, int nblocks, int nrhs, double *out
   k_reduce_final_rhs<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, out);
 }
 
+__global__ void k_acc_f2d(long n, const float2 *lo, double2 *hi) {
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double2 h = hi[i];
+  float2 l = lo[i];
+  hi[i] = make_double2(h.x + (double)l.x, h.y + (double)l.y);
+}
+void accumulate_f2d(long n, const void *lo, void *hi, hipStream_t s) {
+  if (n > 0) k_acc_f2d<<<elem_blocks(n), 256, 0, s>>>(n, (const float2 *)lo, (double2 *)hi);
+}
+template <typename R> __global__ void k_zero(long n, typename c2<R>::t *x) {
+  using C = typename c2<R>::t;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) x[i] = cmk<C>(0, 0);
+}
+template <typename R> void zero(long n, void *x, hipStream_t s) {
+  using C = typename c2<R>::t;
+  if (n > 0) k_zero<R><<<elem_blocks(n), 256, 0, s>>>(n, (C *)x);
+}
+template void zero<double>(long, void *, hipStream_t);
+template void zero<float>(long, void *, hipStream_t);
+
 template <typename R>
 __global__ void k_scale_rhs(long n, int N, const double *f, typename c2<R>::t *x) {
   using C = typename c2<R>::t;

@@ -161,6 +161,23 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
                           long maxiter, double *history, long *iterations, int *converged, double *r2,
                           double *b2, void *stream);
 
+/* Mixed-precision eo2_ne solve with reliable updates (no cgpt counterpart;
+ * the precision pattern of lib/gpt/algorithms/inverter/defect_correcting.py
+ * and mixed_precision.py, done inside one CG).  `op` is the FP64 operator,
+ * `inner` the same operator in the inner precision (wc_create with
+ * precision WC_SINGLE, or WC_DOUBLE).  The CG recursion runs in the inner
+ * precision; FP64 keeps the solution, the source terms and the true
+ * residuals.  When a right-hand side's iterated |r|^2 has fallen by delta^2
+ * since the last reliable update, the inner solution is added to the FP64
+ * one and r = b - A x is recomputed in FP64 (the search direction is kept).
+ * Stops per right-hand side when the FP64 true residual of the full system
+ * satisfies |src - M dst| <= tol |src| (checked at reliable updates).
+ * dst holds the initial guess on entry.  iterations / converged /
+ * true_residual: nrhs entries (may be NULL); reliable_updates: one long. */
+int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int nrhs, void *dst, const void *src,
+                          double tol, long maxiter, double delta, long *iterations, int *converged,
+                          double *true_residual, long *reliable_updates, void *stream);
+
 /* message of the last failed call on this thread */
 const char *wc_last_error(void);
 
