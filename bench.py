@@ -37,7 +37,12 @@ def parse():
     ap.add_argument("--pairs", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-cg", action="store_true", help="skip the CG solve measurement")
-    ap.add_argument("--only", default=None, help="run one variant: fp64_18, fp64_12, fp32_18, fp32_12")
+    ap.add_argument("--only", default=None,
+                    help="comma-separated variants to run: fp64_18, fp64_12, fp32_18, fp32_12, "
+                         "fp64_18_rhs12, fp32_18_rhs12")
+    ap.add_argument("--cg-lattice", default="24,24,24,48")
+    ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
+    ap.add_argument("--nrhs", type=int, default=12)
     return ap.parse_args()
 
 
@@ -175,17 +180,22 @@ def main():
     torch.cuda.synchronize()
     log(f"bench: links generated in {time.perf_counter() - t:.1f} s")
 
-    variants = [("fp64_18", g.double, False), ("fp64_12", g.double, True),
-                ("fp32_18", g.single, False), ("fp32_12", g.single, True)]
+    variants = [("fp64_18", g.double, False, 1), ("fp64_12", g.double, True, 1),
+                ("fp32_18", g.single, False, 1), ("fp32_12", g.single, True, 1),
+                ("fp64_18_rhs12", g.double, False, 12), ("fp32_18_rhs12", g.single, False, 12)]
     if args.only:
-        variants = [v for v in variants if v[0] == args.only]
+        keep = args.only.split(",")
+        variants = [v for v in variants if v[0] in keep]
+    if args.no_dhop:
+        variants = []
     res = {}
-    for name, prec, comp in variants:
+    for name, prec, comp, nrhs in variants:
         w = g.qcd.fermion.wilson_clover(U, mass=0.08, csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0,
                                         boundary_phases=[1, 1, 1, -1], precision=prec,
-                                        compress_links=comp)
+                                        compress_links=comp, n_rhs=nrhs)
         pairs = []
-        for k in range(args.pairs):
+        npairs = args.pairs if nrhs == 1 else 2
+        for k in range(npairs):
             s = gaussian_half(torch, g, w.F_grid_eo, g.odd, 2002 + k, dev)
             d = g.lattice(w.F_grid_eo)
             d.checkerboard(g.even)
@@ -193,29 +203,29 @@ def main():
         t_ev, t_wall = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
         real_b = prec.nbytes
         nl = 12 if comp else 18
-        bytes_site = (8 * nl + 2 * 24) * real_b
+        # compulsory bytes per output site: links once, nrhs spinors in and out
+        bytes_site = (8 * nl + nrhs * 2 * 24) * real_b
         res[name] = {"us_per_call": round(t_ev * 1e6, 2), "wall_us_per_call": round(t_wall * 1e6, 2),
-                     "GBps": round(bytes_site * Vh / t_ev / 1e9, 2),
-                     "GFLOPps": round(1320.0 * Vh / t_ev / 1e9, 2), "bytes_per_site": bytes_site,
-                     "reference_effective_GBps": round(2880.0 / 8 * real_b * Vh / t_ev / 1e9, 2)}
+                     "GBps": round(bytes_site * Vh / t_ev / 1e9, 2), "nrhs": nrhs,
+                     "GFLOPps": round(1320.0 * nrhs * Vh / t_ev / 1e9, 2), "bytes_per_site": bytes_site,
+                     "reference_effective_GBps": round(2880.0 / 8 * real_b * nrhs * Vh / t_ev / 1e9, 2)}
         log(f"bench: {name}: {res[name]}")
         del w, pairs
         torch.cuda.empty_cache()
 
-    copy_gbs = copy_bandwidth(torch, dev)
+    copy_gbs = copy_bandwidth(torch, dev) if not args.no_dhop else 0.0
     log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
 
     cg_res = None
     if not args.no_cg:
-        cg_res = cg_measure(torch, g, dev)
-        log(f"bench: cg {cg_res}")
+        cg_res = cg_measure(torch, g, dev, fd=[int(x) for x in args.cg_lattice.split(",")], nrhs=args.nrhs)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
         log(f"bench: cpu baseline {cpu}")
 
-    head = res.get("fp64_18") or next(iter(res.values()))
+    head = res.get("fp64_18") or next(iter(res.values()), {"GBps": 0.0, "us_per_call": 0.0})
     achieved = head["GBps"]
     out = {
         "metric": "EO Dhop GB/s (% HBM peak); mixed-prec CG solves/s + p99 latency at 1e-10",
@@ -243,31 +253,96 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cg_measure(torch, g, dev, fd=(16, 16, 16, 32)):
-    """FP64 eo2_ne + CG to 1e-10 on one Gaussian source (solves/s)."""
+def true_residual(g, w, x, src):
+    """|src - M x| / |src| with the FP64 operator (per right-hand side)"""
+    r = g(src - w * x)
+    n2r, n2s = g.norm2(r), g.norm2(src)
+    if not isinstance(n2r, list):
+        n2r, n2s = [n2r], [n2s]
+    return [float(np.sqrt(a / b)) for a, b in zip(n2r, n2s)]
+
+
+def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10, reps=3):
+    """Solves/s to a true relative residual tol: plain FP64 eo2_ne CG one RHS
+    at a time, the mixed-precision reliable-update solve (FP32 inner) one RHS
+    at a time, and the same on nrhs right-hand sides at once."""
+    from gpt_amd.qcd.fermion.operator import pack
+
     fd = list(fd)
-    U = smooth_links_device(torch, fd, 0.3, 4001, dev)
-    w = g.qcd.fermion.wilson_clover(U, mass=0.08, csw_r=1.0, csw_t=1.0, boundary_phases=[1, 1, 1, -1])
+    U = smooth_links_device(torch, fd, 0.3, 3001, dev)
+    prm = dict(mass=mass, csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0, boundary_phases=[1, 1, 1, -1])
+    w = g.qcd.fermion.wilson_clover(U, prm)
+    wN = g.qcd.fermion.wilson_clover(U, prm, n_rhs=nrhs)
     inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
-    cg = inv.cg(eps=1e-10, maxiter=5000)
-    slv = w.propagator(inv.preconditioned(pc.eo2_ne(), cg))
-    src = g.lattice(w.F_grid)
     gen = torch.Generator(device=dev)
-    gen.manual_seed(4100)
-    src.data.copy_(torch.randn(src.data.shape, generator=gen, dtype=torch.float64, device=dev))
-    g(slv * src)
-    torch.cuda.synchronize()
-    lat = []
-    its = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        g(slv * src)
+    gen.manual_seed(3002)
+    srcs = []
+    for _ in range(nrhs):
+        x = g.lattice(w.F_grid)
+        x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
+        srcs.append(x)
+
+    def timed(fn):
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t0
+
+    out = {"lattice": fd, "mass": mass, "csw": 1.0, "tol_true_residual": tol, "nrhs": nrhs}
+    # (a) plain FP64 CG on the normal equation, eps tightened until the true residual meets tol
+    eps = tol
+    for _ in range(6):
+        cg = inv.cg(eps=eps, maxiter=20000)
+        slv = w.propagator(inv.preconditioned(pc.eo2_ne(), cg))
+        x, _t = timed(lambda: g(slv * srcs[0]))
+        if true_residual(g, w, x, srcs[0])[0] <= tol:
+            break
+        eps /= 2.0
+    lat, res, its = [], [], []
+    for i in range(reps):
+        x, t = timed(lambda: g(slv * srcs[i]))
+        lat.append(t)
+        res.append(true_residual(g, w, x, srcs[i])[0])
         its.append(len(cg.history))
-    return {"lattice": fd, "precision": "fp64", "tol": 1e-10, "iterations": its[-1],
-            "solves_per_s": round(1.0 / np.median(lat), 3), "ms_per_solve": round(np.median(lat) * 1e3, 2),
-            "ms_per_iteration": round(np.median(lat) * 1e3 / its[-1], 4)}
+    x64 = [g(slv * srcs[i]) for i in range(2)]
+    out["fp64_cg"] = {"eps_normal_equation": eps, "iterations": its, "true_residual": max(res),
+                      "s_per_solve": float(np.median(lat)), "solves_per_s": float(1.0 / np.median(lat))}
+    log(f"bench: cg fp64 {out['fp64_cg']}")
+    # (b) mixed precision, one RHS at a time
+    mcg = inv.cg_mixed(eps=tol, maxiter=20000)
+    slvm = w.propagator(inv.preconditioned(pc.eo2_ne(), mcg))
+    g(slvm * srcs[0])
+    lat, res, its, err = [], [], [], []
+    for i in range(reps):
+        x, t = timed(lambda: g(slvm * srcs[i]))
+        lat.append(t)
+        res.append(true_residual(g, w, x, srcs[i])[0])
+        its.append(mcg.iterations[0])
+        if i < 2:
+            err.append(float(np.sqrt(g.norm2(g(x - x64[i])) / g.norm2(x64[i]))))
+    out["mixed_1rhs"] = {"iterations": its, "reliable_updates": mcg.reliable_updates, "true_residual": max(res),
+                         "rel_error_vs_fp64_solution": max(err), "s_per_solve": float(np.median(lat)),
+                         "solves_per_s": float(1.0 / np.median(lat))}
+    log(f"bench: cg mixed 1 {out['mixed_1rhs']}")
+    # (c) mixed precision, nrhs at once
+    mN = inv.cg_mixed(eps=tol, maxiter=20000)
+    slvN = wN.propagator(inv.preconditioned(pc.eo2_ne(), mN))
+    S = pack(srcs)
+    g(slvN * S)
+    lat = []
+    for _ in range(2):
+        X, t = timed(lambda: g(slvN * S))
+        lat.append(t)
+    res = true_residual(g, wN, X, S)
+    err = [float(np.sqrt(g.norm2(g(X.rhs(i) - x64[i])) / g.norm2(x64[i]))) for i in range(2)]
+    tN = float(np.median(lat))
+    out["mixed_multi"] = {"iterations": mN.iterations, "reliable_updates": mN.reliable_updates,
+                          "true_residual": max(res), "rel_error_vs_fp64_solution": max(err),
+                          "s_per_batch": tN, "solves_per_s": float(nrhs / tN)}
+    log(f"bench: cg mixed {nrhs} {out['mixed_multi']}")
+    out["speedup_mixed_multi_vs_fp64"] = out["mixed_multi"]["solves_per_s"] / out["fp64_cg"]["solves_per_s"]
+    return out
 
 
 if __name__ == "__main__":
