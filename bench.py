@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--cg-lattice", default="24,24,24,48")
     ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
     ap.add_argument("--nrhs", type=int, default=12)
+    ap.add_argument("--serve", type=int, default=0, help="serving stream: number of requests (0: skip)")
     return ap.parse_args()
 
 
@@ -200,6 +201,7 @@ def main():
             d = g.lattice(w.F_grid_eo)
             d.checkerboard(g.even)
             pairs.append((s, d))
+        shape, _ = w.tune_hop(nrhs, reps=3)
         t_ev, t_wall = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
         real_b = prec.nbytes
         nl = 12 if comp else 18
@@ -208,6 +210,8 @@ def main():
         res[name] = {"us_per_call": round(t_ev * 1e6, 2), "wall_us_per_call": round(t_wall * 1e6, 2),
                      "GBps": round(bytes_site * Vh / t_ev / 1e9, 2), "nrhs": nrhs,
                      "GFLOPps": round(1320.0 * nrhs * Vh / t_ev / 1e9, 2), "bytes_per_site": bytes_site,
+                     "shape": {"rhs_per_thread": shape[0], "threads_per_block": 64 * shape[1],
+                               "max_blocks_per_cu": shape[2], "t_runs": shape[3]},
                      "reference_effective_GBps": round(2880.0 / 8 * real_b * nrhs * Vh / t_ev / 1e9, 2)}
         log(f"bench: {name}: {res[name]}")
         del w, pairs
@@ -219,6 +223,10 @@ def main():
     cg_res = None
     if not args.no_cg:
         cg_res = cg_measure(torch, g, dev, fd=[int(x) for x in args.cg_lattice.split(",")], nrhs=args.nrhs)
+
+    serve_res = None
+    if args.serve > 0:
+        serve_res = serve_measure(torch, g, dev, args.serve)
 
     cpu = None
     if not args.no_cpu:
@@ -249,6 +257,7 @@ def main():
         "variants": res,
         "copy_kernel_GBps": round(copy_gbs, 1),
         "cg": cg_res,
+        "serving": serve_res,
     }
     print(json.dumps(out), flush=True)
 
@@ -342,6 +351,72 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
                           "s_per_batch": tN, "solves_per_s": float(nrhs / tN)}
     log(f"bench: cg mixed {nrhs} {out['mixed_multi']}")
     out["speedup_mixed_multi_vs_fp64"] = out["mixed_multi"]["solves_per_s"] / out["fp64_cg"]["solves_per_s"]
+    return out
+
+
+def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10):
+    """Serving stream: requests over volumes {8^4, 16^3x32, 24^3x48, 32^3x64},
+    3 resident fields per volume (seeds 5101..5103), mass U[0.02, 0.2], 1..12
+    Gaussian right-hand sides, tolerance tol on the true residual.  First all
+    nreq requests at once (capacity, right-hand sides per second), then nreq
+    new ones with Poisson arrivals at load x that capacity: p50/p99 latency
+    and solves/s."""
+    from gpt_amd import serve
+
+    vols = [[8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64]]
+    prm = dict(csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0, boundary_phases=[1, 1, 1, -1])
+    srv = serve.server(prm, nslots=nslots, streams=4)
+    for vi, fd in enumerate(vols):
+        for k in range(3):
+            srv.add_field((vi, k), smooth_links_device(torch, fd, 0.3, 5101 + k, dev))
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+
+    def make(n):
+        reqs = []
+        for _ in range(n):
+            vi, k = int(rng.integers(4)), int(rng.integers(3))
+            m, nr = float(rng.uniform(0.02, 0.2)), int(rng.integers(1, 13))
+            grid = g.grid(vols[vi], g.double)
+            srcs = []
+            for _ in range(nr):
+                x = g.lattice(grid)
+                x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
+                srcs.append(x)
+            reqs.append(serve.request((vi, k), m, srcs, tol=tol, maxiter=50000))
+        return reqs
+
+    def summary(done, T):
+        nrhs = sum(len(r.sources) for r in done)
+        st = serve.latency_stats(done)
+        return {"requests": len(done), "rhs": nrhs, "wall_s": T, "solves_per_s": nrhs / T,
+                "p50_latency_s": st["p50_s"], "p99_latency_s": st["p99_s"],
+                "max_true_residual": max(max(r.true_residual) for r in done),
+                "all_converged": all(all(r.converged) for r in done)}
+
+    def progress(r):
+        log(f"bench: serving request {r.id} done: {len(r.sources)} rhs, field {r.field}, mass {r.mass:.3f}, "
+            f"latency {r.latency:.3f} s, iterations {max(r.iterations)}")
+
+    reqs = make(nreq)
+    t0 = time.perf_counter()
+    done = srv.run([(0.0, r) for r in reqs], on_done=progress)
+    cap = summary(done, time.perf_counter() - t0)
+    log(f"bench: serving capacity {cap}")
+    del reqs, done
+    reqs = make(nreq)
+    mean_rhs = float(np.mean([len(r.sources) for r in reqs]))
+    lam = load * cap["solves_per_s"] / mean_rhs
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    t0 = time.perf_counter()
+    done = srv.run(list(zip(arrivals.tolist(), reqs)), on_done=progress)
+    pois = summary(done, time.perf_counter() - t0)
+    pois["arrival_rate_per_s"] = lam
+    log(f"bench: serving poisson {pois}")
+    out = {"nslots": nslots, "streams": len(srv.streams), "load": load, "capacity": cap, "poisson": pois,
+           "peak_groups": srv.peak_groups}
+    srv.close()
     return out
 
 

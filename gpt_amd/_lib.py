@@ -66,6 +66,15 @@ SIGNATURES = {
     "wc_cg_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
     "wc_solve_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
     "wc_solve_eo2_ne_mixed": (_I, [_P, _P, _I, _I, _P, _P, _D, _L, _D, _P, _P, _P, _P, _P]),
+    "wc_tune_hop": (_I, [_P, _I, _I, _P, _P, _P]),
+    "wc_set_hop_shape": (_I, [_P, _I, _P]),
+    "wc_get_hop_shape": (_I, [_P, _I, _P]),
+    "wc_batch_create": (_I, [_P, _P, _I, _I, _D, _P, ctypes.POINTER(_P)]),
+    "wc_batch_destroy": (_I, [_P]),
+    "wc_batch_admit": (_I, [_P, _I, _P, _P, _D, _L]),
+    "wc_batch_step": (_I, [_P, _I, _P, _P]),
+    "wc_batch_extract": (_I, [_P, _I, _P, _P, _P, _P]),
+    "wc_batch_slots": (_I, [_P, _P, _P]),
     "wc_last_error": (ctypes.c_char_p, []),
 }
 

@@ -11,18 +11,20 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libgpt_amd_wc.so")
-SOURCES = ["wc_kernels.hip", "wc_api.hip"]
+SOURCES = ["wc_kernels.hip", "wc_hop_f64.hip", "wc_hop_f32.hip", "wc_api.hip", "wc_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
     # keep the compiler from emitting packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fused multiply-adds only where the source writes fma(): same roundings in every instantiation
+    "-ffp-contract=off",
     "-Wall", "-Wno-unused-result",
 ]
 
 
-def build(verbose=False, jobs=2):
+def build(verbose=False):
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
     objs = []

@@ -3,116 +3,23 @@
 // dispatch, the Schur-complement compositions and the CG driver.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
-#include "../../include/gpt_amd_wc.h"
-#include "wc_kernels.h"
+#include "wc_internal.h"
 
-
-using wck::HopLaunch;
-
-static thread_local std::string g_err;
-
-#define WC_FAIL(code, ...)                       \
-  do {                                           \
-    char _b[512];                                \
-    snprintf(_b, sizeof _b, __VA_ARGS__);        \
-    g_err = _b;                                  \
-    return code;                                 \
-  } while (0)
-
-#define WC_HIP(x)                                                                                    \
-  do {                                                                                               \
-    hipError_t _e = (x);                                                                             \
-    if (_e != hipSuccess)                                                                            \
-      WC_FAIL(WC_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, __LINE__); \
-  } while (0)
-
-// launches are asynchronous: surface launch-configuration errors right away
-#define WC_LAUNCHED() WC_HIP(hipGetLastError())
-
-struct wc_operator {
-  wc_params p{};
-  Geom g{};
-  int prec = WC_DOUBLE;
-  int nl = 9;
-  size_t csz = 16;  // bytes per complex value
-  double mass = 0, diag = 0, cs = 0, ct = -0.5;
-  bool has_clover = false, open_bc = false;
-  void *W[2] = {nullptr, nullptr};
-  void *A[2] = {nullptr, nullptr};
-  void *Ai[2] = {nullptr, nullptr};
-  void *Unat = nullptr;  // FP64 links [4][V][9], kept to rebuild the clover on set_mass
-  int *err = nullptr;    // device flag of the clover build
-  std::vector<void *> tmp;      // half-field scratch, tmp_bytes each
-  size_t tmp_bytes = 0;
-  double *partials = nullptr;   // per-block reduction partials
-  size_t npartials = 0;
-  double *st = nullptr;         // device scalars (CG state), nst doubles
-  double *host = nullptr;       // pinned host mirror, nst doubles
-  size_t nst = 0;
-};
-
-static size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
-
-// n scratch half fields of nrhs right-hand sides each
-static int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
-  const size_t bytes = half_bytes(op) * (size_t)nrhs;
-  if (bytes > op->tmp_bytes) {
-    for (void *q : op->tmp) (void)hipFree(q);
-    op->tmp.clear();
-    op->tmp_bytes = bytes;
-  }
-  while (op->tmp.size() < n) {
-    void *q = nullptr;
-    WC_HIP(hipMalloc(&q, op->tmp_bytes));
-    op->tmp.push_back(q);
-  }
-  return WC_OK;
-}
-
-// reduction partials and CG scalars for nrhs right-hand sides
-static int ensure_red(wc_operator *op, int nrhs) {
-  const size_t np = (size_t)std::max(wck::kRedBlocks, wck::site_blocks(op->g.Vh * nrhs)) * nrhs;
-  if (np > op->npartials) {
-    (void)hipFree(op->partials);
-    op->partials = nullptr;
-    WC_HIP(hipMalloc((void **)&op->partials, sizeof(double) * np));
-    op->npartials = np;
-  }
-  const size_t ns = (size_t)16 * nrhs;
-  if (ns > op->nst) {
-    (void)hipFree(op->st);
-    if (op->host) (void)hipHostFree(op->host);
-    op->st = nullptr;
-    op->host = nullptr;
-    WC_HIP(hipMalloc((void **)&op->st, sizeof(double) * ns));
-    WC_HIP(hipHostMalloc((void **)&op->host, sizeof(double) * ns, hipHostMallocDefault));
-    op->nst = ns;
-  }
-  return WC_OK;
-}
-
-static void free_op(wc_operator *op) {
-  if (!op) return;
-  for (int p = 0; p < 2; ++p) {
-    (void)hipFree(op->W[p]);
-    (void)hipFree(op->A[p]);
-    (void)hipFree(op->Ai[p]);
-  }
-  (void)hipFree(op->Unat);
-  (void)hipFree(op->err);
-  for (void *q : op->tmp) (void)hipFree(q);
-  (void)hipFree(op->partials);
-  (void)hipFree(op->st);
-  if (op->host) (void)hipHostFree(op->host);
-  delete op;
-}
+thread_local std::string g_err;
+std::mutex g_shape_mu;
+std::map<ShapeKey, wck::HopShape> g_shape_cache;
 
 // gamma matrices of gpt's chiral basis (lib/gpt/core/gamma.py:28-42)
 typedef std::complex<double> cdbl;
@@ -224,6 +131,10 @@ int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out
   op->g.Vh = op->g.V / 2;
   op->cs = -0.5 * p->nu / p->xi_0;
   op->ct = -0.5;
+  {
+    const long slice = op->g.Vh / op->g.L[3];
+    op->bps = (slice % 64 == 0) ? (int)(slice / 64) : 0;
+  }
   set_mass_values(op, p->mass);
   int rc = WC_OK;
   auto chk = [&](hipError_t e, const char *what) {
@@ -288,84 +199,6 @@ int wc_precision(const wc_operator *op) { return op ? op->prec : -1; }
 
 }  // extern "C"
 
-// ------------------------------------------------------------ compositions
-template <typename R>
-static HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, int epi, const void *in, void *out) {
-  HopLaunch a{};
-  a.g = op->g;
-  a.nrhs = nrhs;
-  a.pout = pout;
-  a.dag = dag;
-  a.nl = op->nl;
-  a.epi = epi;
-  a.in = in;
-  a.out = out;
-  a.W = op->W[pout];
-  a.cs = op->cs;
-  a.ct = op->ct;
-  for (int mu = 0; mu < 4; ++mu) {
-    a.ph[mu][0] = op->p.boundary_phases[mu][0];
-    a.ph[mu][1] = op->p.boundary_phases[mu][1];
-  }
-  a.open_bc = op->open_bc ? 1 : 0;
-  a.diag = op->diag;
-  a.clov_scalar = 1.0 / op->diag;
-  return a;
-}
-
-// out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
-template <typename R>
-static void site_op(const wc_operator *op, int nrhs, int p, int inv, const void *in, void *out, const void *y,
-                    hipStream_t s) {
-  const void *B = op->has_clover ? (inv ? op->Ai[p] : op->A[p]) : nullptr;
-  double sc = inv ? 1.0 / op->diag : op->diag;
-  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s);
-}
-
-// Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd on nrhs
-// right-hand sides; dotv (optional) makes the last hop write per-block,
-// per-RHS partials of Re<dotv, out>.  Scratch: tmp[0..2].
-template <typename R>
-static int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
-                    hipStream_t s) {
-  const int pc = 1 - pd;
-  int rc = ensure_tmp(op, 3, nrhs);
-  if (rc) return rc;
-  if ((rc = ensure_red(op, nrhs))) return rc;
-  void *tD = op->tmp[0], *tC = op->tmp[1], *tY = op->tmp[2];
-  const void *clovC = op->has_clover ? op->Ai[pc] : nullptr;
-  const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
-  const void *src = in;
-  void *dst_mpc = (which == 2) ? tY : out;
-  if (which == 0 || which == 2) {
-    site_op<R>(op, nrhs, pd, 1, src, tD, nullptr, s);
-    HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV, tD, tC);
-    a.clov = clovC;
-    wck::hop<R>(a, s);
-    HopLaunch b = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC,
-                              dst_mpc);
-    b.y = src;
-    b.dotv = dotv;
-    b.partial = op->partials;
-    wck::hop<R>(b, s);
-    src = dst_mpc;
-  }
-  if (which == 1 || which == 2) {
-    HopLaunch a = hop_args<R>(op, nrhs, pc, 1, wck::EPI_CLOV, src, tC);
-    a.clov = clovC;
-    wck::hop<R>(a, s);
-    HopLaunch b = hop_args<R>(op, nrhs, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC,
-                              out);
-    b.clov = clovD;
-    b.y = src;
-    b.dotv = dotv;
-    b.partial = op->partials;
-    wck::hop<R>(b, s);
-  }
-  WC_LAUNCHED();
-  return WC_OK;
-}
-
 template <typename R>
 static int apply_impl(wc_operator *op, int nrhs, int opcode, void *dst, const void *src, int cb, hipStream_t s) {
   const size_t hb = half_bytes(op) * (size_t)nrhs;
@@ -381,7 +214,7 @@ static int apply_impl(wc_operator *op, int nrhs, int opcode, void *dst, const vo
         HopLaunch a = hop_args<R>(op, nrhs, p, dag, withdiag ? wck::EPI_DIAG : 0, half(src, 1 - p), half(dst, p));
         a.diag_clov = op->has_clover ? op->A[p] : nullptr;
         a.y = half(src, p);
-        wck::hop<R>(a, s);
+        WC_HOP(R, a, s);
       }
       break;
     }
@@ -405,7 +238,7 @@ static int apply_impl(wc_operator *op, int nrhs, int opcode, void *dst, const vo
         WC_FAIL(WC_ERR_ARG, "wc_apply: DhopEO maps odd to even (src_cb must be 1)");
       const int dag = (opcode == 2004 || opcode == 4002);
       HopLaunch a = hop_args<R>(op, nrhs, 1 - cb, dag, 0, src, dst);
-      wck::hop<R>(a, s);
+      WC_HOP(R, a, s);
       break;
     }
     case 2005:
@@ -422,18 +255,6 @@ static int apply_impl(wc_operator *op, int nrhs, int opcode, void *dst, const vo
   return WC_OK;
 }
 
-// per-RHS |x|^2 into dev_out[0..nrhs)
-template <typename R>
-static int norm2_rhs(wc_operator *op, int nrhs, const void *x, double *dev_out, hipStream_t s) {
-  int rc = ensure_red(op, nrhs);
-  if (rc) return rc;
-  const long n = op->g.Vh * 12 * nrhs;
-  wck::dot_partials_rhs<R>(n, nrhs, x, x, op->partials, nullptr, s);
-  wck::reduce_final_rhs(op->partials, wck::kRedBlocks, nrhs, dev_out, s);
-  WC_LAUNCHED();
-  return WC_OK;
-}
-
 // CG of lib/gpt/algorithms/inverter/cg.py:48-112 on Mpc^dag Mpc, run for nrhs
 // independent right-hand sides at once (each its own scalars, the operator
 // applied to all of them in one pass).  A right-hand side that converges is
@@ -442,13 +263,14 @@ static int norm2_rhs(wc_operator *op, int nrhs, const void *x, double *dev_out, 
 template <typename R>
 static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, double eps, long maxiter,
                    double *history, long *iters, int *conv, double *r2, double *b2, hipStream_t s) {
-  int rc = ensure_tmp(op, 6, nrhs);
+  int rc = ensure_tmp(op, 10, nrhs);
   if (rc) return rc;
   if ((rc = ensure_red(op, nrhs))) return rc;
   const int N = nrhs;
-  void *r = op->tmp[3], *p = op->tmp[4], *mmp = op->tmp[5];
+  void *r = op->tmp[3], *p = op->tmp[4], *mmp = op->tmp[5], *aDp = op->tmp[9];
+  const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
   const long n = op->g.Vh * 12 * N;
-  const int hop_blocks = wck::site_blocks(op->g.Vh * N);
+  const int hop_blocks = wck::hop_blocks(op->g);
   double *st = op->st, *hs = op->host;
   // mmp = A x ; r = b - mmp ; p = r ; c = |p|^2 ; ssq = |b|^2
   if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, x, nullptr, s))) return rc;
@@ -481,14 +303,17 @@ static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, do
   }
   for (int i = 0; i < N; ++i) hs[i] = active[i];
   WC_HIP(hipMemcpyAsync(st + 5 * N, hs, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  // A_D^-1 p, kept up to date by the fused x/p update
+  site_op<R>(op, N, pd, 1, p, aDp, nullptr, s);
   long k = 0;
   for (; k < maxiter && nactive > 0; ++k) {
     // mmp = A p with per-RHS partials of Re<p, mmp>
-    if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, p, p, s))) return rc;
+    if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, p, p, s, aDp))) return rc;
     wck::cg_alpha(op->partials, hop_blocks, N, st, s);
-    wck::cg_update1<R>(n, N, st, r, mmp, x, p, op->partials, s);
+    wck::cg_update_r<R>(n, N, st, r, mmp, op->partials, s);
     wck::cg_beta(op->partials, wck::kRedBlocks, N, st, s);
-    wck::cg_update2<R>(n, N, st, r, p, s);
+    // x += alpha p ; p = beta p + r ; A_D^-1 p for the next iteration
+    wck::cg_update_xp<R>(op->g, N, pd, st, r, p, x, clovD, 1.0 / op->diag, aDp, op->open_bc ? 1 : 0, s);
     WC_LAUNCHED();
     WC_HIP(hipMemcpyAsync(hs + 4 * N, st + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     WC_HIP(hipStreamSynchronize(s));
@@ -536,7 +361,7 @@ static int solve_impl(wc_operator *op, int nrhs, int pd, void *dst, const void *
   {
     HopLaunch a = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY, tC, tR);
     a.y = half(src, pd);
-    wck::hop<R>(a, s);
+    WC_HOP(R, a, s);
   }
   // R_ne src = Mpc^dag R src
   if ((rc = mpc_impl<R>(op, nrhs, pd, 1, pb, tR, nullptr, s))) return rc;
@@ -551,7 +376,7 @@ static int solve_impl(wc_operator *op, int nrhs, int pd, void *dst, const void *
     HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
     a.clov = op->has_clover ? op->Ai[pc] : nullptr;
     a.y = tR;
-    wck::hop<R>(a, s);
+    WC_HOP(R, a, s);
   }
   WC_LAUNCHED();
   return WC_OK;
@@ -571,15 +396,14 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   const int pc = 1 - pd;
   int rc;
   if ((rc = ensure_tmp(hi, 9, N))) return rc;
-  if ((rc = ensure_tmp(lo, 7, N))) return rc;
+  if ((rc = ensure_tmp(lo, 8, N))) return rc;
   if ((rc = ensure_red(hi, N))) return rc;
   if ((rc = ensure_red(lo, N))) return rc;
   const size_t hb = half_bytes(hi) * (size_t)N;
   auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
   void *bs = hi->tmp[3], *xh = hi->tmp[4], *rs = hi->tmp[5], *rh = hi->tmp[6], *tC = hi->tmp[7], *mp = hi->tmp[8];
-  void *rl = lo->tmp[3], *pl = lo->tmp[4], *ml = lo->tmp[5], *xl = lo->tmp[6];
+  void *rl = lo->tmp[3], *pl = lo->tmp[4], *ml = lo->tmp[5], *xl = lo->tmp[6], *aDl = lo->tmp[7];
   const long n = hi->g.Vh * 12 * N;
-  const int hop_blocks = wck::site_blocks(hi->g.Vh * N);
   double *sth = hi->st, *hsh = hi->host, *stl = lo->st, *hsl = lo->host;
   const bool lo_single = lo->prec == WC_SINGLE;
 
@@ -588,7 +412,7 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   {
     HopLaunch a = hop_args<double>(hi, N, pd, 0, wck::EPI_AXPY, tC, bs);
     a.y = half(src, pd);
-    wck::hop<double>(a, s);
+    WC_HOP(double, a, s);
   }
   // |src|^2 over both parities (a full field is two half fields back to back)
   wck::dot_partials_rhs<double>(2 * n, N, src, src, hi->partials, nullptr, s);
@@ -646,24 +470,13 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   if (lo_single) wck::zero<float>(n, xl, s);
   else wck::zero<double>(n, xl, s);
   WC_HIP(hipMemcpyAsync(pl, rl, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
+  if (lo_single) site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s);
+  else site_op<double>(lo, N, pd, 1, pl, aDl, nullptr, s);
   long k = 0;
   for (; k < maxiter && nactive > 0; ++k) {
-    if (lo_single) {
-      if ((rc = mpc_impl<float>(lo, N, pd, 2, ml, pl, pl, s))) return rc;
-    } else {
-      if ((rc = mpc_impl<double>(lo, N, pd, 2, ml, pl, pl, s))) return rc;
-    }
-    wck::cg_alpha(lo->partials, hop_blocks, N, stl, s);
-    if (lo_single) {
-      wck::cg_update1<float>(n, N, stl, rl, ml, xl, pl, lo->partials, s);
-      wck::cg_beta(lo->partials, wck::kRedBlocks, N, stl, s);
-      wck::cg_update2<float>(n, N, stl, rl, pl, s);
-    } else {
-      wck::cg_update1<double>(n, N, stl, rl, ml, xl, pl, lo->partials, s);
-      wck::cg_beta(lo->partials, wck::kRedBlocks, N, stl, s);
-      wck::cg_update2<double>(n, N, stl, rl, pl, s);
-    }
-    WC_LAUNCHED();
+    rc = lo_single ? inner_iteration<float>(lo, N, pd, rl, pl, ml, xl, aDl, s)
+                   : inner_iteration<double>(lo, N, pd, rl, pl, ml, xl, aDl, s);
+    if (rc) return rc;
     WC_HIP(hipMemcpyAsync(hsl + 4 * N, stl + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     WC_HIP(hipStreamSynchronize(s));
     bool trigger = false;
@@ -699,20 +512,143 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     HopLaunch a = hop_args<double>(hi, N, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
     a.clov = hi->has_clover ? hi->Ai[pc] : nullptr;
     a.y = tC;
-    wck::hop<double>(a, s);
+    WC_HOP(double, a, s);
   }
   WC_LAUNCHED();
   return WC_OK;
 }
 
-static int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
-  if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);
-  if ((double)half_bytes(op) * nrhs >= 4294967296.0)
-    WC_FAIL(WC_ERR_ARG, "%s: a half field of %d right-hand sides exceeds the 4 GB gather window", fn, nrhs);
+// ------------------------------------------------------------- hop tuner
+// Times the hop (output parity 0, no epilogue) of nrhs right-hand sides for
+// every valid launch shape and returns the fastest.  Scratch: tmp[0..1],
+// zeroed first so that no shape sees denormals or NaN.
+template <typename R>
+static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopShape &best, float &best_ms) {
+  int rc = ensure_tmp(op, 2, N);
+  if (rc) return rc;
+  const long n = op->g.Vh * 12 * N;
+  wck::zero<R>(n, op->tmp[0], s);
+  wck::zero<R>(n, op->tmp[1], s);
+  std::vector<wck::HopShape> cand;
+  const int occs[] = {0, 2, 4, 8, 16};
+  const int segs[] = {1, 4, 8, 16};
+  for (int rb = 1; rb <= max_rb_prec(op->prec); ++rb)
+    for (int tpb = 1; tpb <= wck::kMaxTpb; tpb *= 2)
+      for (int oc : occs)
+        for (int sg : segs) {
+          wck::HopShape sh;
+          sh.rb = rb;
+          sh.tpb = tpb;
+          sh.occ = oc;
+          sh.seg = sg;
+          if (shape_valid(op, sh)) cand.push_back(sh);
+        }
+  hipEvent_t e0, e1;
+  WC_HIP(hipEventCreate(&e0));
+  WC_HIP(hipEventCreate(&e1));
+  best_ms = -1.0f;
+  for (const wck::HopShape &sh : cand) {
+    HopLaunch a = hop_args<R>(op, N, 0, 0, 0, op->tmp[1], op->tmp[0]);
+    a.shape = sh;
+    WC_HOP(R, a, s);  // warm-up
+    if (hipGetLastError() != hipSuccess) continue;  // a shape this device cannot launch
+    WC_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; ++k) WC_HOP(R, a, s);
+    WC_HIP(hipEventRecord(e1, s));
+    WC_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    WC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (best_ms < 0.0f || ms < best_ms) {
+      best_ms = ms;
+      best = sh;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  best_ms /= (float)reps;
   return WC_OK;
 }
 
+static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *ms_out) {
+  wck::HopShape best;
+  float ms = 0.0f;
+  int rc = op->prec == WC_DOUBLE ? tune_impl<double>(op, nrhs, reps, s, best, ms)
+                                 : tune_impl<float>(op, nrhs, reps, s, best, ms);
+  if (rc) return rc;
+  op->shape[nrhs] = best;
+  op->shape_set[nrhs] = true;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shape_cache[shape_key(op, nrhs)] = best;
+  }
+  if (ms_out) *ms_out = ms;
+  return WC_OK;
+}
+
+// Solvers tune the hop on first use for lattices where launch shapes matter
+// (>= 2^17 site-RHS pairs; WC_AUTOTUNE=0 turns it off).
+int maybe_tune(wc_operator *op, int nrhs, hipStream_t s) {
+  if (op->shape_set[nrhs]) return WC_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    if (g_shape_cache.count(shape_key(op, nrhs))) return WC_OK;
+  }
+  const char *env = getenv("WC_AUTOTUNE");
+  if (env && env[0] == '0') return WC_OK;
+  if (op->g.Vh * (long)nrhs < (1L << 17)) return WC_OK;
+  return tune_hop(op, nrhs, 3, s, nullptr);
+}
+
 extern "C" {
+
+int wc_tune_hop(wc_operator *op, int nrhs, int reps, int *shape, float *ms_per_call, void *stream) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_tune_hop: null operator");
+  int rc = check_nrhs(op, nrhs, "wc_tune_hop");
+  if (rc) return rc;
+  if (reps < 1) WC_FAIL(WC_ERR_ARG, "wc_tune_hop: reps must be >= 1");
+  if ((rc = tune_hop(op, nrhs, reps, (hipStream_t)stream, ms_per_call))) return rc;
+  if (shape) {
+    shape[0] = op->shape[nrhs].rb;
+    shape[1] = op->shape[nrhs].tpb;
+    shape[2] = op->shape[nrhs].occ;
+    shape[3] = op->shape[nrhs].seg;
+  }
+  return WC_OK;
+}
+
+int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_set_hop_shape: null operator");
+  int rc = check_nrhs(op, nrhs, "wc_set_hop_shape");
+  if (rc) return rc;
+  if (!shape) {  // back to automatic
+    op->shape_set[nrhs] = false;
+    return WC_OK;
+  }
+  wck::HopShape sh;
+  sh.rb = shape[0];
+  sh.tpb = shape[1];
+  sh.occ = shape[2];
+  sh.seg = shape[3];
+  if (!shape_valid(op, sh))
+    WC_FAIL(WC_ERR_ARG,
+            "wc_set_hop_shape: rb=%d tpb=%d occ=%d seg=%d invalid (rb 1..%d, tpb 1..%d, occ 0..%d, seg 1 or "
+            "2..%d)",
+            sh.rb, sh.tpb, sh.occ, sh.seg, max_rb_prec(op->prec), wck::kMaxTpb, wck::kMaxOcc, op->bps);
+  op->shape[nrhs] = sh;
+  op->shape_set[nrhs] = true;
+  return WC_OK;
+}
+
+int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape) {
+  if (!op || !shape) WC_FAIL(WC_ERR_ARG, "wc_get_hop_shape: null argument");
+  if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "wc_get_hop_shape: nrhs = %d must be in [1, 64]", nrhs);
+  const wck::HopShape sh = shape_for(op, nrhs);
+  shape[0] = sh.rb;
+  shape[1] = sh.tpb;
+  shape[2] = sh.occ;
+  shape[3] = sh.seg;
+  return WC_OK;
+}
 
 int wc_apply_block(wc_operator *op, int opcode, int nrhs, void *dst, const void *src, int src_cb, void *stream) {
   if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply: null argument");
@@ -755,6 +691,7 @@ int wc_cg_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *x, const voi
   int rc = check_nrhs(op, nrhs, "wc_cg_eo2_ne");
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if ((rc = maybe_tune(op, nrhs, s))) return rc;
   return op->prec == WC_DOUBLE
              ? cg_impl<double>(op, nrhs, parity, x, b, eps, maxiter, history, iterations, converged, r2, b2, s)
              : cg_impl<float>(op, nrhs, parity, x, b, eps, maxiter, history, iterations, converged, r2, b2, s);
@@ -785,6 +722,7 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
   int rc = check_nrhs(op, nrhs, "wc_solve_eo2_ne");
   if (rc) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if ((rc = maybe_tune(op, nrhs, s))) return rc;
   return op->prec == WC_DOUBLE ? solve_impl<double>(op, nrhs, parity, dst, src, eps, maxiter, history, iterations,
                                                     converged, r2, b2, s)
                                : solve_impl<float>(op, nrhs, parity, dst, src, eps, maxiter, history, iterations,
@@ -797,6 +735,7 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
   if (!op || !inner || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: null argument");
   if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: dst must not alias src");
   if (op->prec != WC_DOUBLE) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: the outer operator must be FP64");
+  if (op == inner) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: inner must be a separate operator object");
   for (int mu = 0; mu < 4; ++mu)
     if (op->g.L[mu] != inner->g.L[mu]) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: operators of different lattices");
   if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: parity must be 0 or 1");
@@ -804,6 +743,8 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
     WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: need maxiter >= 0, tol > 0, 0 < delta < 1");
   int rc = check_nrhs(op, nrhs, "wc_solve_eo2_ne_mixed");
   if (rc) return rc;
+  if ((rc = maybe_tune(op, nrhs, (hipStream_t)stream))) return rc;
+  if ((rc = maybe_tune(inner, nrhs, (hipStream_t)stream))) return rc;
   return mixed_impl(op, inner, nrhs, parity, dst, src, tol, maxiter, delta, iterations, converged, true_residual,
                     reliable_updates, (hipStream_t)stream);
 }

@@ -1,0 +1,397 @@
+// Continuous-batching mixed-precision eo2_ne solver (C ABI in
+// include/gpt_amd_wc.h, wc_batch_*).  N slots share one operator pair (FP64
+// outer, inner precision for the CG recursion) and advance together: every
+// inner iteration applies Mpc^dag Mpc to all N slots in one pass of the
+// multi-RHS kernels.  A request enters a free slot whenever one is free
+// (its source terms and initial residual are computed on its own, FP64), and
+// leaves when its FP64 true residual meets its tolerance, so slots are
+// recycled while the others keep iterating.
+//
+// Per-slot state: the FP64 solution y and R src in N-RHS fields, the inner
+// r, p, x, A_D^-1 p in N-RHS fields of the inner precision, and the CG
+// scalars of the slot (st[j * N + slot]).  Reliable updates (y += x, r
+// recomputed in FP64 from y, p kept) happen for all iterating slots at once,
+// when any slot's iterated residual has fallen by delta^2 or a slot reaches
+// its iteration limit; a slot finishes at the reliable update that finds its
+// FP64 true residual within its tolerance.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "wc_internal.h"
+
+namespace {
+
+// slot r of an N-RHS half field <-> a single-RHS half field: element e of
+// the single field is element e * N + r of the N-RHS one (same AoSoA order)
+template <typename CS, typename CD> __device__ __forceinline__ CD cvt(CS v) {
+  CD o;
+  o.x = (decltype(o.x))v.x;
+  o.y = (decltype(o.y))v.y;
+  return o;
+}
+template <typename CS, typename CD> __global__ void k_to_slot(long n, int N, int slot, const CS *src, CD *dst) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) dst[e * N + slot] = cvt<CS, CD>(src[e]);
+}
+template <typename C> __global__ void k_from_slot(long n, int N, int slot, const C *src, C *dst) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) dst[e] = src[e * N + slot];
+}
+template <typename C> __global__ void k_zero_slot(long n, int N, int slot, C *dst) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) {
+    C z;
+    z.x = 0;
+    z.y = 0;
+    dst[e * N + slot] = z;
+  }
+}
+inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+struct wc_batch {
+  wc_operator *hi = nullptr, *lo = nullptr;
+  int N = 0, pd = 1;
+  double delta = 0.1;
+  hipStream_t s = nullptr;
+  // N-RHS half fields: FP64 y (solution of Mpc y = R src), R src, scratch
+  void *xh = nullptr, *bs = nullptr, *mp = nullptr, *rs = nullptr, *rh = nullptr;
+  // N-RHS half fields in the inner precision
+  void *rl = nullptr, *pl = nullptr, *ml = nullptr, *xl = nullptr, *aD = nullptr;
+  std::vector<void *> src;  // per slot: the request's full FP64 source (kept for S src)
+  double *st = nullptr, *host = nullptr, *partials = nullptr;  // CG scalars, reduction partials
+  // host-side slot state
+  std::vector<int> occupied, active, finished, conv;
+  std::vector<long> iters, maxiter;
+  std::vector<double> tol, src2, maxr, tres;
+  long updates = 0;
+};
+
+static size_t hbytes(const wc_operator *op) { return half_bytes(op); }
+
+static void free_batch(wc_batch *b) {
+  if (!b) return;
+  for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh, b->rl, b->pl, b->ml, b->xl, b->aD}) (void)hipFree(q);
+  for (void *q : b->src) (void)hipFree(q);
+  (void)hipFree(b->st);
+  (void)hipFree(b->partials);
+  if (b->host) (void)hipHostFree(b->host);
+  delete b;
+}
+
+// single-RHS FP64 temporaries of the outer operator (tmp[3..8]) and of the
+// inner one (tmp[3..4]) used by admission and extraction
+static int ensure_single(wc_batch *b) {
+  int rc;
+  if ((rc = ensure_tmp(b->hi, 9, 1))) return rc;
+  if ((rc = ensure_tmp(b->lo, 5, 1))) return rc;
+  if ((rc = ensure_red(b->hi, 1))) return rc;
+  return ensure_red(b->lo, 1);
+}
+
+// FP64: y += x_inner for every slot, x_inner = 0, then r_s = b_s - Mpc y,
+// r_h = Mpc^dag r_s, r_inner = r_h; |r_s|^2 and |r_h|^2 per slot to the host
+static int batch_reliable(wc_batch *b) {
+  wc_operator *hi = b->hi, *lo = b->lo;
+  const int N = b->N;
+  const long n = hi->g.Vh * 12 * N;
+  hipStream_t s = b->s;
+  int rc;
+  const bool single = lo->prec == WC_SINGLE;
+  if (single) wck::accumulate_f2d(n, b->xl, b->xh, s);
+  else wck::axpy<double>(n, 1.0, 0.0, b->xl, b->xh, b->xh, s);
+  if (single) wck::zero<float>(n, b->xl, s);
+  else wck::zero<double>(n, b->xl, s);
+  if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, nullptr, b->partials))) return rc;
+  wck::axpy<double>(n, -1.0, 0.0, b->mp, b->bs, b->rs, s);
+  if ((rc = mpc_impl<double>(hi, N, b->pd, 1, b->rh, b->rs, nullptr, s, nullptr, b->partials))) return rc;
+  if (single) wck::convert_d2f(n, b->rh, b->rl, s);
+  else WC_HIP(hipMemcpyAsync(b->rl, b->rh, hbytes(hi) * N, hipMemcpyDeviceToDevice, s));
+  wck::dot_partials_rhs<double>(n, N, b->rs, b->rs, b->partials, nullptr, s);
+  wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
+  wck::dot_partials_rhs<double>(n, N, b->rh, b->rh, b->partials, nullptr, s);
+  wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 9 * N, s);
+  WC_LAUNCHED();
+  WC_HIP(hipMemcpyAsync(b->host + 8 * N, b->st + 8 * N, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  ++b->updates;
+  for (int i = 0; i < N; ++i) {
+    if (!b->active[i]) continue;
+    const double t2 = b->host[8 * N + i], c = b->host[9 * N + i];
+    if (!std::isfinite(t2) || !std::isfinite(c))
+      WC_FAIL(WC_ERR_NUMERIC, "wc_batch: true residual of slot %d became %g", i, t2);
+    b->tres[i] = b->src2[i] > 0 ? std::sqrt(t2 / b->src2[i]) : 0.0;
+    b->maxr[i] = c;
+    if (t2 <= b->tol[i] * b->tol[i] * b->src2[i]) {
+      b->active[i] = 0;
+      b->finished[i] = 1;
+      b->conv[i] = 1;
+    } else if (b->iters[i] >= b->maxiter[i]) {
+      b->active[i] = 0;
+      b->finished[i] = 1;
+      b->conv[i] = 0;
+    }
+  }
+  // c = |r_h|^2 for the slots that go on, active flags (st rows 0 and 5)
+  for (int i = 0; i < N; ++i) {
+    b->host[i] = b->active[i] ? b->host[9 * N + i] : 0.0;
+    b->host[5 * N + i] = b->active[i] ? 1.0 : 0.0;
+  }
+  WC_HIP(hipMemcpyAsync(b->st, b->host, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  WC_HIP(hipMemcpyAsync(b->st + 5 * N, b->host + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  return WC_OK;
+}
+
+extern "C" {
+
+int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots, double delta, void *stream,
+                    wc_batch **out) {
+  if (!op || !inner || !out) WC_FAIL(WC_ERR_ARG, "wc_batch_create: null argument");
+  *out = nullptr;
+  if (op->prec != WC_DOUBLE) WC_FAIL(WC_ERR_ARG, "wc_batch_create: the outer operator must be FP64");
+  if (op == inner) WC_FAIL(WC_ERR_ARG, "wc_batch_create: inner must be a separate operator object");
+  for (int mu = 0; mu < 4; ++mu)
+    if (op->g.L[mu] != inner->g.L[mu]) WC_FAIL(WC_ERR_ARG, "wc_batch_create: operators of different lattices");
+  if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_batch_create: parity must be 0 or 1");
+  if (!(delta > 0.0 && delta < 1.0)) WC_FAIL(WC_ERR_ARG, "wc_batch_create: need 0 < delta < 1");
+  int rc = check_nrhs(op, nslots, "wc_batch_create");
+  if (rc) return rc;
+  wc_batch *b = new wc_batch();
+  b->hi = op;
+  b->lo = inner;
+  b->N = nslots;
+  b->pd = parity;
+  b->delta = delta;
+  b->s = (hipStream_t)stream;
+  const int N = nslots;
+  const size_t hh = hbytes(op) * N, hl = hbytes(inner) * N;
+  auto alloc = [&](void **q, size_t bytes) {
+    if (rc == WC_OK && hipMalloc(q, bytes) != hipSuccess) {
+      g_err = "wc_batch_create: hipMalloc failed";
+      rc = WC_ERR_HIP;
+    }
+  };
+  for (void **q : {&b->xh, &b->bs, &b->mp, &b->rs, &b->rh}) alloc(q, hh);
+  for (void **q : {&b->rl, &b->pl, &b->ml, &b->xl, &b->aD}) alloc(q, hl);
+  b->src.assign(N, nullptr);
+  for (int i = 0; i < N; ++i) alloc(&b->src[i], 2 * hbytes(op));
+  alloc((void **)&b->st, sizeof(double) * 16 * N);
+  alloc((void **)&b->partials, sizeof(double) * std::max(wck::kRedBlocks, wck::hop_blocks(op->g)) * N);
+  if (rc == WC_OK && hipHostMalloc((void **)&b->host, sizeof(double) * 16 * N, hipHostMallocDefault) != hipSuccess) {
+    g_err = "wc_batch_create: hipHostMalloc failed";
+    rc = WC_ERR_HIP;
+  }
+  if (rc == WC_OK) rc = ensure_single(b);
+  if (rc == WC_OK) rc = maybe_tune(op, N, b->s);
+  if (rc == WC_OK) rc = maybe_tune(inner, N, b->s);
+  if (rc != WC_OK) {
+    std::string keep = g_err;
+    free_batch(b);
+    g_err = keep;
+    return rc;
+  }
+  // every slot starts empty: zero fields and scalars
+  const long n = op->g.Vh * 12 * N;
+  hipStream_t s = b->s;
+  for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh}) wck::zero<double>(n, q, s);
+  for (void *q : {b->rl, b->pl, b->ml, b->xl, b->aD}) {
+    if (inner->prec == WC_SINGLE) wck::zero<float>(n, q, s);
+    else wck::zero<double>(n, q, s);
+  }
+  std::memset(b->host, 0, sizeof(double) * 16 * N);
+  if (hipMemcpyAsync(b->st, b->host, sizeof(double) * 16 * N, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    free_batch(b);
+    WC_FAIL(WC_ERR_HIP, "wc_batch_create: initialising the slots failed");
+  }
+  b->occupied.assign(N, 0);
+  b->active.assign(N, 0);
+  b->finished.assign(N, 0);
+  b->conv.assign(N, 0);
+  b->iters.assign(N, 0);
+  b->maxiter.assign(N, 0);
+  b->tol.assign(N, 0.0);
+  b->src2.assign(N, 0.0);
+  b->maxr.assign(N, 0.0);
+  b->tres.assign(N, 0.0);
+  *out = b;
+  return WC_OK;
+}
+
+int wc_batch_destroy(wc_batch *b) {
+  free_batch(b);
+  return WC_OK;
+}
+
+int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter) {
+  if (!b || !src) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: null argument");
+  if (slot < 0 || slot >= b->N) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: slot %d out of range [0, %d)", slot, b->N);
+  if (b->occupied[slot]) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: slot %d is occupied", slot);
+  if (!(tol > 0.0) || maxiter < 0) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: need tol > 0 and maxiter >= 0");
+  wc_operator *hi = b->hi, *lo = b->lo;
+  const int N = b->N, pd = b->pd, pc = 1 - pd;
+  hipStream_t s = b->s;
+  int rc;
+  if ((rc = ensure_single(b))) return rc;
+  const size_t hb = hbytes(hi);
+  const long n1 = hi->g.Vh * 12;
+  void *t1 = hi->tmp[3], *bs1 = hi->tmp[4], *y1 = hi->tmp[5], *mp1 = hi->tmp[6], *rs1 = hi->tmp[7],
+       *rh1 = hi->tmp[8];
+  void *pl1 = lo->tmp[3], *aD1 = lo->tmp[4];
+  auto half = [&](const void *f, int p) { return (const void *)((const char *)f + (size_t)p * hb); };
+  WC_HIP(hipMemcpyAsync(b->src[slot], src, 2 * hb, hipMemcpyDefault, s));
+  // b_s = R src = src_D - Meooe A_C^-1 src_C
+  site_op<double>(hi, 1, pc, 1, half(src, pc), t1, nullptr, s);
+  {
+    HopLaunch a = hop_args<double>(hi, 1, pd, 0, wck::EPI_AXPY, t1, bs1);
+    a.y = half(src, pd);
+    WC_HOP(double, a, s);
+  }
+  // y = L^-1 guess = A_D guess_D, or 0
+  if (guess) site_op<double>(hi, 1, pd, 0, half(guess, pd), y1, nullptr, s);
+  else wck::zero<double>(n1, y1, s);
+  // r_s = b_s - Mpc y ; r_h = Mpc^dag r_s
+  if ((rc = mpc_impl<double>(hi, 1, pd, 0, mp1, y1, nullptr, s))) return rc;
+  wck::axpy<double>(n1, -1.0, 0.0, mp1, bs1, rs1, s);
+  if ((rc = mpc_impl<double>(hi, 1, pd, 1, rh1, rs1, nullptr, s))) return rc;
+  // into the slot: y, b_s; inner r = p = r_h, x = 0, A_D^-1 p
+  k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)y1, (double2 *)b->xh);
+  k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)bs1, (double2 *)b->bs);
+  if (lo->prec == WC_SINGLE) {
+    wck::convert_d2f(n1, rh1, pl1, s);
+    site_op<float>(lo, 1, pd, 1, pl1, aD1, nullptr, s);
+    k_to_slot<float2, float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const float2 *)pl1, (float2 *)b->rl);
+    k_to_slot<float2, float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const float2 *)pl1, (float2 *)b->pl);
+    k_zero_slot<float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (float2 *)b->xl);
+    k_to_slot<float2, float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const float2 *)aD1, (float2 *)b->aD);
+  } else {
+    site_op<double>(lo, 1, pd, 1, rh1, aD1, nullptr, s);
+    k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)rh1, (double2 *)b->rl);
+    k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)rh1, (double2 *)b->pl);
+    k_zero_slot<double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (double2 *)b->xl);
+    k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)aD1, (double2 *)b->aD);
+  }
+  WC_LAUNCHED();
+  // |src|^2 (both parities), |r_s|^2, |r_h|^2
+  double *st1 = hi->st;
+  if ((rc = norm2_rhs<double>(hi, 1, src, st1 + 0, s))) return rc;
+  {
+    // the full field is two half fields: add the second half's norm
+    if ((rc = norm2_rhs<double>(hi, 1, half(src, 1), st1 + 1, s))) return rc;
+  }
+  if ((rc = norm2_rhs<double>(hi, 1, rs1, st1 + 2, s))) return rc;
+  if ((rc = norm2_rhs<double>(hi, 1, rh1, st1 + 3, s))) return rc;
+  double h4[4];
+  WC_HIP(hipMemcpyAsync(h4, st1, sizeof h4, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  const double s2 = h4[0] + h4[1], t2 = h4[2], c = h4[3];
+  if (!std::isfinite(t2) || !std::isfinite(c)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch_admit: initial residual is %g", t2);
+  b->occupied[slot] = 1;
+  b->finished[slot] = 0;
+  b->conv[slot] = 0;
+  b->iters[slot] = 0;
+  b->maxiter[slot] = maxiter;
+  b->tol[slot] = tol;
+  b->src2[slot] = s2;
+  b->maxr[slot] = c;
+  b->tres[slot] = s2 > 0 ? std::sqrt(t2 / s2) : 0.0;
+  const bool done = (t2 <= tol * tol * s2) || maxiter == 0;
+  b->active[slot] = done ? 0 : 1;
+  if (done) {
+    b->finished[slot] = 1;
+    b->conv[slot] = (t2 <= tol * tol * s2) ? 1 : 0;
+  }
+  // the slot's inner CG scalars: c = |r_h|^2, active flag
+  double cs[2] = {done ? 0.0 : c, done ? 0.0 : 1.0};
+  WC_HIP(hipMemcpyAsync(b->st + 0 * N + slot, &cs[0], sizeof(double), hipMemcpyHostToDevice, s));
+  WC_HIP(hipMemcpyAsync(b->st + 5 * N + slot, &cs[1], sizeof(double), hipMemcpyHostToDevice, s));
+  WC_HIP(hipStreamSynchronize(s));
+  return WC_OK;
+}
+
+int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished) {
+  if (!b) WC_FAIL(WC_ERR_ARG, "wc_batch_step: null batch");
+  if (iters < 0) WC_FAIL(WC_ERR_ARG, "wc_batch_step: iters must be >= 0");
+  const int N = b->N;
+  hipStream_t s = b->s;
+  wc_operator *lo = b->lo;
+  int rc;
+  auto count_active = [&]() {
+    int k = 0;
+    for (int i = 0; i < N; ++i) k += b->active[i];
+    return k;
+  };
+  for (int it = 0; it < iters && count_active() > 0; ++it) {
+    rc = lo->prec == WC_SINGLE
+             ? inner_iteration<float>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s, b->st, b->partials)
+             : inner_iteration<double>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s, b->st, b->partials);
+    if (rc) return rc;
+    WC_HIP(hipMemcpyAsync(b->host + 4 * N, b->st + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    WC_HIP(hipStreamSynchronize(s));
+    bool trigger = false;
+    for (int i = 0; i < N; ++i) {
+      if (!b->active[i]) continue;
+      ++b->iters[i];
+      const double cp = b->host[4 * N + i];
+      if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch: residual of slot %d became %g", i, cp);
+      if (cp > b->maxr[i]) b->maxr[i] = cp;
+      if (cp < b->delta * b->delta * b->maxr[i] || b->iters[i] >= b->maxiter[i]) trigger = true;
+    }
+    if (trigger && (rc = batch_reliable(b))) return rc;
+  }
+  int k = 0;
+  for (int i = 0; i < N; ++i) {
+    if (finished) finished[i] = b->finished[i];
+    k += b->finished[i];
+  }
+  if (nfinished) *nfinished = k;
+  return WC_OK;
+}
+
+int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double *true_residual, int *converged) {
+  if (!b || !dst) WC_FAIL(WC_ERR_ARG, "wc_batch_extract: null argument");
+  if (slot < 0 || slot >= b->N) WC_FAIL(WC_ERR_ARG, "wc_batch_extract: slot %d out of range", slot);
+  if (!b->occupied[slot] || !b->finished[slot]) WC_FAIL(WC_ERR_ARG, "wc_batch_extract: slot %d has no finished solve", slot);
+  wc_operator *hi = b->hi;
+  const int N = b->N, pd = b->pd, pc = 1 - pd;
+  hipStream_t s = b->s;
+  int rc;
+  if ((rc = ensure_single(b))) return rc;
+  const size_t hb = hbytes(hi);
+  const long n1 = hi->g.Vh * 12;
+  void *y1 = hi->tmp[5], *t1 = hi->tmp[3];
+  auto half = [&](void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
+  k_from_slot<double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)b->xh, (double2 *)y1);
+  // dst = L y + S src: dst_D = A_D^-1 y ; dst_C = A_C^-1 src_C - A_C^-1 Meooe dst_D
+  site_op<double>(hi, 1, pd, 1, y1, half(dst, pd), nullptr, s);
+  site_op<double>(hi, 1, pc, 1, half(b->src[slot], pc), t1, nullptr, s);
+  {
+    HopLaunch a = hop_args<double>(hi, 1, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
+    a.clov = hi->has_clover ? hi->Ai[pc] : nullptr;
+    a.y = t1;
+    WC_HOP(double, a, s);
+  }
+  WC_LAUNCHED();
+  WC_HIP(hipStreamSynchronize(s));
+  if (iterations) *iterations = b->iters[slot];
+  if (true_residual) *true_residual = b->tres[slot];
+  if (converged) *converged = b->conv[slot];
+  b->occupied[slot] = 0;
+  b->finished[slot] = 0;
+  return WC_OK;
+}
+
+int wc_batch_slots(const wc_batch *b, int *occupied, int *active) {
+  if (!b) WC_FAIL(WC_ERR_ARG, "wc_batch_slots: null batch");
+  for (int i = 0; i < b->N; ++i) {
+    if (occupied) occupied[i] = b->occupied[i];
+    if (active) active[i] = b->active[i];
+  }
+  return WC_OK;
+}
+
+}  // extern "C"

@@ -31,12 +31,24 @@ template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltyp
 }
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+// Products are written with explicit fused multiply-adds and the kernels
+// are compiled with contraction off, so every kernel instantiation (any
+// launch shape, any epilogue) performs the same roundings and gives the
+// same bits.
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-  return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return cmk<C>(fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x));
 }
 // conj(a) * b
 template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
-  return cmk<C>(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+  return cmk<C>(fma(a.x, b.x, a.y * b.y), fma(a.x, b.y, -(a.y * b.x)));
+}
+// acc + a * b
+template <typename C> __device__ __forceinline__ C cmadd(C acc, C a, C b) {
+  return cmk<C>(fma(a.x, b.x, fma(-a.y, b.y, acc.x)), fma(a.x, b.y, fma(a.y, b.x, acc.y)));
+}
+// acc + conj(a) * b
+template <typename C> __device__ __forceinline__ C cmaddc(C acc, C a, C b) {
+  return cmk<C>(fma(a.x, b.x, fma(a.y, b.y, acc.x)), fma(a.x, b.y, fma(-a.y, b.x, acc.y)));
 }
 template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) {
   return cmk<C>(a.x * s, a.y * s);

@@ -1,0 +1,343 @@
+// The even-odd hop (Meooe) kernel of the Wilson-clover operator for gfx950,
+// with fused epilogues.  Instantiated once per precision (wc_hop_f64.hip,
+// wc_hop_f32.hip).
+//
+// A work item is a site and RB of its right-hand sides.  For each of the 8
+// directions an item gathers the
+// neighbour spinors of its RB right-hand sides, projects each to a half
+// spinor with (1 -+ gamma_mu), multiplies by the double-stored link (9
+// complex, or 6 with the third row rebuilt), loaded and scaled once for the
+// RB spinors, and accumulates the reconstructed 4-spinors.  Epilogues fuse
+// the clover (inverse) block, the Schur complement's axpy and the CG inner
+// product, so the normal operator needs no extra passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "wc_kernels.h"
+#include "wc_site.h"
+
+namespace wck {
+
+// spin projection h = (1 - s gamma_mu) psi restricted to the upper spins, for
+// gpt's chiral basis (gamma_mu = [[0, X_mu], [X_mu^dag, 0]]):
+//   mu=0: h0 = p0 - s i p3, h1 = p1 - s i p2      mu=1: h0 = p0 + s p3, h1 = p1 - s p2
+//   mu=2: h0 = p0 - s i p2, h1 = p1 + s i p3      mu=3: h0 = p0 - s p2, h1 = p1 - s p3
+template <int MU, int S, typename C>
+__device__ __forceinline__ void project(const C p0, const C p1, const C p2, const C p3, C &h0, C &h1) {
+  if (MU == 0) {
+    h0 = S > 0 ? csub(p0, cimul(p3)) : cadd(p0, cimul(p3));
+    h1 = S > 0 ? csub(p1, cimul(p2)) : cadd(p1, cimul(p2));
+  } else if (MU == 1) {
+    h0 = S > 0 ? cadd(p0, p3) : csub(p0, p3);
+    h1 = S > 0 ? csub(p1, p2) : cadd(p1, p2);
+  } else if (MU == 2) {
+    h0 = S > 0 ? csub(p0, cimul(p2)) : cadd(p0, cimul(p2));
+    h1 = S > 0 ? cadd(p1, cimul(p3)) : csub(p1, cimul(p3));
+  } else {
+    h0 = S > 0 ? csub(p0, p2) : cadd(p0, p2);
+    h1 = S > 0 ? csub(p1, p3) : cadd(p1, p3);
+  }
+}
+
+// lower spins of (1 - s gamma_mu) psi from chi = U h: -s X_mu^dag chi
+//   mu=0: l2 = s i c1, l3 = s i c0      mu=1: l2 = -s c1, l3 = s c0
+//   mu=2: l2 = s i c0, l3 = -s i c1     mu=3: l2 = -s c0, l3 = -s c1
+template <int MU, int S, typename C>
+__device__ __forceinline__ void accum_lower(C &a2, C &a3, const C c0, const C c1) {
+  if (MU == 0) {
+    a2 = S > 0 ? cadd(a2, cimul(c1)) : csub(a2, cimul(c1));
+    a3 = S > 0 ? cadd(a3, cimul(c0)) : csub(a3, cimul(c0));
+  } else if (MU == 1) {
+    a2 = S > 0 ? csub(a2, c1) : cadd(a2, c1);
+    a3 = S > 0 ? cadd(a3, c0) : csub(a3, c0);
+  } else if (MU == 2) {
+    a2 = S > 0 ? cadd(a2, cimul(c0)) : csub(a2, cimul(c0));
+    a3 = S > 0 ? csub(a3, cimul(c1)) : cadd(a3, cimul(c1));
+  } else {
+    a2 = S > 0 ? csub(a2, c0) : cadd(a2, c0);
+    a3 = S > 0 ? csub(a3, c1) : cadd(a3, c1);
+  }
+}
+
+// direction (MU, FB): the hop coefficient (and the boundary phase on links
+// that wrap around) is folded into the link once, then every spinor is
+// projected, multiplied and accumulated
+template <typename R, int NL, int DAG, int MU, int FB, int RB>
+__device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], typename c2<R>::t u[9],
+                                            const typename c2<R>::t psi[RB][12],
+                                            typename c2<R>::t acc[RB][12]) {
+  using C = typename c2<R>::t;
+  const Geom &g = a.g;
+  const bool bnd = (FB == 0) ? (c[MU] == g.L[MU] - 1) : (c[MU] == 0);
+  constexpr int S = ((FB == 0) != (DAG != 0)) ? 1 : -1;
+  if (NL == 6) {
+    // third row = conj(row0 x row1) for SU(3)
+    u[6] = cconj(csub(cmul(u[1], u[5]), cmul(u[2], u[4])));
+    u[7] = cconj(csub(cmul(u[2], u[3]), cmul(u[0], u[5])));
+    u[8] = cconj(csub(cmul(u[0], u[4]), cmul(u[1], u[3])));
+  }
+  const R coef = (R)(MU == 3 ? a.ct : a.cs);
+  if (bnd) {
+    C f = cmk<C>((R)(a.ph[MU][0]), (R)(FB == 0 ? a.ph[MU][1] : -a.ph[MU][1]));
+    f = cscale(f, coef);
+#pragma unroll
+    for (int e = 0; e < 9; ++e) u[e] = cmul(f, u[e]);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) u[e] = cscale(u[e], coef);
+  }
+#pragma unroll
+  for (int j = 0; j < RB; ++j) {
+    C hs[2][3];
+#pragma unroll
+    for (int col = 0; col < 3; ++col)
+      project<MU, S>(psi[j][0 + col], psi[j][3 + col], psi[j][6 + col], psi[j][9 + col], hs[0][col],
+                     hs[1][col]);
+    C ch[2][3];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        C t = cmul(u[3 * i + 0], hs[s][0]);
+        t = cmadd(t, u[3 * i + 1], hs[s][1]);
+        t = cmadd(t, u[3 * i + 2], hs[s][2]);
+        ch[s][i] = t;
+      }
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+      acc[j][0 + col] = cadd(acc[j][0 + col], ch[0][col]);
+      acc[j][3 + col] = cadd(acc[j][3 + col], ch[1][col]);
+      accum_lower<MU, S>(acc[j][6 + col], acc[j][9 + col], ch[0][col], ch[1][col]);
+    }
+  }
+}
+
+// The 8 directions run as a real loop (not unrolled): the loads of one
+// direction cannot be hoisted into another, so the live registers stay at one
+// direction's operands plus the accumulators; the other waves on the SIMD
+// keep HBM busy meanwhile.  The projection variant is picked by a
+// wave-uniform switch.  Neighbour offsets come from uniform strides of the
+// half-site index h = ((t*Lz + z)*Ly + y)*Lxh + x/2.  rl[j] is the
+// right-hand side a spinor is read for (clamped into range for padded lanes).
+template <typename R, int NL, int DAG, int RB>
+__device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int rl[RB], const int cx,
+                                         const int cy, const int cz, const int ct,
+                                         typename c2<R>::t acc[RB][12]) {
+  // coordinates as scalars: a select among array elements would become a
+  // dynamically indexed (scratch) array
+  const int c[4] = {cx, cy, cz, ct};
+  using C = typename c2<R>::t;
+  const Geom &g = a.g;
+  const uint32_t N = (uint32_t)a.nrhs;
+  const uint32_t hh = (uint32_t)h;
+  const wc_rsrc in = make_rsrc(a.in, 12u * (uint32_t)g.Vh * N * (uint32_t)sizeof(C));
+  // a wave's sites are consecutive and lie in at most two aligned 64-site
+  // blocks: its links are addressed from the first active lane's block
+  const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
+  const wc_rsrc rw = make_rsrc((const C *)a.W + blk * (8 * NL * kBlk), 2u * 8u * NL * kBlk * (uint32_t)sizeof(C));
+  const uint32_t vw = (uint32_t)(aidx_site(h, 8 * NL) - blk * (8 * NL * kBlk)) * (uint32_t)sizeof(C);
+  const uint32_t kstride = (uint32_t)kBlk * N * (uint32_t)sizeof(C);
+  const uint32_t xh = (uint32_t)cx >> 1;
+  const uint32_t st1 = (uint32_t)g.Lxh, st2 = st1 * (uint32_t)g.L[1], st3 = st2 * (uint32_t)g.L[2];
+#pragma unroll 1
+  for (int dir = 0; dir < 8; ++dir) {
+    const int mu = dir >> 1, fb = dir & 1;
+    uint32_t nb;
+    if (mu == 0) {
+      const int xn = fb == 0 ? (cx == g.L[0] - 1 ? 0 : cx + 1) : (cx == 0 ? g.L[0] - 1 : cx - 1);
+      nb = hh - xh + ((uint32_t)xn >> 1);
+    } else {
+      const uint32_t stride = mu == 1 ? st1 : (mu == 2 ? st2 : st3);
+      const int cm = mu == 1 ? cy : (mu == 2 ? cz : ct);
+      const int Lm = mu == 1 ? g.L[1] : (mu == 2 ? g.L[2] : g.L[3]);
+      const uint32_t wrap = (uint32_t)(Lm - 1) * stride;
+      if (fb == 0)
+        nb = (cm == Lm - 1) ? hh - wrap : hh + stride;
+      else
+        nb = (cm == 0) ? hh + wrap : hh - stride;
+    }
+    C u[9];
+    const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
+#pragma unroll
+    for (int e = 0; e < NL; ++e) u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
+    const uint32_t vs = (uint32_t)aidx_site(nb, 12) * N;
+    C psi[RB][12];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(C);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) psi[j][k] = bload<C, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride);
+    }
+    switch (dir) {
+      case 0: compute_dir<R, NL, DAG, 0, 0, RB>(a, c, u, psi, acc); break;
+      case 1: compute_dir<R, NL, DAG, 0, 1, RB>(a, c, u, psi, acc); break;
+      case 2: compute_dir<R, NL, DAG, 1, 0, RB>(a, c, u, psi, acc); break;
+      case 3: compute_dir<R, NL, DAG, 1, 1, RB>(a, c, u, psi, acc); break;
+      case 4: compute_dir<R, NL, DAG, 2, 0, RB>(a, c, u, psi, acc); break;
+      case 5: compute_dir<R, NL, DAG, 2, 1, RB>(a, c, u, psi, acc); break;
+      case 6: compute_dir<R, NL, DAG, 3, 0, RB>(a, c, u, psi, acc); break;
+      default: compute_dir<R, NL, DAG, 3, 1, RB>(a, c, u, psi, acc); break;
+    }
+  }
+}
+
+// Visiting order of the 64-site blocks, seg > 1: the bps blocks of a time
+// slice are cut into seg runs of consecutive blocks (whole z-y rows), and a
+// run is visited for t = 0, 1, ... before the next run, so the +-t
+// neighbours of a block were read a few blocks earlier and are still in L2.
+// A bijection of [0, nb) for any 1 < seg <= bps (nb = bps * lt).
+__device__ __forceinline__ long block_order(long T, int seg, int bps, int lt) {
+  if (seg <= 1) return T;
+  // run s holds slice blocks [lo(s), lo(s+1)), lo(s) = s * bps / seg
+  const long u = T / lt;
+  long s = u * seg / bps;
+  while ((s + 1) * bps / seg <= u) ++s;
+  const long lo = s * bps / seg, len = (s + 1) * bps / seg - lo;
+  const long off = T - (long)lt * lo;
+  const long t = off / len;
+  return t * bps + lo + (off - t * len);
+}
+
+// waves per SIMD the register allocation must allow: 4 for the FP32 hop
+// with one right-hand side per thread (<= 128 VGPRs), else 2 (<= 256; keeps
+// the compiler from parking accumulators in AGPRs at 1 wave)
+template <typename R, int RB> constexpr int hop_min_waves() { return (sizeof(R) == 4 && RB == 1) ? 4 : 2; }
+
+// Work items are (site h, right-hand-side group g) with g fastest, g covering
+// the RB right-hand sides g*RB .. g*RB+RB-1, one item per thread over the
+// whole lattice: consecutive lanes read consecutive right-hand sides of a
+// site and then of the next site, so every gather is one contiguous run and
+// the sites in flight stay few (their neighbours stay in L2).  EPI_DOT
+// writes each (site, right-hand side) product to dots[h * N + r]; dot_blocks
+// sums them per 64-site block in site order, so the sums do not depend on
+// the launch shape.
+template <typename R, int NL, int DAG, int EPI, int RB>
+__global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const HopLaunch a) {
+  using C = typename c2<R>::t;
+  const Geom &g = a.g;
+  const int N = a.nrhs;
+  const int Gt = (N + RB - 1) / RB;
+  const long q = xcd_tile(blockIdx.x, gridDim.x) * (long)blockDim.x + threadIdx.x;
+  const long v = q / Gt;  // site in visiting order
+  if (v >= g.Vh) return;
+  const int rb = (int)(q - v * Gt) * RB;
+  // a 64-site block has 64 * Gt items, a whole number of waves: a wave's
+  // sites lie in one block, and blocks can be visited in any order
+  const long h = a.shape.seg > 1 ? block_order(v >> 6, a.shape.seg, a.bps, g.L[3]) * kBlk + (v & 63) : v;
+  int rl[RB];
+#pragma unroll
+  for (int j = 0; j < RB; ++j) rl[j] = min(rb + j, N - 1);
+  int c[4];
+  half_coords(g, h, a.pout, c);
+  C acc[RB][12];
+#pragma unroll
+  for (int j = 0; j < RB; ++j)
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
+  hop_site<R, NL, DAG, RB>(a, h, rl, c[0], c[1], c[2], c[3], acc);
+  // epilogue
+  const bool zb = a.open_bc && (c[3] == 0 || c[3] == g.L[3] - 1);
+  if (zb) {
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+#pragma unroll
+      for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
+  } else if (EPI & EPI_CLOV) {
+    if (a.clov) {
+      clov_apply_inplace<R, RB>((const C *)a.clov, h, acc);
+    } else {
+#pragma unroll
+      for (int j = 0; j < RB; ++j)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[j][k] = cscale(acc[j][k], (R)a.clov_scalar);
+    }
+  }
+  if (EPI & (EPI_DIAG | EPI_AXPY)) {
+    C y[RB][12];
+#pragma unroll
+    for (int j = 0; j < RB; ++j) load12((const C *)a.y, h, rl[j], N, y[j]);
+    if (EPI & EPI_DIAG) {
+      if (!zb) {
+        if (a.diag_clov) {
+          clov_apply_inplace<R, RB>((const C *)a.diag_clov, h, y);
+        } else {
+#pragma unroll
+          for (int j = 0; j < RB; ++j)
+#pragma unroll
+            for (int k = 0; k < 12; ++k) y[j][k] = cscale(y[j][k], (R)a.diag);
+        }
+#pragma unroll
+        for (int j = 0; j < RB; ++j)
+#pragma unroll
+          for (int k = 0; k < 12; ++k) acc[j][k] = cadd(acc[j][k], y[j][k]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < RB; ++j)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[j][k] = csub(y[j][k], acc[j][k]);
+    }
+  }
+  if (EPI & EPI_DOT) {
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      C v[12];
+      load12((const C *)a.dotv, h, rl[j], N, v);
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < 12; ++k)
+        s = fma((double)v[k].y, (double)acc[j][k].y, fma((double)v[k].x, (double)acc[j][k].x, s));
+      if (rb + j < N) a.dots[h * N + rb + j] = s;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < RB; ++j)
+    if (rb + j < N) store12_stream((C *)a.out, g.Vh, h, rb + j, N, acc[j]);
+}
+
+template <typename R, int NL, int DAG, int RB>
+static void hop_epi(const HopLaunch &a, hipStream_t s) {
+  const long threads = kBlk * a.shape.tpb;
+  const long items = a.g.Vh * (long)((a.nrhs + RB - 1) / RB);
+  const unsigned grid = (unsigned)((items + threads - 1) / threads);
+  // LDS reserved only to cap the workgroups per CU (the kernel uses none)
+  const size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
+  switch (a.epi) {
+#define WC_EPI(E) \
+  case E: k_hop<R, NL, DAG, E, RB><<<grid, (unsigned)threads, lds, s>>>(a); break;
+    WC_EPI(0)
+    WC_EPI(EPI_CLOV)
+    WC_EPI(EPI_DIAG)
+    WC_EPI(EPI_AXPY)
+    WC_EPI(EPI_CLOV | EPI_AXPY)
+    WC_EPI(EPI_CLOV | EPI_AXPY | EPI_DOT)
+    WC_EPI(EPI_AXPY | EPI_DOT)
+#undef WC_EPI
+    default: break;
+  }
+}
+
+template <typename R, int NL, int DAG>
+static void hop_rb(const HopLaunch &a, hipStream_t s) {
+  switch (a.shape.rb) {
+    case 1: hop_epi<R, NL, DAG, 1>(a, s); break;
+    case 2: if constexpr (max_rb<R>() >= 2) hop_epi<R, NL, DAG, 2>(a, s); break;
+    default: break;
+  }
+}
+
+template <typename R> bool hop_launch(const HopLaunch &a, hipStream_t s) {
+  const HopShape &sh = a.shape;
+  if (sh.tpb < 1 || sh.tpb > kMaxTpb || sh.rb < 1 || sh.rb > max_rb<R>() || sh.occ < 0 || sh.occ > kMaxOcc ||
+      a.nrhs < 1 || a.nrhs > 64 || a.g.Vh % kBlk)
+    return false;
+  if (sh.seg < 1 || (sh.seg > 1 && (a.bps <= 0 || sh.seg > a.bps))) return false;
+  if (a.nl == 9) {
+    if (a.dag) hop_rb<R, 9, 1>(a, s); else hop_rb<R, 9, 0>(a, s);
+  } else {
+    if (a.dag) hop_rb<R, 6, 1>(a, s); else hop_rb<R, 6, 0>(a, s);
+  }
+  return true;
+}
+
+}  // namespace wck

@@ -1,0 +1,312 @@
+// Internal host-side state and compositions shared by the C-ABI sources
+// (wc_api.hip, wc_batch.hip).  Not exported.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/gpt_amd_wc.h"
+#include "wc_kernels.h"
+
+using wck::HopLaunch;
+
+extern thread_local std::string g_err;  // message of the last failure (wc_api.hip)
+
+#define WC_FAIL(code, ...)                       \
+  do {                                           \
+    char _b[512];                                \
+    snprintf(_b, sizeof _b, __VA_ARGS__);        \
+    g_err = _b;                                  \
+    return code;                                 \
+  } while (0)
+
+#define WC_HIP(x)                                                                                    \
+  do {                                                                                               \
+    hipError_t _e = (x);                                                                             \
+    if (_e != hipSuccess)                                                                            \
+      WC_FAIL(WC_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e), __FILE__, __LINE__); \
+  } while (0)
+
+// launches are asynchronous: surface launch-configuration errors right away
+#define WC_LAUNCHED() WC_HIP(hipGetLastError())
+
+struct wc_operator {
+  wc_params p{};
+  Geom g{};
+  int prec = WC_DOUBLE;
+  int nl = 9;
+  size_t csz = 16;  // bytes per complex value
+  double mass = 0, diag = 0, cs = 0, ct = -0.5;
+  bool has_clover = false, open_bc = false;
+  void *W[2] = {nullptr, nullptr};
+  void *A[2] = {nullptr, nullptr};
+  void *Ai[2] = {nullptr, nullptr};
+  void *Unat = nullptr;  // FP64 links [4][V][9], kept to rebuild the clover on set_mass
+  int *err = nullptr;    // device flag of the clover build
+  std::vector<void *> tmp;      // half-field scratch, tmp_bytes each
+  size_t tmp_bytes = 0;
+  double *partials = nullptr;   // per-block reduction partials
+  size_t npartials = 0;
+  double *dots = nullptr;       // per-(site, RHS) products of the hop's EPI_DOT
+  size_t ndots = 0;
+  double *st = nullptr;         // device scalars (CG state), nst doubles
+  double *host = nullptr;       // pinned host mirror, nst doubles
+  size_t nst = 0;
+  int bps = 0;                  // 64-site blocks per time slice (0: not a whole number)
+  wck::HopShape shape[65];      // hop launch shape per right-hand-side count
+  bool shape_set[65] = {};      // shape chosen (by the tuner or the caller)
+};
+
+inline size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
+
+// n scratch half fields of nrhs right-hand sides each
+inline int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
+  const size_t bytes = half_bytes(op) * (size_t)nrhs;
+  if (bytes > op->tmp_bytes) {
+    for (void *q : op->tmp) (void)hipFree(q);
+    op->tmp.clear();
+    op->tmp_bytes = bytes;
+  }
+  while (op->tmp.size() < n) {
+    void *q = nullptr;
+    WC_HIP(hipMalloc(&q, op->tmp_bytes));
+    op->tmp.push_back(q);
+  }
+  return WC_OK;
+}
+
+// reduction partials and CG scalars for nrhs right-hand sides
+inline int ensure_red(wc_operator *op, int nrhs) {
+  const size_t np = (size_t)std::max(wck::kRedBlocks, wck::hop_blocks(op->g)) * nrhs;
+  if (np > op->npartials) {
+    (void)hipFree(op->partials);
+    op->partials = nullptr;
+    WC_HIP(hipMalloc((void **)&op->partials, sizeof(double) * np));
+    op->npartials = np;
+  }
+  const size_t nd = (size_t)op->g.Vh * nrhs;
+  if (nd > op->ndots) {
+    (void)hipFree(op->dots);
+    op->dots = nullptr;
+    WC_HIP(hipMalloc((void **)&op->dots, sizeof(double) * nd));
+    op->ndots = nd;
+  }
+  const size_t ns = (size_t)16 * nrhs;
+  if (ns > op->nst) {
+    (void)hipFree(op->st);
+    if (op->host) (void)hipHostFree(op->host);
+    op->st = nullptr;
+    op->host = nullptr;
+    WC_HIP(hipMalloc((void **)&op->st, sizeof(double) * ns));
+    WC_HIP(hipHostMalloc((void **)&op->host, sizeof(double) * ns, hipHostMallocDefault));
+    op->nst = ns;
+  }
+  return WC_OK;
+}
+
+inline void free_op(wc_operator *op) {
+  if (!op) return;
+  for (int p = 0; p < 2; ++p) {
+    (void)hipFree(op->W[p]);
+    (void)hipFree(op->A[p]);
+    (void)hipFree(op->Ai[p]);
+  }
+  (void)hipFree(op->Unat);
+  (void)hipFree(op->err);
+  for (void *q : op->tmp) (void)hipFree(q);
+  (void)hipFree(op->partials);
+  (void)hipFree(op->dots);
+  (void)hipFree(op->st);
+  if (op->host) (void)hipHostFree(op->host);
+  delete op;
+}
+
+// ------------------------------------------------------- hop launch shapes
+template <typename R> inline int max_rb_of() { return wck::max_rb<R>(); }
+inline int max_rb_prec(int prec) { return prec == WC_DOUBLE ? max_rb_of<double>() : max_rb_of<float>(); }
+
+// Shape used until the tuner (or the caller) picks one: rb right-hand sides
+// per thread with the least padding (ties: the larger rb), 64 threads.
+inline wck::HopShape default_shape(const wc_operator *op, int nrhs) {
+  wck::HopShape sh;
+  if (nrhs <= 1) return sh;
+  const int mrb = max_rb_prec(op->prec);
+  int best = 1, waste_best = 1 << 30;
+  for (int rb = mrb; rb >= 1; --rb) {
+    const int waste = ((nrhs + rb - 1) / rb) * rb - nrhs;
+    if (waste < waste_best) {
+      waste_best = waste;
+      best = rb;
+    }
+  }
+  sh.rb = best;
+  return sh;
+}
+
+inline bool shape_valid(const wc_operator *op, const wck::HopShape &sh) {
+  return sh.tpb >= 1 && sh.tpb <= wck::kMaxTpb && sh.rb >= 1 && sh.rb <= max_rb_prec(op->prec) && sh.occ >= 0 &&
+         sh.occ <= wck::kMaxOcc && sh.seg >= 1 && (sh.seg == 1 || (op->bps > 0 && sh.seg <= op->bps));
+}
+
+// process-wide cache of tuned shapes: (dims, precision, link format, nrhs)
+typedef std::tuple<int, int, int, int, int, int, int> ShapeKey;
+extern std::mutex g_shape_mu;
+extern std::map<ShapeKey, wck::HopShape> g_shape_cache;
+inline ShapeKey shape_key(const wc_operator *op, int nrhs) {
+  return ShapeKey(op->g.L[0], op->g.L[1], op->g.L[2], op->g.L[3], op->prec, op->nl, nrhs);
+}
+
+inline wck::HopShape shape_for(const wc_operator *op, int nrhs) {
+  if (nrhs >= 1 && nrhs <= 64 && op->shape_set[nrhs]) return op->shape[nrhs];
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shape_cache.find(shape_key(op, nrhs));
+    if (it != g_shape_cache.end()) return it->second;
+  }
+  return default_shape(op, nrhs);
+}
+
+// ------------------------------------------------------------ compositions
+template <typename R>
+inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, int epi, const void *in, void *out) {
+  HopLaunch a{};
+  a.g = op->g;
+  a.nrhs = nrhs;
+  a.pout = pout;
+  a.dag = dag;
+  a.nl = op->nl;
+  a.epi = epi;
+  a.in = in;
+  a.out = out;
+  a.W = op->W[pout];
+  a.cs = op->cs;
+  a.ct = op->ct;
+  for (int mu = 0; mu < 4; ++mu) {
+    a.ph[mu][0] = op->p.boundary_phases[mu][0];
+    a.ph[mu][1] = op->p.boundary_phases[mu][1];
+  }
+  a.open_bc = op->open_bc ? 1 : 0;
+  a.diag = op->diag;
+  a.clov_scalar = 1.0 / op->diag;
+  a.shape = shape_for(op, nrhs);
+  a.bps = op->bps;
+  return a;
+}
+
+// launch a hop; an invalid shape is an error, never a silent no-op
+#define WC_HOP(R, a, s)                                                                        \
+  do {                                                                                         \
+    if (!wck::hop<R>((a), (s)))                                                                \
+      WC_FAIL(WC_ERR_ARG, "hop: launch shape rb=%d tpb=%d occ=%d seg=%d is outside the limits",   \
+              (a).shape.rb, (a).shape.tpb, (a).shape.occ, (a).shape.seg);                      \
+  } while (0)
+
+// out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
+template <typename R>
+inline void site_op(const wc_operator *op, int nrhs, int p, int inv, const void *in, void *out, const void *y,
+                    hipStream_t s) {
+  const void *B = op->has_clover ? (inv ? op->Ai[p] : op->A[p]) : nullptr;
+  double sc = inv ? 1.0 / op->diag : op->diag;
+  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s);
+}
+
+// Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd on nrhs
+// right-hand sides; dotv (optional) makes the last hop write per-block,
+// per-RHS partials of Re<dotv, out>.  tD_in (optional, which 0 or 2): a
+// field already holding A_D^-1 in.  Scratch: tmp[0..2].
+template <typename R>
+inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
+                    hipStream_t s, const void *tD_in = nullptr, double *partials = nullptr) {
+  if (!partials) partials = op->partials;
+  const int pc = 1 - pd;
+  int rc = ensure_tmp(op, 3, nrhs);
+  if (rc) return rc;
+  if ((rc = ensure_red(op, nrhs))) return rc;
+  void *tD = op->tmp[0], *tC = op->tmp[1], *tY = op->tmp[2];
+  const void *clovC = op->has_clover ? op->Ai[pc] : nullptr;
+  const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
+  const void *src = in;
+  void *dst_mpc = (which == 2) ? tY : out;
+  if (which == 0 || which == 2) {
+    const void *aD = tD_in;
+    if (!aD) {
+      site_op<R>(op, nrhs, pd, 1, src, tD, nullptr, s);
+      aD = tD;
+    }
+    HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV, aD, tC);
+    a.clov = clovC;
+    WC_HOP(R, a, s);
+    HopLaunch b = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC,
+                              dst_mpc);
+    b.y = src;
+    b.dotv = dotv;
+    b.dots = op->dots;
+    WC_HOP(R, b, s);
+    if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s);
+    src = dst_mpc;
+  }
+  if (which == 1 || which == 2) {
+    HopLaunch a = hop_args<R>(op, nrhs, pc, 1, wck::EPI_CLOV, src, tC);
+    a.clov = clovC;
+    WC_HOP(R, a, s);
+    HopLaunch b = hop_args<R>(op, nrhs, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC,
+                              out);
+    b.clov = clovD;
+    b.y = src;
+    b.dotv = dotv;
+    b.dots = op->dots;
+    WC_HOP(R, b, s);
+    if (dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s);
+  }
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+// per-RHS |x|^2 into dev_out[0..nrhs)
+template <typename R>
+inline int norm2_rhs(wc_operator *op, int nrhs, const void *x, double *dev_out, hipStream_t s) {
+  int rc = ensure_red(op, nrhs);
+  if (rc) return rc;
+  const long n = op->g.Vh * 12 * nrhs;
+  wck::dot_partials_rhs<R>(n, nrhs, x, x, op->partials, nullptr, s);
+  wck::reduce_final_rhs(op->partials, wck::kRedBlocks, nrhs, dev_out, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+// one iteration of the inner CG on lo: mmp = A p (A_D^-1 p in aD), alpha,
+// r -= alpha mmp, beta, x += alpha p, p = beta p + r, aD = A_D^-1 p
+template <typename R>
+inline int inner_iteration(wc_operator *lo, int N, int pd, void *r, void *p, void *mmp, void *x, void *aD,
+                           hipStream_t s, double *st = nullptr, double *partials = nullptr) {
+  int rc;
+  const long n = lo->g.Vh * 12 * N;
+  if (!st) st = lo->st;
+  if (!partials) partials = lo->partials;
+  if ((rc = mpc_impl<R>(lo, N, pd, 2, mmp, p, p, s, aD, partials))) return rc;
+  wck::cg_alpha(partials, wck::hop_blocks(lo->g), N, st, s);
+  wck::cg_update_r<R>(n, N, st, r, mmp, partials, s);
+  wck::cg_beta(partials, wck::kRedBlocks, N, st, s);
+  wck::cg_update_xp<R>(lo->g, N, pd, st, r, p, x, lo->has_clover ? lo->Ai[pd] : nullptr, 1.0 / lo->diag, aD,
+                       lo->open_bc ? 1 : 0, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+inline int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
+  if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);
+  if ((double)half_bytes(op) * nrhs >= 4294967296.0)
+    WC_FAIL(WC_ERR_ARG, "%s: a half field of %d right-hand sides exceeds the 4 GB gather window", fn, nrhs);
+  return WC_OK;
+}
+
+
+// tunes the hop of nrhs right-hand sides on first use (wc_api.hip)
+int maybe_tune(wc_operator *op, int nrhs, hipStream_t s);

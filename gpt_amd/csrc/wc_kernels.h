@@ -12,8 +12,27 @@ enum : int {
   EPI_CLOV = 1,  // res = A res       (A = packed blocks of the output parity)
   EPI_DIAG = 2,  // res = res + D y   (D = packed blocks or the scalar diag)
   EPI_AXPY = 4,  // res = y - res
-  EPI_DOT = 8,   // partial[block] = Re <dotv, res>
+  EPI_DOT = 8,   // dots[h * nrhs + r] = Re <dotv, res> at site h (then dot_blocks)
 };
+
+// Launch shape of the hop: a thread handles rb right-hand sides of a site,
+// 64 * tpb threads per workgroup, and with occ > 0 each workgroup reserves
+// 1/occ of a CU's LDS so that at most occ workgroups share a CU (a smaller
+// window of sites in flight, whose neighbours then stay in L2); seg > 1
+// visits the 64-site blocks in t-marching order.  Every shape gives bitwise
+// the same result.
+struct HopShape {
+  int rb = 1;
+  int tpb = 1;
+  int occ = 0;
+  int seg = 1;  // > 1: t-marching order, seg runs per time slice (block_order)
+};
+constexpr int kLdsPerCU = 160 * 1024;
+constexpr int kMaxOcc = 32;
+constexpr int kMaxTpb = 8;  // up to 512 threads per workgroup
+// largest RB instantiated for the precision (beyond it the accumulators and
+// one direction's operands exceed 256 VGPRs and spill)
+template <typename R> constexpr int max_rb() { return sizeof(R) == 8 ? 1 : 2; }
 
 struct HopLaunch {
   Geom g;
@@ -31,22 +50,30 @@ struct HopLaunch {
   double diag;       // scalar diagonal when diag_clov is null
   const void *y;     // EPI_DIAG / EPI_AXPY operand (parity pout)
   const void *dotv;  // EPI_DOT left vector (parity pout)
-  double *partial;   // EPI_DOT per-block partial sums
+  double *dots;      // EPI_DOT per-(site, right-hand side) products, Vh * nrhs
   double cs;         // spatial hop coefficient -c_s/2
   double ct;         // temporal hop coefficient -1/2
   double ph[4][2];   // boundary phases
   int open_bc;
+  HopShape shape;    // launch shape (1 <= tpb <= kMaxTpb, 1 <= rb <= max_rb, 0 <= occ <= kMaxOcc)
+  int bps;           // 64-site blocks per time slice (0: not a whole number; then seg must be 1)
 };
 
-// number of blocks a hop / site launch uses for Vh sites
+// number of blocks a site launch uses for Vh sites
 int site_blocks(long Vh);
+// number of 64-site blocks (the per-RHS partial count of dot_blocks)
+inline int hop_blocks(const Geom &g) { return (int)(g.Vh / 64); }
+// partial[r * (Vh / 64) + b] = sum over the 64 sites of block b, in site
+// order, of dots[h * nrhs + r]
+void dot_blocks(const Geom &g, int nrhs, const double *dots, double *partial, hipStream_t s);
 #ifndef WC_SITE_THREADS
 #define WC_SITE_THREADS 64
 #endif
 constexpr int kSiteThreads = WC_SITE_THREADS;
 constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
 
-template <typename R> void hop(const HopLaunch &a, hipStream_t s);
+// returns false (nothing launched) for a shape outside the limits above
+template <typename R> bool hop(const HopLaunch &a, hipStream_t s);
 
 // out = A in (blocks != null) or diag * in on parity p, for nrhs right-hand
 // sides; zeroes the open boundary; optional axpy (out = y - A in)
@@ -55,7 +82,7 @@ void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag
                  const void *y, int open_bc, hipStream_t s);
 
 // Per-right-hand-side reductions over fields whose innermost index is the
-// RHS (nrhs of them): partials[b * nrhs + r] for b < kRedBlocks, then
+// RHS (nrhs of them): partials[r * kRedBlocks + b] for b < kRedBlocks, then
 // out[r] = sum over b in fixed order.
 template <typename R>
 void dot_partials_rhs(long n, int nrhs, const void *a, const void *b, double *pre, double *pim, hipStream_t s);
@@ -109,13 +136,17 @@ void build_clover(const Geom &g, const void *Unat, const CloverParams &cp, void 
                   void *Ai0, void *Ai1, int *err_flag, hipStream_t s);
 
 // CG on nrhs independent right-hand sides with device scalars:
-// st[j * nrhs + r], j = 0 c, 1 d, 2 alpha, 3 beta, 4 cp, 5 active (1/0)
+// st[j * nrhs + r], j = 0 c, 1 d, 2 alpha, 3 beta, 4 cp, 5 active (1/0);
+// partials[r * nblocks + b]
 void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s);
 void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s);
+// r -= alpha_r mmp with per-RHS partials of |r|^2 (kRedBlocks blocks)
 template <typename R>
-void cg_update1(long n, int nrhs, const double *st, void *r, const void *mmp, void *x, const void *p,
-                double *partials, hipStream_t s);
+void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s);
+// x += alpha_r p ; p = beta_r p + r ; t = A p with A the clover-inverse
+// blocks of parity par (or diag when null), zero on an open boundary
 template <typename R>
-void cg_update2(long n, int nrhs, const double *st, const void *r, void *p, hipStream_t s);
+void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
+                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s);
 
 }  // namespace wck

@@ -1,355 +1,15 @@
-// HIP kernels of the even-odd Wilson-clover operator for gfx950 (MI355X).
-//
-// Hot kernel: k_hop, one thread per output half-site.  For each of the 8
-// directions it gathers the neighbour spinor (12 complex, SoA so a wave reads
-// 1 KB contiguous per component), projects it to a half spinor with
-// (1 -+ gamma_mu), multiplies by the double-stored link (9 complex, or 6 with
-// the third row rebuilt), and accumulates the reconstructed 4-spinor.
-// Epilogues fuse the clover (inverse) block, the Schur complement's axpy and
-// the CG inner product, so the normal operator needs no extra passes.
+// HIP kernels of the even-odd Wilson-clover operator for gfx950 (MI355X),
+// other than the hop (wc_hop.h): the site-local clover, BLAS-1 with
+// deterministic reductions, the CG scalar and update kernels, layout
+// conversions and the link / clover set-up.
 #include <hip/hip_runtime.h>
 
 #include "wc_kernels.h"
+#include "wc_site.h"
 
 namespace wck {
 
 int site_blocks(long Vh) { return (int)((Vh + kSiteThreads - 1) / kSiteThreads); }
-
-// ------------------------------------------------------------ clover blocks
-// packed Hermitian 6x6: slots 0..2 hold the real diagonal in pairs, slots
-// 3..17 the strictly-lower entries in row order (1,0),(2,0),(2,1),(3,0),...
-__host__ __device__ constexpr int tri(int i, int j) { return 3 + i * (i - 1) / 2 + j; }
-
-template <typename R, typename C>
-__device__ __forceinline__ void clov_apply(const C *A, long Vh, long h, const C v[12], C o[12]) {
-#pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    const C *Ab = A + aidx(b * 18, h, 36);
-    R d[6];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      C dd = Ab[(long)k * kBlk];
-      d[2 * k] = dd.x;
-      d[2 * k + 1] = dd.y;
-    }
-    C L[15];
-#pragma unroll
-    for (int k = 0; k < 15; ++k) L[k] = Ab[(long)(3 + k) * kBlk];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      C acc = cscale(v[6 * b + i], d[i]);
-#pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        if (j < i) acc = cadd(acc, cmul(L[tri(i, j) - 3], v[6 * b + j]));
-        if (j > i) acc = cadd(acc, cmulc(L[tri(j, i) - 3], v[6 * b + j]));
-      }
-      o[6 * b + i] = acc;
-    }
-  }
-}
-
-// spinor of site h, right-hand side r of N: entry k at aidx(k, h, 12) * N + r
-template <typename C>
-__device__ __forceinline__ void load12(const C *f, long h, int r, int N, C v[12]) {
-  const C *p = f + aidx_site(h, 12) * N + r;
-#pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = p[(long)k * kBlk * N];
-}
-template <typename C>
-__device__ __forceinline__ void store12(C *f, long h, int r, int N, const C v[12]) {
-  C *p = f + aidx_site(h, 12) * N + r;
-#pragma unroll
-  for (int k = 0; k < 12; ++k) p[(long)k * kBlk * N] = v[k];
-}
-// output of a hop: written once, not re-read by this kernel
-template <typename C>
-__device__ __forceinline__ void store12_stream(C *f, long Vh, long h, int r, int N, const C v[12]) {
-#if WC_NT_STORE
-  const wc_rsrc rs = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)N * (uint32_t)sizeof(C)));
-  const uint32_t vo = (uint32_t)(aidx_site(h, 12) * N + r) * (uint32_t)sizeof(C);
-#pragma unroll
-  for (int k = 0; k < 12; ++k) bstore_t<C, 2>::go(v[k], rs, vo, (uint32_t)(k * kBlk * N * sizeof(C)));
-#else
-  store12(f, h, r, N, v);
-#endif
-}
-
-// ---------------------------------------------------------------- the hop
-// spin projection h = (1 - s gamma_mu) psi restricted to the upper spins, for
-// gpt's chiral basis (gamma_mu = [[0, X_mu], [X_mu^dag, 0]]):
-//   mu=0: h0 = p0 - s i p3, h1 = p1 - s i p2      mu=1: h0 = p0 + s p3, h1 = p1 - s p2
-//   mu=2: h0 = p0 - s i p2, h1 = p1 + s i p3      mu=3: h0 = p0 - s p2, h1 = p1 - s p3
-template <int MU, int S, typename C>
-__device__ __forceinline__ void project(const C p0, const C p1, const C p2, const C p3, C &h0, C &h1) {
-  if (MU == 0) {
-    h0 = S > 0 ? csub(p0, cimul(p3)) : cadd(p0, cimul(p3));
-    h1 = S > 0 ? csub(p1, cimul(p2)) : cadd(p1, cimul(p2));
-  } else if (MU == 1) {
-    h0 = S > 0 ? cadd(p0, p3) : csub(p0, p3);
-    h1 = S > 0 ? csub(p1, p2) : cadd(p1, p2);
-  } else if (MU == 2) {
-    h0 = S > 0 ? csub(p0, cimul(p2)) : cadd(p0, cimul(p2));
-    h1 = S > 0 ? cadd(p1, cimul(p3)) : csub(p1, cimul(p3));
-  } else {
-    h0 = S > 0 ? csub(p0, p2) : cadd(p0, p2);
-    h1 = S > 0 ? csub(p1, p3) : cadd(p1, p3);
-  }
-}
-
-// lower spins of (1 - s gamma_mu) psi from chi = U h: -s X_mu^dag chi
-//   mu=0: l2 = s i c1, l3 = s i c0      mu=1: l2 = -s c1, l3 = s c0
-//   mu=2: l2 = s i c0, l3 = -s i c1     mu=3: l2 = -s c0, l3 = -s c1
-template <int MU, int S, typename C>
-__device__ __forceinline__ void accum_lower(C &a2, C &a3, const C c0, const C c1) {
-  if (MU == 0) {
-    a2 = S > 0 ? cadd(a2, cimul(c1)) : csub(a2, cimul(c1));
-    a3 = S > 0 ? cadd(a3, cimul(c0)) : csub(a3, cimul(c0));
-  } else if (MU == 1) {
-    a2 = S > 0 ? csub(a2, c1) : cadd(a2, c1);
-    a3 = S > 0 ? cadd(a3, c0) : csub(a3, c0);
-  } else if (MU == 2) {
-    a2 = S > 0 ? cadd(a2, cimul(c0)) : csub(a2, cimul(c0));
-    a3 = S > 0 ? csub(a3, cimul(c1)) : cadd(a3, cimul(c1));
-  } else {
-    a2 = S > 0 ? csub(a2, c0) : cadd(a2, c0);
-    a3 = S > 0 ? csub(a3, c1) : cadd(a3, c1);
-  }
-}
-
-// one direction's operands: the neighbour spinor and the link
-template <typename C> struct DirData {
-  C psi[12];
-  C u[9];
-  long pad;
-};
-
-// project, multiply, accumulate direction (MU, FB)
-template <typename R, int NL, int DAG, int MU, int FB>
-__device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4],
-                                            DirData<typename c2<R>::t> &d, typename c2<R>::t acc[12]) {
-  using C = typename c2<R>::t;
-  const Geom &g = a.g;
-  const bool bnd = (FB == 0) ? (c[MU] == g.L[MU] - 1) : (c[MU] == 0);
-  constexpr int S = ((FB == 0) != (DAG != 0)) ? 1 : -1;
-  C hs[2][3];
-#pragma unroll
-  for (int col = 0; col < 3; ++col)
-    project<MU, S>(d.psi[0 + col], d.psi[3 + col], d.psi[6 + col], d.psi[9 + col], hs[0][col], hs[1][col]);
-  // hop coefficient, with the boundary phase on links that wrap around
-  const R coef = (R)(MU == 3 ? a.ct : a.cs);
-  if (bnd) {
-    C f = cmk<C>((R)(a.ph[MU][0]), (R)(FB == 0 ? a.ph[MU][1] : -a.ph[MU][1]));
-    f = cscale(f, coef);
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int col = 0; col < 3; ++col) hs[s][col] = cmul(f, hs[s][col]);
-  } else {
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int col = 0; col < 3; ++col) hs[s][col] = cscale(hs[s][col], coef);
-  }
-  C *u = d.u;
-  if (NL == 6) {
-    // third row = conj(row0 x row1) for SU(3)
-    u[6] = cconj(csub(cmul(u[1], u[5]), cmul(u[2], u[4])));
-    u[7] = cconj(csub(cmul(u[2], u[3]), cmul(u[0], u[5])));
-    u[8] = cconj(csub(cmul(u[0], u[4]), cmul(u[1], u[3])));
-  }
-  C ch[2][3];
-#pragma unroll
-  for (int s = 0; s < 2; ++s)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      C t = cmul(u[3 * i + 0], hs[s][0]);
-      t = cadd(t, cmul(u[3 * i + 1], hs[s][1]));
-      t = cadd(t, cmul(u[3 * i + 2], hs[s][2]));
-      ch[s][i] = t;
-    }
-#pragma unroll
-  for (int col = 0; col < 3; ++col) {
-    acc[0 + col] = cadd(acc[0 + col], ch[0][col]);
-    acc[3 + col] = cadd(acc[3 + col], ch[1][col]);
-    accum_lower<MU, S>(acc[6 + col], acc[9 + col], ch[0][col], ch[1][col]);
-  }
-}
-
-// The 8 directions run as a real loop (not unrolled): the loads of one
-// direction cannot be hoisted into another, so the live registers stay at one
-// direction's operands plus the accumulator; the other waves on the SIMD keep
-// HBM busy meanwhile.  The projection variant is picked by a wave-uniform
-// switch.  Neighbour offsets come from uniform strides of the half-site index
-// h = ((t*Lz + z)*Ly + y)*Lxh + x/2.
-template <typename R, int NL, int DAG>
-__device__ __forceinline__ void hop_site(const HopLaunch &a, const typename c2<R>::t *inp,
-                                         const typename c2<R>::t *W, long h, int r, int N, const int c[4],
-                                         typename c2<R>::t acc[12]) {
-  using C = typename c2<R>::t;
-  const Geom &g = a.g;
-  const uint32_t Vh = (uint32_t)g.Vh;
-  const uint32_t hh = (uint32_t)h;
-  const wc_rsrc in = make_rsrc(inp, 12u * Vh * (uint32_t)N * (uint32_t)sizeof(C));
-  // a wave's sites lie in at most two aligned 64-site blocks (one when N = 1):
-  // their links are addressed from the wave's first block, a wave-uniform base
-  const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
-  const wc_rsrc rw = make_rsrc(W + blk * (8 * NL * kBlk), 2u * 8u * NL * kBlk * (uint32_t)sizeof(C));
-  const uint32_t vw = (uint32_t)(aidx_site(h, 8 * NL) - blk * (8 * NL * kBlk)) * (uint32_t)sizeof(C);
-  const uint32_t kstride = (uint32_t)(kBlk * N * sizeof(C));
-  const uint32_t xh = (uint32_t)c[0] >> 1;
-  const uint32_t st1 = (uint32_t)g.Lxh, st2 = st1 * (uint32_t)g.L[1], st3 = st2 * (uint32_t)g.L[2];
-#pragma unroll 1
-  for (int dir = 0; dir < 8; ++dir) {
-    const int mu = dir >> 1, fb = dir & 1;
-    uint32_t nb;
-    if (mu == 0) {
-      const int x = c[0];
-      const int xn = fb == 0 ? (x == g.L[0] - 1 ? 0 : x + 1) : (x == 0 ? g.L[0] - 1 : x - 1);
-      nb = hh - xh + ((uint32_t)xn >> 1);
-    } else {
-      const uint32_t stride = mu == 1 ? st1 : (mu == 2 ? st2 : st3);
-      const int cm = mu == 1 ? c[1] : (mu == 2 ? c[2] : c[3]);
-      const int Lm = g.L[mu];
-      const uint32_t wrap = (uint32_t)(Lm - 1) * stride;
-      if (fb == 0)
-        nb = (cm == Lm - 1) ? hh - wrap : hh + stride;
-      else
-        nb = (cm == 0) ? hh + wrap : hh - stride;
-    }
-    DirData<C> d;
-    const uint32_t vs = ((uint32_t)aidx_site(nb, 12) * (uint32_t)N + (uint32_t)r) * (uint32_t)sizeof(C);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) d.psi[k] = bload<C, WC_SPINOR_AUX>(in, vs, (uint32_t)k * kstride);
-    const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
-#pragma unroll
-    for (int e = 0; e < NL; ++e)
-      d.u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
-    switch (dir) {
-      case 0: compute_dir<R, NL, DAG, 0, 0>(a, c, d, acc); break;
-      case 1: compute_dir<R, NL, DAG, 0, 1>(a, c, d, acc); break;
-      case 2: compute_dir<R, NL, DAG, 1, 0>(a, c, d, acc); break;
-      case 3: compute_dir<R, NL, DAG, 1, 1>(a, c, d, acc); break;
-      case 4: compute_dir<R, NL, DAG, 2, 0>(a, c, d, acc); break;
-      case 5: compute_dir<R, NL, DAG, 2, 1>(a, c, d, acc); break;
-      case 6: compute_dir<R, NL, DAG, 3, 0>(a, c, d, acc); break;
-      default: compute_dir<R, NL, DAG, 3, 1>(a, c, d, acc); break;
-    }
-  }
-}
-
-template <typename R, int NL, int DAG, int EPI>
-__global__ __launch_bounds__(kSiteThreads, WC_HOP_MINWAVES) void k_hop(const HopLaunch a) {
-  using C = typename c2<R>::t;
-  __shared__ double red[kSiteThreads];
-  const Geom &g = a.g;
-  const long Vh = g.Vh;
-  const int N = a.nrhs;
-  // thread t -> (site h, right-hand side r); the N threads of a site are
-  // adjacent, so they share each link load
-  const long t0 = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads;
-  const long t = t0 + threadIdx.x;
-  const long h = (N == 1) ? t : t / N;
-  const int r = (int)(t - h * N);
-  double dot = 0.0;
-  if (h < Vh) {
-    int c[4];
-    half_coords(g, h, a.pout, c);
-    const C *in = (const C *)a.in;
-    const C *W = (const C *)a.W;
-    C acc[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = cmk<C>(0, 0);
-    hop_site<R, NL, DAG>(a, in, W, h, r, N, c, acc);
-    // epilogue
-    const bool zb = a.open_bc && (c[3] == 0 || c[3] == g.L[3] - 1);
-    C res[12];
-    if (zb) {
-#pragma unroll
-      for (int k = 0; k < 12; ++k) res[k] = cmk<C>(0, 0);
-    } else if (EPI & EPI_CLOV) {
-      if (a.clov) {
-        clov_apply<R>((const C *)a.clov, Vh, h, acc, res);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 12; ++k) res[k] = cscale(acc[k], (R)a.clov_scalar);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 12; ++k) res[k] = acc[k];
-    }
-    if (EPI & (EPI_DIAG | EPI_AXPY)) {
-      C y[12];
-      load12((const C *)a.y, h, r, N, y);
-      if (EPI & EPI_DIAG) {
-        if (!zb) {
-          C dy[12];
-          if (a.diag_clov) {
-            clov_apply<R>((const C *)a.diag_clov, Vh, h, y, dy);
-          } else {
-#pragma unroll
-            for (int k = 0; k < 12; ++k) dy[k] = cscale(y[k], (R)a.diag);
-          }
-#pragma unroll
-          for (int k = 0; k < 12; ++k) res[k] = cadd(res[k], dy[k]);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 12; ++k) res[k] = csub(y[k], res[k]);
-      }
-    }
-    if (EPI & EPI_DOT) {
-      C v[12];
-      load12((const C *)a.dotv, h, r, N, v);
-#pragma unroll
-      for (int k = 0; k < 12; ++k) dot += (double)v[k].x * (double)res[k].x + (double)v[k].y * (double)res[k].y;
-    }
-    store12_stream((C *)a.out, Vh, h, r, N, res);
-  }
-  if (EPI & EPI_DOT) {
-    if (N == 1) {
-      double s = block_sum<kSiteThreads>(dot, red);
-      if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
-    } else {
-      // per-RHS sums of the block in fixed lane order: partial[block * N + r]
-      red[threadIdx.x] = dot;
-      __syncthreads();
-      if (threadIdx.x < N) {
-        double s = 0.0;
-        const int r0 = (int)(t0 % N);
-        for (int i = ((int)threadIdx.x - r0 + N) % N; i < kSiteThreads; i += N) s += red[i];
-        a.partial[(long)blockIdx.x * N + threadIdx.x] = s;
-      }
-    }
-  }
-}
-
-template <typename R, int NL, int DAG>
-static void hop_epi(const HopLaunch &a, hipStream_t s) {
-  const int nb = site_blocks(a.g.Vh * a.nrhs);
-  switch (a.epi) {
-#define WC_EPI(E) \
-  case E: k_hop<R, NL, DAG, E><<<nb, kSiteThreads, 0, s>>>(a); break;
-    WC_EPI(0)
-    WC_EPI(EPI_CLOV)
-    WC_EPI(EPI_DIAG)
-    WC_EPI(EPI_AXPY)
-    WC_EPI(EPI_CLOV | EPI_AXPY)
-    WC_EPI(EPI_CLOV | EPI_AXPY | EPI_DOT)
-    WC_EPI(EPI_AXPY | EPI_DOT)
-#undef WC_EPI
-    default: break;
-  }
-}
-
-template <typename R> void hop(const HopLaunch &a, hipStream_t s) {
-  if (a.nl == 9) {
-    if (a.dag) hop_epi<R, 9, 1>(a, s); else hop_epi<R, 9, 0>(a, s);
-  } else {
-    if (a.dag) hop_epi<R, 6, 1>(a, s); else hop_epi<R, 6, 0>(a, s);
-  }
-}
-template void hop<double>(const HopLaunch &, hipStream_t);
-template void hop<float>(const HopLaunch &, hipStream_t);
 
 // ------------------------------------------------------- site-local clover
 template <typename R>
@@ -374,7 +34,7 @@ __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = cmk<C>(0, 0);
   } else if (A) {
-    clov_apply<R>(A, g.Vh, h, v, o);
+    clov_apply<R>(A, h, v, o);
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = cscale(v[k], (R)diag);
@@ -746,7 +406,6 @@ __global__ __launch_bounds__(128) void k_build_clover(Geom g, const double2 *U, 
 
   C *A = p ? A1 : A0;
   C *Ai = p ? I1 : I0;
-  const long Vh = g.Vh;
 #pragma unroll
   for (int ch = 0; ch < 2; ++ch) {
     // store A (packed)
@@ -863,8 +522,8 @@ __global__ __launch_bounds__(256) void k_dot_rhs(long n, int N, const typename c
   li[threadIdx.x] = si;
   __syncthreads();
   if ((int)threadIdx.x < N) {
-    pre[(long)blockIdx.x * N + threadIdx.x] = seg_sum(lr, blockDim.x, N);
-    if (pim) pim[(long)blockIdx.x * N + threadIdx.x] = seg_sum(li, blockDim.x, N);
+    pre[(long)threadIdx.x * gridDim.x + blockIdx.x] = seg_sum(lr, blockDim.x, N);
+    if (pim) pim[(long)threadIdx.x * gridDim.x + blockIdx.x] = seg_sum(li, blockDim.x, N);
   }
 }
 template <typename R>
@@ -875,13 +534,36 @@ void dot_partials_rhs(long n, int nrhs, const void *a, const void *b, double *pr
 template void dot_partials_rhs<double>(long, int, const void *, const void *, double *, double *, hipStream_t);
 template void dot_partials_rhs<float>(long, int, const void *, const void *, double *, double *, hipStream_t);
 
-// out[r] = sum_b partials[b * N + r]; one block per RHS, fixed order
+// sum of p[r * nb + b] over b in a fixed order (thread t: b = t mod 256,
+// then the block tree); p is read contiguously
+__device__ __forceinline__ double rhs_total(const double *p, int nb, int r, double *red) {
+  const double *q = p + (long)r * nb;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) s += q[b];
+  return block_sum<256>(s, red);
+}
+
+// per-(block, RHS) sums of per-site products, sites in order
+__global__ void k_dot_blocks(long nb, int N, const double *dots, double *partial) {
+  const long t = (long)blockIdx.x * 256 + threadIdx.x;
+  if (t >= nb * N) return;
+  const long b = t / N;
+  const int r = (int)(t - b * N);
+  const double *p = dots + b * kBlk * N + r;
+  double s = 0.0;
+  for (int i = 0; i < kBlk; ++i) s += p[(long)i * N];
+  partial[(long)r * nb + b] = s;
+}
+void dot_blocks(const Geom &g, int nrhs, const double *dots, double *partial, hipStream_t s) {
+  const long nb = hop_blocks(g);
+  k_dot_blocks<<<(unsigned)((nb * nrhs + 255) / 256), 256, 0, s>>>(nb, nrhs, dots, partial);
+}
+
+// out[r] = sum_b partials[r * nb + b]; one block per RHS
 __global__ __launch_bounds__(256) void k_reduce_final_rhs(const double *p, int nb, int N, double *out) {
   __shared__ double red[256 / 64 + 1];
   const int r = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) s += p[(long)b * N + r];
-  s = block_sum<256>(s, red);
+  const double s = rhs_total(p, nb, r, red);
   if (threadIdx.x == 0) out[r] = s;
 }
 void reduce_final_rhs(const double *partials, int nblocks, int nrhs, double *out, hipStream_t s) {
@@ -931,9 +613,7 @@ template void scale_rhs<float>(long, int, const double *, void *, hipStream_t);
 __global__ __launch_bounds__(256) void k_cg_alpha(const double *p, int nb, int N, double *st) {
   __shared__ double red[256 / 64 + 1];
   const int r = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) s += p[(long)b * N + r];
-  s = block_sum<256>(s, red);
+  const double s = rhs_total(p, nb, r, red);
   if (threadIdx.x == 0) {
     st[1 * N + r] = s;
     st[2 * N + r] = (st[5 * N + r] != 0.0 && s != 0.0) ? st[0 * N + r] / s : 0.0;
@@ -942,9 +622,7 @@ __global__ __launch_bounds__(256) void k_cg_alpha(const double *p, int nb, int N
 __global__ __launch_bounds__(256) void k_cg_beta(const double *p, int nb, int N, double *st) {
   __shared__ double red[256 / 64 + 1];
   const int r = blockIdx.x;
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += 256) s += p[(long)b * N + r];
-  s = block_sum<256>(s, red);
+  const double s = rhs_total(p, nb, r, red);
   if (threadIdx.x == 0) {
     st[4 * N + r] = s;
     if (st[5 * N + r] != 0.0) {
@@ -962,11 +640,10 @@ void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStrea
   k_cg_beta<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st);
 }
 
-// r -= alpha_r mmp; x += alpha_r p; partial |r|^2 per RHS
+// r -= alpha_r mmp; partial |r|^2 per RHS
 template <typename R>
-__global__ __launch_bounds__(256) void k_cg_update1(long n, int N, const double *st, typename c2<R>::t *r,
-                                                    const typename c2<R>::t *mmp, typename c2<R>::t *x,
-                                                    const typename c2<R>::t *p, double *pre) {
+__global__ __launch_bounds__(256) void k_cg_update_r(long n, int N, const double *st, typename c2<R>::t *r,
+                                                     const typename c2<R>::t *mmp, double *pre) {
   using C = typename c2<R>::t;
   __shared__ double lds[256];
   const long stride = (long)gridDim.x * blockDim.x;
@@ -974,45 +651,74 @@ __global__ __launch_bounds__(256) void k_cg_update1(long n, int N, const double 
   const R a = (R)st[2 * N + (int)(i0 % N)];
   double s = 0.0;
   for (long i = i0; i < n; i += stride) {
-    C rv = r[i], mv = mmp[i], xv = x[i], pv = p[i];
+    C rv = r[i];
+    const C mv = mmp[i];
     rv = cmk<C>(rv.x - a * mv.x, rv.y - a * mv.y);
-    xv = cmk<C>(xv.x + a * pv.x, xv.y + a * pv.y);
     r[i] = rv;
-    x[i] = xv;
     s += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
   }
   lds[threadIdx.x] = s;
   __syncthreads();
-  if ((int)threadIdx.x < N) pre[(long)blockIdx.x * N + threadIdx.x] = seg_sum(lds, blockDim.x, N);
+  if ((int)threadIdx.x < N) pre[(long)threadIdx.x * gridDim.x + blockIdx.x] = seg_sum(lds, blockDim.x, N);
 }
 template <typename R>
-void cg_update1(long n, int nrhs, const double *st, void *r, const void *mmp, void *x, const void *p,
-                double *partials, hipStream_t s) {
+void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s) {
   using C = typename c2<R>::t;
-  k_cg_update1<R><<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, st, (C *)r, (const C *)mmp, (C *)x,
-                                                                 (const C *)p, partials);
+  k_cg_update_r<R><<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, st, (C *)r, (const C *)mmp, partials);
 }
-template void cg_update1<double>(long, int, const double *, void *, const void *, void *, const void *,
-                                 double *, hipStream_t);
-template void cg_update1<float>(long, int, const double *, void *, const void *, void *, const void *,
-                                double *, hipStream_t);
+template void cg_update_r<double>(long, int, const double *, void *, const void *, double *, hipStream_t);
+template void cg_update_r<float>(long, int, const double *, void *, const void *, double *, hipStream_t);
 
-// p = beta_r p + r
+// x += alpha_r p ; p = beta_r p + r ; t = A p (site-local, parity par: the
+// clover inverse blocks A, or diag when A is null; zero on an open boundary)
 template <typename R>
-__global__ void k_cg_update2(long n, int N, const double *st, const typename c2<R>::t *r,
-                             typename c2<R>::t *p) {
+__global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, int par, const double *st,
+                                                              const typename c2<R>::t *r, typename c2<R>::t *p,
+                                                              typename c2<R>::t *x, const typename c2<R>::t *A,
+                                                              double diag, typename c2<R>::t *t, int open_bc) {
   using C = typename c2<R>::t;
-  long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const R b = (R)st[3 * N + (int)(i % N)];
-  C pv = p[i], rv = r[i];
-  p[i] = cmk<C>(b * pv.x + rv.x, b * pv.y + rv.y);
+  const long tid = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
+  const long h = (N == 1) ? tid : tid / N;
+  const int rr = (int)(tid - h * N);
+  if (h >= g.Vh) return;
+  const R a = (R)st[2 * N + rr], b = (R)st[3 * N + rr];
+  C pv[12], v[12];
+  load12(p, h, rr, N, pv);
+  load12(x, h, rr, N, v);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) v[k] = cmk<C>(v[k].x + a * pv[k].x, v[k].y + a * pv[k].y);
+  store12(x, h, rr, N, v);
+  load12(r, h, rr, N, v);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) pv[k] = cmk<C>(b * pv[k].x + v[k].x, b * pv[k].y + v[k].y);
+  store12(p, h, rr, N, pv);
+  bool zb = false;
+  if (open_bc) {
+    int c[4];
+    half_coords(g, h, par, c);
+    zb = (c[3] == 0 || c[3] == g.L[3] - 1);
+  }
+  if (zb) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cmk<C>(0, 0);
+  } else if (A) {
+    clov_apply<R>(A, h, pv, v);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = cscale(pv[k], (R)diag);
+  }
+  store12(t, h, rr, N, v);
 }
-template <typename R> void cg_update2(long n, int nrhs, const double *st, const void *r, void *p, hipStream_t s) {
+template <typename R>
+void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
+                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s) {
   using C = typename c2<R>::t;
-  k_cg_update2<R><<<elem_blocks(n), 256, 0, s>>>(n, nrhs, st, (const C *)r, (C *)p);
+  k_cg_update_xp<R><<<site_blocks(g.Vh * nrhs), kSiteThreads, 0, s>>>(
+      g, nrhs, par, st, (const C *)r, (C *)p, (C *)x, (const C *)blocks, diag, (C *)t, open_bc);
 }
-template void cg_update2<double>(long, int, const double *, const void *, void *, hipStream_t);
-template void cg_update2<float>(long, int, const double *, const void *, void *, hipStream_t);
+template void cg_update_xp<double>(const Geom &, int, int, const double *, const void *, void *, void *,
+                                   const void *, double, void *, int, hipStream_t);
+template void cg_update_xp<float>(const Geom &, int, int, const double *, const void *, void *, void *,
+                                  const void *, double, void *, int, hipStream_t);
 
 }  // namespace wck

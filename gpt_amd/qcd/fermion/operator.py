@@ -93,6 +93,13 @@ class interface:
             operator_limbo.setdefault(operator_tag[self.obj], []).append(self.obj)
             self.obj = None
 
+    def release(self):
+        """free the native operator now instead of parking it in the pool"""
+        if self.obj is not None:
+            operator_tag.pop(self.obj, None)
+            _lib.check(_lib.load().wc_destroy(self.obj))
+            self.obj = None
+
     def update(self, params, U):
         if self.obj is None:
             self._setup(*self.setup_arguments)
@@ -188,6 +195,36 @@ class fine_operator(matrix_operator):
             self.interface._setup(*self.interface.setup_arguments)
         return self.interface.obj
 
+    # hop launch shapes (rb, tpb, occ, seg): rb right-hand sides per thread,
+    # 64 * tpb threads per workgroup, at most occ workgroups per CU (0: no cap),
+    # seg t-marching runs per time slice (1: natural order)
+    # (wc_tune_hop / wc_set_hop_shape / wc_get_hop_shape); every shape gives
+    # bitwise the same result
+    def tune_hop(self, n_rhs=None, reps=5):
+        """time every launch shape of the hop for n_rhs right-hand sides and keep
+        the fastest; returns (shape, milliseconds per call)"""
+        n = self.n_rhs if n_rhs is None else int(n_rhs)
+        sh = (ctypes.c_int * 4)()
+        ms = ctypes.c_float()
+        _lib.check(_lib.load().wc_tune_hop(self.handle, n, int(reps), ctypes.addressof(sh),
+                                           ctypes.addressof(ms), core.stream_ptr()))
+        return tuple(sh), float(ms.value)
+
+    def hop_shape(self, n_rhs=None):
+        n = self.n_rhs if n_rhs is None else int(n_rhs)
+        sh = (ctypes.c_int * 4)()
+        _lib.check(_lib.load().wc_get_hop_shape(self.handle, n, ctypes.addressof(sh)))
+        return tuple(sh)
+
+    def set_hop_shape(self, shape, n_rhs=None):
+        """shape = (rb, tpb, occ, seg), or None for automatic"""
+        n = self.n_rhs if n_rhs is None else int(n_rhs)
+        if shape is None:
+            _lib.check(_lib.load().wc_set_hop_shape(self.handle, n, None))
+            return
+        sh = (ctypes.c_int * 4)(*[int(x) for x in shape])
+        _lib.check(_lib.load().wc_set_hop_shape(self.handle, n, ctypes.addressof(sh)))
+
     def kappa(self):
         p = self.params
         return 1.0 / (2.0 * (p["mass"] + 1.0 + 3.0 * p["nu"] / p["xi_0"]))
@@ -212,6 +249,11 @@ class fine_operator(matrix_operator):
 
     def suspend(self):
         self.interface.suspend()
+
+    def release(self):
+        """free the operator's device memory now (it is not parked for reuse);
+        it is set up again if used afterwards"""
+        self.interface.release()
 
     def arguments(self):
         return self.U

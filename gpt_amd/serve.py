@@ -1,0 +1,307 @@
+"""Serving loop for propagator solves on resident gauge fields.
+
+No gpt counterpart: gpt solves right-hand sides one after another
+(lib/gpt/core/operator/matrix_operator.py:288-295, container.py:330-357) on
+one operator at a time.  Here requests (a gauge field, a mass, 1..12
+right-hand sides, a tolerance) arrive at any time and are served by
+continuous batching:
+
+* a *group* per (resident field, mass) owns an FP64 / FP32 operator pair and
+  a ``batch`` of ``nslots`` solve slots (``wc_batch_*``: the mixed-precision
+  eo2_ne CG advancing all occupied slots with the multi-RHS kernels);
+* right-hand sides enter free slots of their group whenever slots free up,
+  while the group keeps iterating, and leave when their FP64 true residual
+  |src - M dst| / |src| meets the request's tolerance;
+* a new group is created only if its HBM footprint fits the budget left
+  (``group_bytes``); otherwise the request waits, and idle groups are
+  released;
+* groups step on their own HIP streams from worker threads (ctypes releases
+  the GIL), so the work of different groups overlaps on the device.
+
+Every result is a solve to the request's tolerance, whatever else shares its
+batch: alone or in a mix, solutions agree to within the solver tolerance
+(the reliable-update points depend on the batch, the converged solutions do
+not beyond that).  ``latency`` is arrival to result on the host.
+"""
+import collections
+import concurrent.futures as cf
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib, core
+from .core import lattice
+
+
+class batch:
+    """One wc_batch: nslots slots of the mixed-precision eo2_ne solve on the
+    operator pair (op FP64, inner in the inner precision)."""
+
+    def __init__(self, op, inner, nslots, parity=core.odd, delta=0.1):
+        self.op, self.inner = op, inner
+        self.nslots = int(nslots)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.load().wc_batch_create(op.handle, inner.handle, parity.tag, self.nslots, float(delta),
+                                               core.stream_ptr(), ctypes.byref(h)))
+        self.handle = h.value
+        self.free = list(range(self.nslots))
+        self.slot_of = {}
+
+    def admit(self, src, tol, maxiter, guess=None):
+        slot = self.free.pop(0)
+        _lib.check(_lib.load().wc_batch_admit(self.handle, slot, src.ptr(), guess.ptr() if guess is not None else None,
+                                              float(tol), int(maxiter)))
+        return slot
+
+    def step(self, iters):
+        fin = np.zeros(self.nslots, dtype=np.int32)
+        nf = ctypes.c_int()
+        _lib.check(_lib.load().wc_batch_step(self.handle, int(iters), fin.ctypes.data, ctypes.byref(nf)))
+        return [i for i in range(self.nslots) if fin[i]]
+
+    def extract(self, slot, dst):
+        it = ctypes.c_long()
+        tres = ctypes.c_double()
+        conv = ctypes.c_int()
+        _lib.check(_lib.load().wc_batch_extract(self.handle, slot, dst.ptr(), ctypes.byref(it), ctypes.byref(tres),
+                                                ctypes.byref(conv)))
+        self.free.append(slot)
+        self.free.sort()
+        return int(it.value), float(tres.value), bool(conv.value)
+
+    def occupied(self):
+        occ = np.zeros(self.nslots, dtype=np.int32)
+        _lib.check(_lib.load().wc_batch_slots(self.handle, occ.ctypes.data, None))
+        return int(occ.sum())
+
+    def close(self):
+        if self.handle is not None:
+            _lib.load().wc_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+
+class request:
+    """A propagator solve: ``sources`` are full FP64 fields (one right-hand
+    side each) on the grid of the resident gauge field ``field``."""
+
+    _ids = 0
+
+    def __init__(self, field, mass, sources, tol=1e-10, maxiter=100000):
+        request._ids += 1
+        self.id = request._ids
+        self.field = field
+        self.mass = float(mass)
+        self.sources = list(sources)
+        self.tol = float(tol)
+        self.maxiter = int(maxiter)
+        n = len(self.sources)
+        self.results = [None] * n
+        self.iterations = [0] * n
+        self.true_residual = [0.0] * n
+        self.converged = [False] * n
+        self.remaining = n
+        self.t_arrival = None
+        self.t_done = None
+
+    @property
+    def done(self):
+        return self.remaining == 0
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+def group_bytes(fdims, nslots):
+    """HBM held by one group: both operators (links, clover and inverse,
+    the FP64 gauge copy each keeps, solver scratch) and the batch state."""
+    V = int(np.prod(fdims))
+    Vh = V // 2
+    h64, h32 = Vh * 12 * 16, Vh * 12 * 8
+    op64 = 2 * Vh * 8 * 9 * 16 + 4 * Vh * 36 * 16 + V * 4 * 9 * 16 + h64 * (3 * nslots + 9)
+    op32 = 2 * Vh * 8 * 9 * 8 + 4 * Vh * 36 * 8 + V * 4 * 9 * 16 + h32 * (3 * nslots + 5)
+    state = 5 * h64 * nslots + 5 * h32 * nslots + 2 * h64 * nslots
+    return op64 + op32 + state
+
+
+class _group:
+    def __init__(self, server, key, U, mass, stream):
+        self.server = server
+        self.key = key
+        self.U = U
+        self.mass = mass
+        self.stream = stream
+        self.queue = collections.deque()  # (request, rhs index)
+        self.in_slot = {}  # slot -> (request, rhs index)
+        self.batch = None
+        self.op = self.inner = None
+        self.bytes = group_bytes(U[0].grid.fdimensions, server.nslots)
+
+    def busy(self):
+        return bool(self.queue) or bool(self.in_slot)
+
+    def work(self, iters):
+        """admit what fits, run iters inner iterations, hand back finished solves
+        (runs on a worker thread, on this group's stream)"""
+        from .qcd import fermion
+
+        done = []
+        with torch.cuda.stream(self.stream):
+            if self.batch is None:
+                p = dict(self.server.params, mass=self.mass)
+                self.op = fermion.wilson_clover(self.U, p)
+                self.inner = self.op.converted(self.server.inner_precision)
+                self.batch = batch(self.op, self.inner, self.server.nslots, delta=self.server.delta)
+            while self.queue and self.batch.free:
+                req, j = self.queue.popleft()
+                slot = self.batch.admit(req.sources[j], req.tol, req.maxiter)
+                self.in_slot[slot] = (req, j)
+            finished = self.batch.step(iters) if self.in_slot else []
+            for slot in finished:
+                req, j = self.in_slot.pop(slot)
+                dst = lattice(self.op.F_grid)
+                it, tres, conv = self.batch.extract(slot, dst)
+                done.append((req, j, dst, it, tres, conv))
+        return done
+
+    def release(self):
+        if self.batch is not None:
+            self.batch.close()
+        for o in (self.inner, self.op):
+            if o is not None:
+                o.release()
+        self.batch = self.op = self.inner = None
+
+
+class server:
+    """Continuous-batching solve server (see the module notes).
+
+    params: operator parameters without the mass (csw_r, csw_t, boundary
+    phases, ...); nslots: slots per group; streams: HIP streams the groups
+    are spread over; iters_per_step: inner iterations a group runs before the
+    scheduler looks again; hbm_budget: bytes the groups may hold (default 80%
+    of the device memory free at start)."""
+
+    def __init__(self, params, nslots=12, streams=4, iters_per_step=25, hbm_budget=None,
+                 inner_precision=core.single, delta=0.1):
+        self.params = {k: v for k, v in dict(params).items() if k not in ("mass", "kappa")}
+        self.nslots = int(nslots)
+        self.iters_per_step = int(iters_per_step)
+        self.inner_precision = inner_precision
+        self.delta = float(delta)
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.8 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.hbm_used = 0
+        self.fields = {}
+        self.groups = {}
+        self.streams = [torch.cuda.Stream() for _ in range(max(1, int(streams)))]
+        self._next_stream = 0
+        self.pool = cf.ThreadPoolExecutor(max_workers=len(self.streams))
+        self.completed = []
+        self.peak_groups = 0
+        self.waited_for_memory = 0
+
+    def add_field(self, key, U):
+        """keep a gauge field resident: 4 link fields (gpt_amd.qcd.gauge) or
+        natural-layout numpy arrays, held in HBM in FP64"""
+        from .qcd import gauge
+
+        if not isinstance(U[0], gauge.link_field):
+            fd = list(reversed(np.asarray(U[0]).shape[:4]))
+            U = gauge.from_numpy(core.grid(fd, core.double), U)
+        self.fields[key] = U
+
+    def _route(self, req):
+        key = (req.field, req.mass)
+        g = self.groups.get(key)
+        if g is None:
+            U = self.fields[req.field]
+            need = group_bytes(U[0].grid.fdimensions, self.nslots)
+            if self.hbm_used + need > self.hbm_budget:
+                return False
+            g = _group(self, key, U, req.mass, self.streams[self._next_stream % len(self.streams)])
+            self._next_stream += 1
+            self.groups[key] = g
+            self.hbm_used += g.bytes
+            self.peak_groups = max(self.peak_groups, len(self.groups))
+        g.queue.extend((req, j) for j in range(len(req.sources)))
+        return True
+
+    def run(self, schedule, on_done=None):
+        """serve requests arriving at the given offsets (seconds from now);
+        schedule: list of (offset, request); on_done(request) is called as
+        each request completes.  Returns the requests in completion order."""
+        schedule = sorted(schedule, key=lambda x: x[0])
+        t0 = time.perf_counter()
+        pending = collections.deque()
+        busy = {}
+        i = 0
+        order = []
+        while True:
+            now = time.perf_counter()
+            while i < len(schedule) and t0 + schedule[i][0] <= now:
+                req = schedule[i][1]
+                req.t_arrival = t0 + schedule[i][0]
+                pending.append(req)
+                i += 1
+            waiting = collections.deque()
+            while pending:
+                req = pending.popleft()
+                if not self._route(req):
+                    waiting.append(req)
+            if waiting:
+                self.waited_for_memory += 1
+            pending = waiting
+            for key, g in self.groups.items():
+                if key not in busy and g.busy():
+                    busy[key] = self.pool.submit(g.work, self.iters_per_step)
+            if not busy:
+                if i >= len(schedule) and not pending:
+                    break
+                if pending:
+                    raise RuntimeError("a request needs more HBM than the budget holds")
+                time.sleep(max(0.0, min(0.01, t0 + schedule[i][0] - time.perf_counter())))
+                continue
+            timeout = None
+            if i < len(schedule):
+                timeout = max(0.0, t0 + schedule[i][0] - time.perf_counter())
+            fin, _ = cf.wait(list(busy.values()), timeout=timeout, return_when=cf.FIRST_COMPLETED)
+            for key in [k for k, f in busy.items() if f in fin]:
+                f = busy.pop(key)
+                for req, j, dst, it, tres, conv in f.result():
+                    req.results[j] = dst
+                    req.iterations[j] = it
+                    req.true_residual[j] = tres
+                    req.converged[j] = conv
+                    req.remaining -= 1
+                    if req.done:
+                        req.t_done = time.perf_counter()
+                        order.append(req)
+                        if on_done is not None:
+                            on_done(req)
+                g = self.groups[key]
+                if not g.busy():
+                    g.release()
+                    self.hbm_used -= g.bytes
+                    del self.groups[key]
+        self.completed.extend(order)
+        return order
+
+    def close(self):
+        for g in self.groups.values():
+            g.release()
+        self.groups.clear()
+        self.pool.shutdown(wait=True)
+
+
+def latency_stats(reqs):
+    lat = np.array([r.latency for r in reqs])
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "max_s": float(lat.max())}

@@ -164,8 +164,8 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
 /* Mixed-precision eo2_ne solve with reliable updates (no cgpt counterpart;
  * the precision pattern of lib/gpt/algorithms/inverter/defect_correcting.py
  * and mixed_precision.py, done inside one CG).  `op` is the FP64 operator,
- * `inner` the same operator in the inner precision (wc_create with
- * precision WC_SINGLE, or WC_DOUBLE).  The CG recursion runs in the inner
+ * `inner` the same operator in the inner precision: a separate operator
+ * object (wc_create with precision WC_SINGLE, or WC_DOUBLE).  The CG recursion runs in the inner
  * precision; FP64 keeps the solution, the source terms and the true
  * residuals.  When a right-hand side's iterated |r|^2 has fallen by delta^2
  * since the last reliable update, the inner solution is added to the FP64
@@ -177,6 +177,56 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
 int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int nrhs, void *dst, const void *src,
                           double tol, long maxiter, double delta, long *iterations, int *converged,
                           double *true_residual, long *reliable_updates, void *stream);
+
+/* Hop launch shapes (no cgpt counterpart: Grid fixes its kernel launch at
+ * compile time).  A shape is four ints {rb, tpb, occ, seg}: rb right-hand
+ * sides per thread (1 in FP64, 1..2 in FP32), 64 * tpb threads per
+ * workgroup (tpb 1..8), occ (0..32) caps the workgroups per CU by reserving
+ * LDS (0: no cap), and seg > 1 visits the 64-site blocks in t-marching order
+ * with seg runs per time slice (needs the time slice to be a whole number
+ * of blocks, seg <= that number).  Every shape gives bitwise the same
+ * result; only the speed differs.
+ *
+ * wc_tune_hop times the hop of nrhs right-hand sides (reps calls per shape)
+ * for every valid shape and keeps the fastest for this operator and, in a
+ * process-wide cache, for every operator of the same dimensions, precision
+ * and link format.  shape (4 ints) and ms_per_call may be NULL.  The CG and
+ * solve entry points tune on first use when Vh * nrhs >= 2^17 (environment
+ * WC_AUTOTUNE=0 turns that off).  wc_set_hop_shape fixes a shape (NULL:
+ * back to automatic); wc_get_hop_shape reports the one in use. */
+int wc_tune_hop(wc_operator *op, int nrhs, int reps, int *shape, float *ms_per_call, void *stream);
+int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape);
+int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape);
+
+/* Continuous-batching solver (no cgpt counterpart: gpt solves right-hand
+ * sides one after another, matrix_operator.py:288-295).  A wc_batch holds
+ * nslots slots of the mixed-precision eo2_ne solve on one operator pair
+ * (op FP64, inner a separate operator object in the inner precision, as for
+ * wc_solve_eo2_ne_mixed) and advances all occupied slots together with the
+ * multi-RHS kernels.  Requests enter free slots at any time and leave when
+ * done, so the batch keeps running while its membership changes.
+ *
+ *   wc_batch_admit   puts one right-hand side (src: full FP64 field; guess:
+ *                    full FP64 field or NULL for zero) into a free slot, with
+ *                    its own tolerance (true relative residual of the full
+ *                    system) and iteration limit.
+ *   wc_batch_step    runs up to iters inner iterations of the occupied slots
+ *                    (fewer if none is left iterating); finished[nslots]
+ *                    (may be NULL) flags the slots whose solve is done.
+ *   wc_batch_extract writes a finished slot's solution dst = L y + S src
+ *                    (full FP64 field) with its statistics, and frees the slot.
+ *   wc_batch_slots   reports occupied / iterating slots (arrays may be NULL).
+ *
+ * Calls on one batch, and on its operators, must not overlap; batches on
+ * different operator pairs and streams run concurrently. */
+typedef struct wc_batch wc_batch;
+int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots, double delta, void *stream,
+                    wc_batch **out);
+int wc_batch_destroy(wc_batch *b);
+int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter);
+int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);
+int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double *true_residual, int *converged);
+int wc_batch_slots(const wc_batch *b, int *occupied, int *active);
 
 /* message of the last failed call on this thread */
 const char *wc_last_error(void);

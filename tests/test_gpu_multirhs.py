@@ -100,3 +100,53 @@ def test_batched_cg_matches_single_solves(setup):
             assert np.abs(a).max() == 0.0 and cgN.iterations[i] == 0
         else:
             assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-10
+
+
+def test_every_hop_shape_gives_identical_bits(setup):
+    """Launch shapes (right-hand sides per thread, threads per block) change
+    only the schedule: the hop, the fused normal operator with its
+    inner-product partials and a whole CG agree bit for bit."""
+    from gpt_amd.qcd.fermion.operator import pack
+
+    gr, U, srcs = setup
+    N = 5  # not a multiple of any G * rb > 1: padded right-hand sides included
+    for pr in (g.double, g.single):
+        w = g.qcd.fermion.wilson_clover(U, P, precision=pr, n_rhs=N)
+        w1 = g.qcd.fermion.wilson_clover(U, P, precision=pr)
+        single = _fields([s.astype(pr.complex_dtype) for s in srcs[:N]], w1.F_grid)
+        H = pack([g.pick_checkerboard(g.odd, x) for x in single])
+        mpc = pc.eo2_ne()(w).Mpc
+        rbs = (1, 2) if pr is g.single else (1,)
+        shapes = [(rb, tpb, occ, seg) for rb in rbs for tpb in (1, 3, 8) for occ in (0, 4) for seg in (1, 2, 4)]
+        ref = None
+        for sh in shapes:
+            w.set_hop_shape(sh)
+            assert w.hop_shape() == sh
+            out = (w.Meooe(H).data.clone(), mpc(H).data.clone())
+            if ref is None:
+                ref = out
+            else:
+                assert torch.equal(out[0], ref[0]) and torch.equal(out[1], ref[1]), (pr.__name__, sh)
+        # a CG (hop partials feed alpha) is identical under two shapes
+        res = []
+        for sh in [(1, 1, 0, 1), (rbs[-1], 3, 8, 4)]:
+            w.set_hop_shape(sh)
+            cgN = inv.cg(eps=1e-6 if pr is g.single else 1e-10, maxiter=500)
+            X = g(w.propagator(inv.preconditioned(pc.eo2_ne(), cgN)) * pack(single))
+            res.append((X.data.clone(), list(cgN.iterations)))
+        assert res[0][1] == res[1][1] and torch.equal(res[0][0], res[1][0])
+        w.set_hop_shape(None)
+        with pytest.raises(RuntimeError, match="invalid"):
+            w.set_hop_shape((3, 1, 0, 1))
+        with pytest.raises(RuntimeError, match="invalid"):
+            w.set_hop_shape((1, 9, 0, 1))
+        with pytest.raises(RuntimeError, match="invalid"):
+            w.set_hop_shape((1, 1, 0, 5))  # 8^3 x 16: 4 blocks per time slice
+
+
+def test_tuner_picks_a_valid_shape(setup):
+    gr, U, srcs = setup
+    w = g.qcd.fermion.wilson_clover(U, P, precision=g.single, n_rhs=4)
+    sh, ms = w.tune_hop(reps=2)
+    assert ms > 0 and sh[0] in (1, 2) and sh[1] in (1, 2, 4, 8) and sh[2] in (0, 2, 4, 8, 16) and sh[3] in (1, 4)
+    assert w.hop_shape() == sh

@@ -1,0 +1,161 @@
+"""Continuous batching (wc_batch_*) and the serving loop (gpt_amd.serve).
+
+Checks, with the oracle's FP64 operator as the referee:
+* every solve leaves the batch with a true residual |src - M x| / |src| at
+  or below its tolerance, evaluated by the numpy oracle;
+* a right-hand side admitted while others are mid-solve, and one solved in a
+  batch of others, agree with the same solve done alone to 1e-8 relative
+  (both are solves to tol = 1e-10; the reliable-update points depend on the
+  batch, so the solutions agree to within a few condition numbers times tol);
+* the server serves requests for two resident fields and several masses,
+  with an HBM budget small enough that requests must wait for memory.
+"""
+import numpy as np
+import pytest
+
+from oracle import gauge as G
+from oracle import rng as R
+from oracle import wilson_clover as W
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no GPU", allow_module_level=True)
+
+import gpt_amd as g  # noqa: E402
+from gpt_amd import serve  # noqa: E402
+
+P = dict(csw_r=1.0, csw_t=1.0, cF=1.0, xi_0=1.0, nu=1.0, isAnisotropic=False, boundary_phases=[1, 1, 1, -1])
+TOL = 1e-10
+AGREE = 1e-8
+
+
+@pytest.fixture(scope="module")
+def setup():
+    gr = R.grid([8, 8, 8, 16])
+    rng = R.random("serve")
+    U = [G.random_links(gr, rng, scale=0.5), G.random_links(gr, rng, scale=0.4)]
+    srcs = [rng.vspincolor(gr) for _ in range(8)]
+    return gr, U, srcs
+
+
+def field(arr, grid):
+    x = g.lattice(grid)
+    x.from_numpy(arr)
+    return x
+
+
+def true_res(ref, x, src):
+    return np.linalg.norm(src - ref.M(x)) / np.linalg.norm(src)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def solve_alone(Uh, mass, src_arr):
+    w = g.qcd.fermion.wilson_clover(Uh, dict(P, mass=mass))
+    inner = w.converted(g.single)
+    b = serve.batch(w, inner, 1)
+    s = field(src_arr, w.F_grid)
+    b.admit(s, TOL, 10000)
+    while not b.step(50):
+        pass
+    x = g.lattice(w.F_grid)
+    it, tres, conv = b.extract(0, x)
+    assert conv and tres <= TOL
+    b.close()
+    return x.to_numpy()
+
+
+def test_batch_slots_and_mid_flight_admission(setup):
+    gr, U, srcs = setup
+    mass = 0.1
+    ref = W.wilson_clover(U[0], mass=mass, **P)
+    w = g.qcd.fermion.wilson_clover(U[0], dict(P, mass=mass))
+    inner = w.converted(g.single)
+    b = serve.batch(w, inner, 4)
+    S = [field(s, w.F_grid) for s in srcs[:4]]
+    owner = {}
+    owner[b.admit(S[0], TOL, 10000)] = 0
+    owner[b.admit(S[1], TOL, 10000)] = 1
+    assert sorted(owner) == [0, 1]
+    assert b.step(7) == []  # both mid-solve
+    owner[b.admit(S[2], TOL, 10000)] = 2  # joins while 0 and 1 iterate
+    results = {}
+    for _ in range(400):
+        for sl in b.step(10):
+            x = g.lattice(w.F_grid)
+            it, tres, conv = b.extract(sl, x)
+            assert conv and 0 < tres <= TOL and it > 0
+            results[owner.pop(sl)] = x.to_numpy()
+            if 3 not in results and 3 not in owner.values():
+                # the freed slot is taken by a new right-hand side at once
+                slot = b.admit(S[3], TOL, 10000)
+                assert slot == sl
+                owner[slot] = 3
+        if len(results) == 4:
+            break
+    assert b.occupied() == 0 and sorted(results) == [0, 1, 2, 3]
+    for k in range(4):
+        assert true_res(ref, results[k], srcs[k]) <= 1.05 * TOL
+        assert rel(results[k], solve_alone(U[0], mass, srcs[k])) < AGREE
+    b.close()
+
+
+def test_batch_reports_iteration_limit_and_zero_source(setup):
+    gr, U, srcs = setup
+    w = g.qcd.fermion.wilson_clover(U[0], dict(P, mass=0.1))
+    inner = w.converted(g.single)
+    b = serve.batch(w, inner, 2)
+    z = g.lattice(w.F_grid)
+    z[:] = 0
+    b.admit(z, TOL, 100)  # zero source: done at admission
+    b.admit(field(srcs[0], w.F_grid), TOL, 5)  # cannot converge in 5 iterations
+    fin = []
+    for _ in range(10):
+        fin = b.step(10)
+        if len(fin) == 2:
+            break
+    assert sorted(fin) == [0, 1]
+    x = g.lattice(w.F_grid)
+    it0, t0, c0 = b.extract(0, x)
+    assert c0 and it0 == 0 and np.abs(x.to_numpy()).max() == 0.0
+    it1, t1, c1 = b.extract(1, x)
+    assert not c1 and it1 >= 5 and t1 > TOL
+    with pytest.raises(RuntimeError, match="no finished solve"):
+        b.extract(1, x)
+    b.close()
+
+
+def test_server_mixed_fields_masses_and_memory_wait(setup):
+    gr, U, srcs = setup
+    Uh = {"a": U[0], "b": U[1]}
+    nslots = 4
+    budget = int(1.5 * serve.group_bytes([8, 8, 8, 16], nslots))  # one group at a time
+    srv = serve.server(P, nslots=nslots, streams=2, iters_per_step=10, hbm_budget=budget)
+    for k, u in Uh.items():
+        srv.add_field(k, u)
+    plan = [("a", 0.1, [0, 1, 2]), ("b", 0.15, [3]), ("a", 0.1, [4]), ("a", 0.2, [5, 6]), ("b", 0.15, [7, 0])]
+    reqs = []
+    schedule = []
+    for n, (f, m, idx) in enumerate(plan):
+        grid = g.grid([8, 8, 8, 16], g.double)
+        r = serve.request(f, m, [field(srcs[i], grid) for i in idx], tol=TOL, maxiter=10000)
+        r.idx = idx
+        reqs.append(r)
+        schedule.append((0.002 * n, r))
+    done = srv.run(schedule)
+    srv.close()
+    assert sorted(r.id for r in done) == sorted(r.id for r in reqs)
+    assert srv.peak_groups == 1 and srv.waited_for_memory > 0
+    for r in reqs:
+        assert all(r.converged) and r.latency > 0
+        ref = W.wilson_clover(Uh[r.field], mass=r.mass, **P)
+        for j, i in enumerate(r.idx):
+            x = r.results[j].to_numpy()
+            assert true_res(ref, x, srcs[i]) <= 1.05 * TOL
+            assert rel(x, solve_alone(Uh[r.field], r.mass, srcs[i])) < AGREE
+    st = serve.latency_stats(done)
+    assert st["p50_s"] <= st["p99_s"]
