@@ -24,6 +24,20 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0
 
 
+def profiled_traffic(shape):
+    """HBM bytes per launch of the headline hop from the committed rocprofv3
+    summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), when the
+    launch shape in use is the profiled one; else None"""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hop_fp64_18_32x32x32x64.md")
+    try:
+        kv = dict(line.split(": ", 1) for line in open(path).read().splitlines() if line.startswith("traffic_"))
+        if tuple(int(x) for x in kv["traffic_shape"].split(",")) == tuple(shape):
+            return int(kv["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        pass
+    return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -44,6 +58,8 @@ def parse():
     ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--serve", type=int, default=0, help="serving stream: number of requests (0: skip)")
+    ap.add_argument("--shape", default=None,
+                    help="hop launch shape rb,tpb,occ,seg for the Dhop variants instead of tuning")
     return ap.parse_args()
 
 
@@ -201,7 +217,11 @@ def main():
             d = g.lattice(w.F_grid_eo)
             d.checkerboard(g.even)
             pairs.append((s, d))
-        shape, _ = w.tune_hop(nrhs, reps=3)
+        if args.shape:
+            w.set_hop_shape(tuple(int(x) for x in args.shape.split(",")))
+            shape = w.hop_shape()
+        else:
+            shape, _ = w.tune_hop(nrhs, reps=3)
         t_ev, t_wall = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
         real_b = prec.nbytes
         nl = 12 if comp else 18
@@ -235,6 +255,11 @@ def main():
 
     head = res.get("fp64_18") or next(iter(res.values()), {"GBps": 0.0, "us_per_call": 0.0})
     achieved = head["GBps"]
+    traffic = None
+    if "fp64_18" in res and [int(x) for x in fd] == [32, 32, 32, 64]:
+        sh = head["shape"]
+        traffic = profiled_traffic((sh["rhs_per_thread"], sh["threads_per_block"] // 64, sh["max_blocks_per_cu"],
+                                    sh["t_runs"]))
     out = {
         "metric": "EO Dhop GB/s (% HBM peak); mixed-prec CG solves/s + p99 latency at 1e-10",
         "value": achieved,
@@ -252,7 +277,7 @@ def main():
                                f"{args.pairs} rotating in/out pairs",
                    "lattice": fd, "half_sites": Vh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "cpu_baseline": cpu,
         "variants": res,
         "copy_kernel_GBps": round(copy_gbs, 1),
