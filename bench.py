@@ -57,7 +57,7 @@ def parse():
     ap.add_argument("--cg-lattice", default="24,24,24,48")
     ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
     ap.add_argument("--nrhs", type=int, default=12)
-    ap.add_argument("--serve", type=int, default=0, help="serving stream: number of requests (0: skip)")
+    ap.add_argument("--serve", type=int, default=12, help="serving stream: number of requests (0: skip)")
     ap.add_argument("--shape", default=None,
                     help="hop launch shape rb,tpb,occ,seg for the Dhop variants instead of tuning")
     return ap.parse_args()
@@ -375,7 +375,26 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
                           "true_residual": max(res), "rel_error_vs_fp64_solution": max(err),
                           "s_per_batch": tN, "solves_per_s": float(nrhs / tN)}
     log(f"bench: cg mixed {nrhs} {out['mixed_multi']}")
+    # (d) the same with 16-bit storage of the inner search direction and Schur intermediates
+    for key, op_, S_, nr in [("mixed_1rhs_half", w, srcs[0], 1), ("mixed_multi_half", wN, S, nrhs)]:
+        mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half")
+        slvh = op_.propagator(inv.preconditioned(pc.eo2_ne(), mh))
+        g(slvh * S_)
+        lat = []
+        for _ in range(2):
+            X, t = timed(lambda: g(slvh * S_))
+            lat.append(t)
+        res = true_residual(g, op_, X, S_)
+        xs = [X] if nr == 1 else [X.rhs(i) for i in range(2)]
+        err = [float(np.sqrt(g.norm2(g(xs[i] - x64[i])) / g.norm2(x64[i]))) for i in range(len(xs))]
+        th = float(np.median(lat))
+        out[key] = {"iterations": mh.iterations, "reliable_updates": mh.reliable_updates,
+                    "true_residual": max(res), "rel_error_vs_fp64_solution": max(err),
+                    "s_per_batch": th, "solves_per_s": float(nr / th)}
+        log(f"bench: cg {key} {out[key]}")
     out["speedup_mixed_multi_vs_fp64"] = out["mixed_multi"]["solves_per_s"] / out["fp64_cg"]["solves_per_s"]
+    out["speedup_mixed_multi_half_vs_fp64"] = (out["mixed_multi_half"]["solves_per_s"]
+                                               / out["fp64_cg"]["solves_per_s"])
     return out
 
 

@@ -12,6 +12,7 @@ LIB_PATH = os.environ.get("GPT_AMD_LIB", os.path.join(_HERE, "libgpt_amd_wc.so")
 
 WC_OK, WC_ERR_ARG, WC_ERR_HIP, WC_ERR_NUMERIC = 0, 1, 2, 3
 WC_DOUBLE, WC_SINGLE = 0, 1
+WC_STORAGE_NATIVE, WC_STORAGE_HALF = 0, 2
 
 
 class wc_params(ctypes.Structure):
@@ -65,7 +66,7 @@ SIGNATURES = {
     "wc_apply_mpc_block": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "wc_cg_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
     "wc_solve_eo2_ne_block": (_I, [_P, _I, _I, _P, _P, _D, _L, _P, _P, _P, _P, _P, _P]),
-    "wc_solve_eo2_ne_mixed": (_I, [_P, _P, _I, _I, _P, _P, _D, _L, _D, _P, _P, _P, _P, _P]),
+    "wc_solve_eo2_ne_mixed": (_I, [_P, _P, _I, _I, _P, _P, _D, _L, _D, _I, _P, _P, _P, _P, _P]),
     "wc_tune_hop": (_I, [_P, _I, _I, _P, _P, _P]),
     "wc_set_hop_shape": (_I, [_P, _I, _P]),
     "wc_get_hop_shape": (_I, [_P, _I, _P]),

@@ -14,6 +14,12 @@
   solution and the true residuals; used as
   ``preconditioned(eo2_ne(), cg_mixed(eps=1e-10))`` it stops on the full
   system's true relative residual |src - M dst| / |src| <= eps.
+  ``inner_storage="half"`` stores the inner search direction and the Schur
+  intermediates as 16-bit floats (FP32 arithmetic); ``inner_compress_links``
+  (default True) gives the inner operator 12-real links.  The inner operator
+  only drives the recursion between reliable updates -- every true residual
+  and the stopping test use the exact FP64 operator -- so approximations in
+  it change the speed, never the result.
 """
 import ctypes
 
@@ -126,8 +132,8 @@ class cg_mixed:
     def __init__(self, params=None, **kw):
         from .. import core as _core
 
-        p = {"eps": 1e-10, "maxiter": 100000, "inner_precision": _core.single, "delta": 0.1,
-             "fail_if_not_converged": False}
+        p = {"eps": 1e-10, "maxiter": 100000, "inner_precision": _core.single, "delta": 0.01,
+             "inner_storage": None, "inner_compress_links": True, "fail_if_not_converged": False}
         p.update(params or {})
         p.update(kw)
         self.params = p
@@ -135,6 +141,12 @@ class cg_mixed:
         self.maxiter = p["maxiter"]
         self.inner_precision = p["inner_precision"]
         self.delta = p["delta"]
+        if p["inner_storage"] not in (None, "half"):
+            raise ValueError("inner_storage must be None or 'half'")
+        if p["inner_storage"] == "half" and self.inner_precision is not _core.single:
+            raise ValueError("16-bit inner storage needs inner_precision=single")
+        self.inner_storage = p["inner_storage"]
+        self.inner_compress_links = bool(p["inner_compress_links"])
         self.fail_if_not_converged = p["fail_if_not_converged"]
         self.iterations = []
         self.true_residual = []
@@ -145,7 +157,12 @@ class cg_mixed:
     def inner_operator(self, op):
         key = id(op)
         if key not in self._inner:
-            self._inner[key] = (op, op.converted(self.inner_precision))
+            if self.inner_compress_links:
+                params = {**op.params_constructor, "compress_links": True}
+                inner = type(op)(op.name, op.U, params, op.daggered, self.inner_precision)
+            else:
+                inner = op.converted(self.inner_precision)
+            self._inner[key] = (op, inner)
         return self._inner[key][1]
 
     def solve(self, op, parity, dst, src):
@@ -157,8 +174,9 @@ class cg_mixed:
         nupd = np.zeros(1, dtype=np.int64)
         _lib.check(_lib.load().wc_solve_eo2_ne_mixed(
             op.handle, inner.handle, parity.tag, N, dst.ptr(), src.ptr(), float(self.eps),
-            int(self.maxiter), float(self.delta), iters.ctypes.data, conv.ctypes.data, tres.ctypes.data,
-            nupd.ctypes.data, core.stream_ptr()))
+            int(self.maxiter), float(self.delta),
+            _lib.WC_STORAGE_HALF if self.inner_storage == "half" else _lib.WC_STORAGE_NATIVE,
+            iters.ctypes.data, conv.ctypes.data, tres.ctypes.data, nupd.ctypes.data, core.stream_ptr()))
         self.iterations = [int(x) for x in iters]
         self.true_residual = [float(x) for x in tres]
         self.reliable_updates = int(nupd[0])

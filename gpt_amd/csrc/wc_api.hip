@@ -391,8 +391,17 @@ static int solve_impl(wc_operator *op, int nrhs, int pd, void *dst, const void *
 // reliable update: |R src - Mpc y| <= tol |src| -- after the L, S
 // reconstruction this is exactly the full system's true residual
 // |src - M dst| / |src| (the C rows are solved exactly).
+//
+// storage16 (FP32 inner operator): the search direction and the Schur
+// intermediates are stored as 16-bit floats.  Each right-hand side's inner
+// state is then kept in a frame scaled by s_r, chosen at every reliable
+// update so that the new residual has unit RMS per component (s_r =
+// sqrt(12 Vh / |r_h|^2)); the search direction is carried over by s_new /
+// s_old, and the inner solution is folded back with 1 / s.  alpha and beta
+// do not change under a common scaling of r and p.
 static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst, const void *src, double tol,
-                      long maxiter, double delta, long *iters, int *conv, double *tres, long *nupd, hipStream_t s) {
+                      long maxiter, double delta, int storage16, long *iters, int *conv, double *tres, long *nupd,
+                      hipStream_t s) {
   const int pc = 1 - pd;
   int rc;
   if ((rc = ensure_tmp(hi, 9, N))) return rc;
@@ -406,6 +415,11 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   const long n = hi->g.Vh * 12 * N;
   double *sth = hi->st, *hsh = hi->host, *stl = lo->st, *hsl = lo->host;
   const bool lo_single = lo->prec == WC_SINGLE;
+  const int h16 = (storage16 && lo_single) ? 1 : 0;
+  // 16-bit storage resolves about 5e-4 of a field's scale: asking one inner
+  // cycle for more than a 1e-2 reduction makes the recursion stagnate or
+  // break down, so reliable updates come at least every 1e-2
+  if (h16) delta = std::max(delta, 1e-2);
 
   // b_s = R src = src_D - Meooe A_C^-1 src_C ; |src|^2 per RHS
   site_op<double>(hi, N, pc, 1, half(src, pc), tC, nullptr, s);
@@ -419,20 +433,30 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   wck::reduce_final_rhs(hi->partials, wck::kRedBlocks, N, sth + 8 * N, s);
   // guess: y = L^-1 dst = A_D dst_D
   site_op<double>(hi, N, pd, 0, half(dst, pd), xh, nullptr, s);
+  if (lo_single) wck::zero<float>(n, xl, s);
+  else wck::zero<double>(n, xl, s);
   WC_LAUNCHED();
 
-  std::vector<double> src2(N), maxr(N), active(N, 1.0);
+  // ratio: |r_s|^2 / |r_h|^2 at the last reliable update, to predict from the
+  // iterated |r_h|^2 when a right-hand side meets tol (and check it then)
+  std::vector<double> src2(N), maxr(N), active(N, 1.0), scale(N, 1.0), ratio(N, 0.0);
   int nactive = N;
   long updates = 0;
-  // reliable update: r_s = b_s - Mpc y, true = |r_s|^2, r_h = Mpc^dag r_s,
-  // r_lo = r_h, c = |r_h|^2 (into the inner CG state)
-  auto reliable = [&](long k) -> int {
+  // scale rows in hi->st: 11 s_new, 12 s_new / s_old, 13 1 / s (for the next fold)
+  for (int i = 0; i < N; ++i) hsh[13 * N + i] = 1.0;
+  WC_HIP(hipMemcpyAsync(sth + 13 * N, hsh + 13 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  // reliable update: y += x_inner / s, x_inner = 0; r_s = b_s - Mpc y,
+  // true = |r_s|^2, r_h = Mpc^dag r_s, r_inner = s r_h, c = s^2 |r_h|^2
+  auto reliable = [&](long k, bool first) -> int {
     int rc2;
+    if (h16) wck::accumulate_f2d_scaled(n, N, sth + 13 * N, xl, xh, s);
+    else if (lo_single) wck::accumulate_f2d(n, xl, xh, s);
+    else wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
+    if (lo_single) wck::zero<float>(n, xl, s);
+    else wck::zero<double>(n, xl, s);
     if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s))) return rc2;
     wck::axpy<double>(n, -1.0, 0.0, mp, bs, rs, s);
     if ((rc2 = mpc_impl<double>(hi, N, pd, 1, rh, rs, nullptr, s))) return rc2;
-    if (lo_single) wck::convert_d2f(n, rh, rl, s);
-    else WC_HIP(hipMemcpyAsync(rl, rh, hb, hipMemcpyDeviceToDevice, s));
     if ((rc2 = norm2_rhs<double>(hi, N, rs, sth + 9 * N, s))) return rc2;
     if ((rc2 = norm2_rhs<double>(hi, N, rh, sth + 10 * N, s))) return rc2;
     WC_HIP(hipMemcpyAsync(hsh + 8 * N, sth + 8 * N, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
@@ -445,16 +469,38 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
       if (!std::isfinite(t2) || !std::isfinite(c))
         WC_FAIL(WC_ERR_NUMERIC, "mixed CG: true residual of right-hand side %d became %g", i, t2);
       if (tres) tres[i] = src2[i] > 0 ? std::sqrt(t2 / src2[i]) : 0.0;
+      ratio[i] = c > 0.0 ? t2 / c : 0.0;
       if (t2 <= tol * tol * src2[i]) {
         active[i] = 0.0;
         --nactive;
         if (iters) iters[i] = k;
         if (conv) conv[i] = 1;
       }
-      maxr[i] = c;
     }
-    // c of the inner recursion = |r_h|^2 for the still active right-hand sides
-    for (int i = 0; i < N; ++i) hsl[i] = active[i] != 0.0 ? hsh[10 * N + i] : 0.0;
+    // the new frame of every right-hand side (1 without 16-bit storage)
+    for (int i = 0; i < N; ++i) {
+      const double c = hsh[10 * N + i];
+      const double sn = (h16 && active[i] != 0.0 && c > 0.0) ? std::sqrt(12.0 * (double)hi->g.Vh / c) : 1.0;
+      hsh[11 * N + i] = sn;
+      hsh[12 * N + i] = sn / scale[i];
+      hsh[13 * N + i] = 1.0 / sn;
+      scale[i] = sn;
+      maxr[i] = c * sn * sn;
+    }
+    if (h16) {
+      WC_HIP(hipMemcpyAsync(sth + 11 * N, hsh + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+      wck::convert_d2f_scaled(n, N, sth + 11 * N, rh, rl, s);
+      if (!first) {
+        wck::scale_rhs_half(n, N, sth + 12 * N, pl, s);
+        wck::scale_rhs_half(n, N, sth + 12 * N, aDl, s);
+      }
+    } else if (lo_single) {
+      wck::convert_d2f(n, rh, rl, s);
+    } else {
+      WC_HIP(hipMemcpyAsync(rl, rh, hb, hipMemcpyDeviceToDevice, s));
+    }
+    // c of the inner recursion (in the new frame) for the still active right-hand sides
+    for (int i = 0; i < N; ++i) hsl[i] = active[i] != 0.0 ? maxr[i] : 0.0;
     WC_HIP(hipMemcpyAsync(stl + 0 * N, hsl, sizeof(double) * N, hipMemcpyHostToDevice, s));
     for (int i = 0; i < N; ++i) hsl[5 * N + i] = active[i];
     WC_HIP(hipMemcpyAsync(stl + 5 * N, hsl + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
@@ -465,16 +511,19 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     if (conv) conv[i] = 0;
     if (tres) tres[i] = 0.0;
   }
-  if ((rc = reliable(0))) return rc;
-  // zero sources converge at once (the true residual is 0 <= 0)
-  if (lo_single) wck::zero<float>(n, xl, s);
-  else wck::zero<double>(n, xl, s);
-  WC_HIP(hipMemcpyAsync(pl, rl, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
-  if (lo_single) site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s);
-  else site_op<double>(lo, N, pd, 1, pl, aDl, nullptr, s);
+  if ((rc = reliable(0, true))) return rc;
+  // zero sources converge at once (the true residual is 0 <= 0); p = r, A_D^-1 p
+  if (h16) {
+    wck::convert_f2h(n, rl, pl, s);
+    site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s, 1);
+  } else {
+    WC_HIP(hipMemcpyAsync(pl, rl, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
+    if (lo_single) site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s);
+    else site_op<double>(lo, N, pd, 1, pl, aDl, nullptr, s);
+  }
   long k = 0;
   for (; k < maxiter && nactive > 0; ++k) {
-    rc = lo_single ? inner_iteration<float>(lo, N, pd, rl, pl, ml, xl, aDl, s)
+    rc = lo_single ? inner_iteration<float>(lo, N, pd, rl, pl, ml, xl, aDl, s, nullptr, nullptr, h16)
                    : inner_iteration<double>(lo, N, pd, rl, pl, ml, xl, aDl, s);
     if (rc) return rc;
     WC_HIP(hipMemcpyAsync(hsl + 4 * N, stl + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
@@ -486,21 +535,15 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
       if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "mixed CG: residual of right-hand side %d became %g", i, cp);
       if (cp > maxr[i]) maxr[i] = cp;
       if (cp < delta * delta * maxr[i]) trigger = true;
+      // predicted to meet tol: check it now rather than at the next delta step
+      if (ratio[i] * cp / (scale[i] * scale[i]) <= tol * tol * src2[i]) trigger = true;
     }
-    if (trigger) {
-      // fold the inner solution into the FP64 one, restart its accumulation
-      if (lo_single) wck::accumulate_f2d(n, xl, xh, s);
-      else wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
-      if (lo_single) wck::zero<float>(n, xl, s);
-      else wck::zero<double>(n, xl, s);
-      if ((rc = reliable(k + 1))) return rc;
-    }
+    // fold the inner solution into the FP64 one, restart its accumulation
+    if (trigger && (rc = reliable(k + 1, false))) return rc;
   }
   if (nactive > 0) {
     // not converged within maxiter: fold what the inner solve has and report
-    if (lo_single) wck::accumulate_f2d(n, xl, xh, s);
-    else wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
-    if ((rc = reliable(k))) return rc;
+    if ((rc = reliable(k, false))) return rc;
     for (int i = 0; i < N; ++i)
       if (active[i] != 0.0 && iters) iters[i] = k;
   }
@@ -730,8 +773,8 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
 }
 
 int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int nrhs, void *dst, const void *src,
-                          double tol, long maxiter, double delta, long *iterations, int *converged,
-                          double *true_residual, long *reliable_updates, void *stream) {
+                          double tol, long maxiter, double delta, int inner_storage, long *iterations,
+                          int *converged, double *true_residual, long *reliable_updates, void *stream) {
   if (!op || !inner || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: null argument");
   if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: dst must not alias src");
   if (op->prec != WC_DOUBLE) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: the outer operator must be FP64");
@@ -745,8 +788,12 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
   if (rc) return rc;
   if ((rc = maybe_tune(op, nrhs, (hipStream_t)stream))) return rc;
   if ((rc = maybe_tune(inner, nrhs, (hipStream_t)stream))) return rc;
-  return mixed_impl(op, inner, nrhs, parity, dst, src, tol, maxiter, delta, iterations, converged, true_residual,
-                    reliable_updates, (hipStream_t)stream);
+  if (inner_storage != WC_STORAGE_NATIVE && inner_storage != WC_STORAGE_HALF)
+    WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: inner_storage %d unknown", inner_storage);
+  if (inner_storage == WC_STORAGE_HALF && inner->prec != WC_SINGLE)
+    WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: 16-bit storage needs an FP32 inner operator");
+  return mixed_impl(op, inner, nrhs, parity, dst, src, tol, maxiter, delta, inner_storage == WC_STORAGE_HALF,
+                    iterations, converged, true_residual, reliable_updates, (hipStream_t)stream);
 }
 
 int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,

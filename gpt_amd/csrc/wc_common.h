@@ -23,6 +23,27 @@ template <typename R> struct c2;
 template <> struct c2<double> { using t = double2; };
 template <> struct c2<float> { using t = float2; };
 
+// 16-bit storage of a complex FP32 value (4 bytes), used for the search
+// direction and the Schur intermediates of the mixed-precision inner solve;
+// arithmetic is always done after widening to FP32
+struct __align__(4) c2h {
+  _Float16 x, y;
+};
+// complex type a stored value is computed in
+template <typename S> struct wide;
+template <> struct wide<double2> { using t = double2; };
+template <> struct wide<float2> { using t = float2; };
+template <> struct wide<c2h> { using t = float2; };
+template <typename C> __host__ __device__ __forceinline__ C widen(C v) { return v; }
+__device__ __forceinline__ float2 widen(c2h v) { return make_float2((float)v.x, (float)v.y); }
+template <typename S, typename C> __device__ __forceinline__ S narrow(C v) { return v; }
+template <> __device__ __forceinline__ c2h narrow<c2h, float2>(float2 v) {
+  c2h o;
+  o.x = (_Float16)v.x;
+  o.y = (_Float16)v.y;
+  return o;
+}
+
 template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b) {
   C r;
   r.x = a;
@@ -152,6 +173,11 @@ template <int AUX> struct bload_t<float2, AUX> {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
   }
 };
+template <int AUX> struct bload_t<c2h, AUX> {
+  static __device__ __forceinline__ c2h go(wc_rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(c2h, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
+  }
+};
 template <typename C, int AUX = 0>
 __device__ __forceinline__ C bload(wc_rsrc r, uint32_t voff, uint32_t soff) {
   return bload_t<C, AUX>::go(r, voff, soff);
@@ -168,6 +194,11 @@ template <int AUX> struct bstore_t<double2, AUX> {
 template <int AUX> struct bstore_t<float2, AUX> {
   static __device__ __forceinline__ void go(float2 v, wc_rsrc r, uint32_t voff, uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(wc_v2i, v), r, (int)voff, (int)soff, AUX);
+  }
+};
+template <int AUX> struct bstore_t<c2h, AUX> {
+  static __device__ __forceinline__ void go(c2h v, wc_rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, AUX);
   }
 };
 

@@ -13,6 +13,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "wc_kernels.h"
 #include "wc_site.h"
 
@@ -119,7 +121,7 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], 
 // wave-uniform switch.  Neighbour offsets come from uniform strides of the
 // half-site index h = ((t*Lz + z)*Ly + y)*Lxh + x/2.  rl[j] is the
 // right-hand side a spinor is read for (clamped into range for padded lanes).
-template <typename R, int NL, int DAG, int RB>
+template <typename R, int NL, int DAG, int RB, typename SI>
 __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int rl[RB], const int cx,
                                          const int cy, const int cz, const int ct,
                                          typename c2<R>::t acc[RB][12]) {
@@ -130,13 +132,13 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
   const Geom &g = a.g;
   const uint32_t N = (uint32_t)a.nrhs;
   const uint32_t hh = (uint32_t)h;
-  const wc_rsrc in = make_rsrc(a.in, 12u * (uint32_t)g.Vh * N * (uint32_t)sizeof(C));
+  const wc_rsrc in = make_rsrc(a.in, 12u * (uint32_t)g.Vh * N * (uint32_t)sizeof(SI));
   // a wave's sites are consecutive and lie in at most two aligned 64-site
   // blocks: its links are addressed from the first active lane's block
   const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
   const wc_rsrc rw = make_rsrc((const C *)a.W + blk * (8 * NL * kBlk), 2u * 8u * NL * kBlk * (uint32_t)sizeof(C));
   const uint32_t vw = (uint32_t)(aidx_site(h, 8 * NL) - blk * (8 * NL * kBlk)) * (uint32_t)sizeof(C);
-  const uint32_t kstride = (uint32_t)kBlk * N * (uint32_t)sizeof(C);
+  const uint32_t kstride = (uint32_t)kBlk * N * (uint32_t)sizeof(SI);
   const uint32_t xh = (uint32_t)cx >> 1;
   const uint32_t st1 = (uint32_t)g.Lxh, st2 = st1 * (uint32_t)g.L[1], st3 = st2 * (uint32_t)g.L[2];
 #pragma unroll 1
@@ -164,9 +166,9 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
     C psi[RB][12];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
-      const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(C);
+      const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(SI);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) psi[j][k] = bload<C, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride);
+      for (int k = 0; k < 12; ++k) psi[j][k] = widen(bload<SI, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride));
     }
     switch (dir) {
       case 0: compute_dir<R, NL, DAG, 0, 0, RB>(a, c, u, psi, acc); break;
@@ -211,7 +213,7 @@ template <typename R, int RB> constexpr int hop_min_waves() { return (sizeof(R) 
 // writes each (site, right-hand side) product to dots[h * N + r]; dot_blocks
 // sums them per 64-site block in site order, so the sums do not depend on
 // the launch shape.
-template <typename R, int NL, int DAG, int EPI, int RB>
+template <typename R, int NL, int DAG, int EPI, int RB, typename SI, typename SO>
 __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const HopLaunch a) {
   using C = typename c2<R>::t;
   const Geom &g = a.g;
@@ -234,7 +236,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   for (int j = 0; j < RB; ++j)
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
-  hop_site<R, NL, DAG, RB>(a, h, rl, c[0], c[1], c[2], c[3], acc);
+  hop_site<R, NL, DAG, RB, SI>(a, h, rl, c[0], c[1], c[2], c[3], acc);
   // epilogue
   const bool zb = a.open_bc && (c[3] == 0 || c[3] == g.L[3] - 1);
   if (zb) {
@@ -255,7 +257,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   if (EPI & (EPI_DIAG | EPI_AXPY)) {
     C y[RB][12];
 #pragma unroll
-    for (int j = 0; j < RB; ++j) load12((const C *)a.y, h, rl[j], N, y[j]);
+    for (int j = 0; j < RB; ++j) load12((const SI *)a.y, h, rl[j], N, y[j]);
     if (EPI & EPI_DIAG) {
       if (!zb) {
         if (a.diag_clov) {
@@ -282,7 +284,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
       C v[12];
-      load12((const C *)a.dotv, h, rl[j], N, v);
+      load12((const SI *)a.dotv, h, rl[j], N, v);
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < 12; ++k)
@@ -292,19 +294,35 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   }
 #pragma unroll
   for (int j = 0; j < RB; ++j)
-    if (rb + j < N) store12_stream((C *)a.out, g.Vh, h, rb + j, N, acc[j]);
+    if (rb + j < N) store12_stream((SO *)a.out, g.Vh, h, rb + j, N, acc[j]);
 }
 
-template <typename R, int NL, int DAG, int RB>
-static void hop_epi(const HopLaunch &a, hipStream_t s) {
+// epilogues instantiated per storage: all of them in the working precision;
+// with 16-bit in/y/dotv those the normal operator of the 16-bit inner solve
+// uses (CLOV, AXPY to 16-bit, CLOV|AXPY|DOT to FP32)
+template <typename R, typename SI, typename SO> constexpr bool epi_instantiated(int e) {
+  using C = typename c2<R>::t;
+  if (std::is_same<SI, C>::value && std::is_same<SO, C>::value) return true;
+  if (std::is_same<SI, c2h>::value && std::is_same<SO, c2h>::value) return e == EPI_CLOV || e == EPI_AXPY;
+  if (std::is_same<SI, c2h>::value && std::is_same<SO, C>::value) return e == (EPI_CLOV | EPI_AXPY | EPI_DOT);
+  return false;
+}
+
+template <typename R, int NL, int DAG, int RB, typename SI, typename SO>
+static bool hop_epi(const HopLaunch &a, hipStream_t s) {
   const long threads = kBlk * a.shape.tpb;
   const long items = a.g.Vh * (long)((a.nrhs + RB - 1) / RB);
   const unsigned grid = (unsigned)((items + threads - 1) / threads);
   // LDS reserved only to cap the workgroups per CU (the kernel uses none)
   const size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
   switch (a.epi) {
-#define WC_EPI(E) \
-  case E: k_hop<R, NL, DAG, E, RB><<<grid, (unsigned)threads, lds, s>>>(a); break;
+#define WC_EPI(E)                                                                        \
+  case E:                                                                                \
+    if constexpr (epi_instantiated<R, SI, SO>(E)) {                                     \
+      k_hop<R, NL, DAG, E, RB, SI, SO><<<grid, (unsigned)threads, lds, s>>>(a);          \
+      return true;                                                                       \
+    }                                                                                    \
+    return false;
     WC_EPI(0)
     WC_EPI(EPI_CLOV)
     WC_EPI(EPI_DIAG)
@@ -313,31 +331,39 @@ static void hop_epi(const HopLaunch &a, hipStream_t s) {
     WC_EPI(EPI_CLOV | EPI_AXPY | EPI_DOT)
     WC_EPI(EPI_AXPY | EPI_DOT)
 #undef WC_EPI
-    default: break;
+    default: return false;
   }
 }
 
-template <typename R, int NL, int DAG>
-static void hop_rb(const HopLaunch &a, hipStream_t s) {
+template <typename R, int NL, int DAG, typename SI, typename SO>
+static bool hop_rb(const HopLaunch &a, hipStream_t s) {
   switch (a.shape.rb) {
-    case 1: hop_epi<R, NL, DAG, 1>(a, s); break;
-    case 2: if constexpr (max_rb<R>() >= 2) hop_epi<R, NL, DAG, 2>(a, s); break;
-    default: break;
+    case 1: return hop_epi<R, NL, DAG, 1, SI, SO>(a, s);
+    case 2:
+      if constexpr (max_rb<R>() >= 2) return hop_epi<R, NL, DAG, 2, SI, SO>(a, s);
+      return false;
+    default: return false;
   }
+}
+
+template <typename R, typename SI, typename SO> static bool hop_links(const HopLaunch &a, hipStream_t s) {
+  if (a.nl == 9) return a.dag ? hop_rb<R, 9, 1, SI, SO>(a, s) : hop_rb<R, 9, 0, SI, SO>(a, s);
+  return a.dag ? hop_rb<R, 6, 1, SI, SO>(a, s) : hop_rb<R, 6, 0, SI, SO>(a, s);
 }
 
 template <typename R> bool hop_launch(const HopLaunch &a, hipStream_t s) {
+  using C = typename c2<R>::t;
   const HopShape &sh = a.shape;
   if (sh.tpb < 1 || sh.tpb > kMaxTpb || sh.rb < 1 || sh.rb > max_rb<R>() || sh.occ < 0 || sh.occ > kMaxOcc ||
       a.nrhs < 1 || a.nrhs > 64 || a.g.Vh % kBlk)
     return false;
   if (sh.seg < 1 || (sh.seg > 1 && (a.bps <= 0 || sh.seg > a.bps))) return false;
-  if (a.nl == 9) {
-    if (a.dag) hop_rb<R, 9, 1>(a, s); else hop_rb<R, 9, 0>(a, s);
-  } else {
-    if (a.dag) hop_rb<R, 6, 1>(a, s); else hop_rb<R, 6, 0>(a, s);
+  if (!a.half_in && !a.half_out) return hop_links<R, C, C>(a, s);
+  if constexpr (sizeof(R) == 4) {
+    if (a.half_in && a.half_out) return hop_links<R, c2h, c2h>(a, s);
+    if (a.half_in) return hop_links<R, c2h, C>(a, s);
   }
-  return true;
+  return false;
 }
 
 }  // namespace wck

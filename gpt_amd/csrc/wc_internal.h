@@ -211,20 +211,24 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
 // out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
 template <typename R>
 inline void site_op(const wc_operator *op, int nrhs, int p, int inv, const void *in, void *out, const void *y,
-                    hipStream_t s) {
+                    hipStream_t s, int half = 0) {
   const void *B = op->has_clover ? (inv ? op->Ai[p] : op->A[p]) : nullptr;
   double sc = inv ? 1.0 / op->diag : op->diag;
-  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s);
+  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s, half);
 }
 
 // Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd on nrhs
 // right-hand sides; dotv (optional) makes the last hop write per-block,
 // per-RHS partials of Re<dotv, out>.  tD_in (optional, which 0 or 2): a
-// field already holding A_D^-1 in.  Scratch: tmp[0..2].
+// field already holding A_D^-1 in.  half (FP32, which 2): in, dotv, tD_in
+// and the intermediates are stored as 16-bit c2h, out in FP32.
+// Scratch: tmp[0..2].
 template <typename R>
 inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
-                    hipStream_t s, const void *tD_in = nullptr, double *partials = nullptr) {
+                    hipStream_t s, const void *tD_in = nullptr, double *partials = nullptr, int half = 0) {
   if (!partials) partials = op->partials;
+  if (half && (sizeof(R) != 4 || which != 2))
+    WC_FAIL(WC_ERR_ARG, "mpc: 16-bit storage is for the FP32 normal operator only");
   const int pc = 1 - pd;
   int rc = ensure_tmp(op, 3, nrhs);
   if (rc) return rc;
@@ -237,17 +241,19 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
   if (which == 0 || which == 2) {
     const void *aD = tD_in;
     if (!aD) {
-      site_op<R>(op, nrhs, pd, 1, src, tD, nullptr, s);
+      site_op<R>(op, nrhs, pd, 1, src, tD, nullptr, s, half);
       aD = tD;
     }
     HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV, aD, tC);
     a.clov = clovC;
+    a.half_in = a.half_out = half;
     WC_HOP(R, a, s);
     HopLaunch b = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC,
                               dst_mpc);
     b.y = src;
     b.dotv = dotv;
     b.dots = op->dots;
+    b.half_in = b.half_out = half;
     WC_HOP(R, b, s);
     if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s);
     src = dst_mpc;
@@ -255,6 +261,7 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
   if (which == 1 || which == 2) {
     HopLaunch a = hop_args<R>(op, nrhs, pc, 1, wck::EPI_CLOV, src, tC);
     a.clov = clovC;
+    a.half_in = a.half_out = half;
     WC_HOP(R, a, s);
     HopLaunch b = hop_args<R>(op, nrhs, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC,
                               out);
@@ -262,6 +269,7 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
     b.y = src;
     b.dotv = dotv;
     b.dots = op->dots;
+    b.half_in = half;  // the product itself stays in the working precision
     WC_HOP(R, b, s);
     if (dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s);
   }
@@ -285,17 +293,17 @@ inline int norm2_rhs(wc_operator *op, int nrhs, const void *x, double *dev_out, 
 // r -= alpha mmp, beta, x += alpha p, p = beta p + r, aD = A_D^-1 p
 template <typename R>
 inline int inner_iteration(wc_operator *lo, int N, int pd, void *r, void *p, void *mmp, void *x, void *aD,
-                           hipStream_t s, double *st = nullptr, double *partials = nullptr) {
+                           hipStream_t s, double *st = nullptr, double *partials = nullptr, int half = 0) {
   int rc;
   const long n = lo->g.Vh * 12 * N;
   if (!st) st = lo->st;
   if (!partials) partials = lo->partials;
-  if ((rc = mpc_impl<R>(lo, N, pd, 2, mmp, p, p, s, aD, partials))) return rc;
+  if ((rc = mpc_impl<R>(lo, N, pd, 2, mmp, p, p, s, aD, partials, half))) return rc;
   wck::cg_alpha(partials, wck::hop_blocks(lo->g), N, st, s);
   wck::cg_update_r<R>(n, N, st, r, mmp, partials, s);
   wck::cg_beta(partials, wck::kRedBlocks, N, st, s);
   wck::cg_update_xp<R>(lo->g, N, pd, st, r, p, x, lo->has_clover ? lo->Ai[pd] : nullptr, 1.0 / lo->diag, aD,
-                       lo->open_bc ? 1 : 0, s);
+                       lo->open_bc ? 1 : 0, s, half);
   WC_LAUNCHED();
   return WC_OK;
 }

@@ -55,6 +55,8 @@ struct HopLaunch {
   double ct;         // temporal hop coefficient -1/2
   double ph[4][2];   // boundary phases
   int open_bc;
+  int half_in;       // FP32 hops: in, y and dotv stored as 16-bit c2h
+  int half_out;      // FP32 hops: out stored as 16-bit c2h
   HopShape shape;    // launch shape (1 <= tpb <= kMaxTpb, 1 <= rb <= max_rb, 0 <= occ <= kMaxOcc)
   int bps;           // 64-site blocks per time slice (0: not a whole number; then seg must be 1)
 };
@@ -76,10 +78,11 @@ constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
 template <typename R> bool hop(const HopLaunch &a, hipStream_t s);
 
 // out = A in (blocks != null) or diag * in on parity p, for nrhs right-hand
-// sides; zeroes the open boundary; optional axpy (out = y - A in)
+// sides; zeroes the open boundary; optional axpy (out = y - A in).  half:
+// FP32 only, in and out stored as 16-bit c2h (y must be null).
 template <typename R>
 void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
-                 const void *y, int open_bc, hipStream_t s);
+                 const void *y, int open_bc, hipStream_t s, int half = 0);
 
 // Per-right-hand-side reductions over fields whose innermost index is the
 // RHS (nrhs of them): partials[r * kRedBlocks + b] for b < kRedBlocks, then
@@ -144,9 +147,18 @@ void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStrea
 template <typename R>
 void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s);
 // x += alpha_r p ; p = beta_r p + r ; t = A p with A the clover-inverse
-// blocks of parity par (or diag when null), zero on an open boundary
+// blocks of parity par (or diag when null), zero on an open boundary.
+// half: FP32 only, p and t stored as 16-bit c2h.
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
-                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s);
+                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half = 0);
+
+// per-right-hand-side scalings of the 16-bit-storage inner solve (n complex
+// entries, RHS innermost): dst = (float) (src * f[r]); x16 *= f[r];
+// hi += (double) lo * f[r]; dst16 = src
+void convert_d2f_scaled(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s);
+void scale_rhs_half(long n, int nrhs, const double *f, void *x, hipStream_t s);
+void accumulate_f2d_scaled(long n, int nrhs, const double *f, const void *lo, void *hi, hipStream_t s);
+void convert_f2h(long n, const void *src, void *dst, hipStream_t s);
 
 }  // namespace wck

@@ -12,11 +12,11 @@ namespace wck {
 int site_blocks(long Vh) { return (int)((Vh + kSiteThreads - 1) / kSiteThreads); }
 
 // ------------------------------------------------------- site-local clover
-template <typename R>
+// S: storage of in and out (y, when given, in the same storage)
+template <typename R, typename S>
 __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int p, const typename c2<R>::t *A,
-                                                             double diag, const typename c2<R>::t *in,
-                                                             typename c2<R>::t *out,
-                                                             const typename c2<R>::t *y, int open_bc) {
+                                                             double diag, const S *in, S *out, const S *y,
+                                                             int open_bc) {
   using C = typename c2<R>::t;
   const long t = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
   const long h = (N == 1) ? t : t / N;
@@ -50,16 +50,23 @@ __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int
 
 template <typename R>
 void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
-                 const void *y, int open_bc, hipStream_t s) {
+                 const void *y, int open_bc, hipStream_t s, int half) {
   using C = typename c2<R>::t;
-  k_site_clover<R><<<site_blocks(g.Vh * nrhs), kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag,
-                                                                     (const C *)in, (C *)out, (const C *)y,
-                                                                     open_bc);
+  const int nb = site_blocks(g.Vh * nrhs);
+  if constexpr (sizeof(R) == 4) {
+    if (half) {
+      k_site_clover<R, c2h><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const c2h *)in,
+                                                        (c2h *)out, (const c2h *)y, open_bc);
+      return;
+    }
+  }
+  k_site_clover<R, C><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const C *)in, (C *)out,
+                                                  (const C *)y, open_bc);
 }
 template void site_clover<double>(const Geom &, int, int, const void *, double, const void *, void *,
-                                  const void *, int, hipStream_t);
+                                  const void *, int, hipStream_t, int);
 template void site_clover<float>(const Geom &, int, int, const void *, double, const void *, void *,
-                                 const void *, int, hipStream_t);
+                                 const void *, int, hipStream_t, int);
 
 // ------------------------------------------------------------------ BLAS-1
 static int elem_blocks(long n) { return (int)((n + 255) / 256); }
@@ -670,12 +677,13 @@ template void cg_update_r<double>(long, int, const double *, void *, const void 
 template void cg_update_r<float>(long, int, const double *, void *, const void *, double *, hipStream_t);
 
 // x += alpha_r p ; p = beta_r p + r ; t = A p (site-local, parity par: the
-// clover inverse blocks A, or diag when A is null; zero on an open boundary)
-template <typename R>
+// clover inverse blocks A, or diag when A is null; zero on an open boundary).
+// SP: storage of p and t.
+template <typename R, typename SP>
 __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, int par, const double *st,
-                                                              const typename c2<R>::t *r, typename c2<R>::t *p,
+                                                              const typename c2<R>::t *r, SP *p,
                                                               typename c2<R>::t *x, const typename c2<R>::t *A,
-                                                              double diag, typename c2<R>::t *t, int open_bc) {
+                                                              double diag, SP *t, int open_bc) {
   using C = typename c2<R>::t;
   const long tid = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
   const long h = (N == 1) ? tid : tid / N;
@@ -686,12 +694,15 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
   load12(p, h, rr, N, pv);
   load12(x, h, rr, N, v);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = cmk<C>(v[k].x + a * pv[k].x, v[k].y + a * pv[k].y);
+  for (int k = 0; k < 12; ++k) v[k] = cmk<C>(fma(a, pv[k].x, v[k].x), fma(a, pv[k].y, v[k].y));
   store12(x, h, rr, N, v);
   load12(r, h, rr, N, v);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) pv[k] = cmk<C>(b * pv[k].x + v[k].x, b * pv[k].y + v[k].y);
+  for (int k = 0; k < 12; ++k) pv[k] = cmk<C>(fma(b, pv[k].x, v[k].x), fma(b, pv[k].y, v[k].y));
   store12(p, h, rr, N, pv);
+  // the clover acts on p as stored (rounded to SP), as a hop reading p sees it
+#pragma unroll
+  for (int k = 0; k < 12; ++k) pv[k] = widen(narrow<SP>(pv[k]));
   bool zb = false;
   if (open_bc) {
     int c[4];
@@ -711,14 +722,60 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
 }
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
-                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s) {
+                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half) {
   using C = typename c2<R>::t;
-  k_cg_update_xp<R><<<site_blocks(g.Vh * nrhs), kSiteThreads, 0, s>>>(
-      g, nrhs, par, st, (const C *)r, (C *)p, (C *)x, (const C *)blocks, diag, (C *)t, open_bc);
+  const int nb = site_blocks(g.Vh * nrhs);
+  if constexpr (sizeof(R) == 4) {
+    if (half) {
+      k_cg_update_xp<R, c2h><<<nb, kSiteThreads, 0, s>>>(g, nrhs, par, st, (const C *)r, (c2h *)p, (C *)x,
+                                                         (const C *)blocks, diag, (c2h *)t, open_bc);
+      return;
+    }
+  }
+  k_cg_update_xp<R, C><<<nb, kSiteThreads, 0, s>>>(g, nrhs, par, st, (const C *)r, (C *)p, (C *)x,
+                                                   (const C *)blocks, diag, (C *)t, open_bc);
 }
 template void cg_update_xp<double>(const Geom &, int, int, const double *, const void *, void *, void *,
-                                   const void *, double, void *, int, hipStream_t);
+                                   const void *, double, void *, int, hipStream_t, int);
 template void cg_update_xp<float>(const Geom &, int, int, const double *, const void *, void *, void *,
-                                  const void *, double, void *, int, hipStream_t);
+                                  const void *, double, void *, int, hipStream_t, int);
+
+// ------------------------------------------- 16-bit-storage inner solve
+__global__ void k_d2f_scaled(long n, int N, const double *f, const double2 *src, float2 *dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double a = f[i % N];
+  dst[i] = make_float2((float)(src[i].x * a), (float)(src[i].y * a));
+}
+void convert_d2f_scaled(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s) {
+  if (n > 0) k_d2f_scaled<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (const double2 *)src, (float2 *)dst);
+}
+__global__ void k_scale_half(long n, int N, const double *f, c2h *x) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float a = (float)f[i % N];
+  const float2 v = widen(x[i]);
+  x[i] = narrow<c2h>(make_float2(a * v.x, a * v.y));
+}
+void scale_rhs_half(long n, int nrhs, const double *f, void *x, hipStream_t s) {
+  if (n > 0) k_scale_half<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (c2h *)x);
+}
+__global__ void k_acc_f2d_scaled(long n, int N, const double *f, const float2 *lo, double2 *hi) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double a = f[i % N];
+  const double2 h = hi[i];
+  hi[i] = make_double2(fma((double)lo[i].x, a, h.x), fma((double)lo[i].y, a, h.y));
+}
+void accumulate_f2d_scaled(long n, int nrhs, const double *f, const void *lo, void *hi, hipStream_t s) {
+  if (n > 0) k_acc_f2d_scaled<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (const float2 *)lo, (double2 *)hi);
+}
+__global__ void k_f2h(long n, const float2 *src, c2h *dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = narrow<c2h>(src[i]);
+}
+void convert_f2h(long n, const void *src, void *dst, hipStream_t s) {
+  if (n > 0) k_f2h<<<elem_blocks(n), 256, 0, s>>>(n, (const float2 *)src, (c2h *)dst);
+}
 
 }  // namespace wck

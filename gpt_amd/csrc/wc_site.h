@@ -69,27 +69,29 @@ __device__ __forceinline__ void clov_apply_inplace(const C *A, long h, C v[RB][1
   }
 }
 
-// spinor of site h, right-hand side r of N: entry k at aidx(k, h, 12) * N + r
-template <typename C>
-__device__ __forceinline__ void load12(const C *f, long h, int r, int N, C v[12]) {
-  const C *p = f + aidx_site(h, 12) * N + r;
+// spinor of site h, right-hand side r of N: entry k at aidx(k, h, 12) * N + r,
+// stored as S (double2, float2 or the 16-bit c2h) and computed in C
+template <typename S, typename C>
+__device__ __forceinline__ void load12(const S *f, long h, int r, int N, C v[12]) {
+  const S *p = f + aidx_site(h, 12) * N + r;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) v[k] = p[(long)k * kBlk * N];
+  for (int k = 0; k < 12; ++k) v[k] = widen(p[(long)k * kBlk * N]);
 }
-template <typename C>
-__device__ __forceinline__ void store12(C *f, long h, int r, int N, const C v[12]) {
-  C *p = f + aidx_site(h, 12) * N + r;
+template <typename S, typename C>
+__device__ __forceinline__ void store12(S *f, long h, int r, int N, const C v[12]) {
+  S *p = f + aidx_site(h, 12) * N + r;
 #pragma unroll
-  for (int k = 0; k < 12; ++k) p[(long)k * kBlk * N] = v[k];
+  for (int k = 0; k < 12; ++k) p[(long)k * kBlk * N] = narrow<S>(v[k]);
 }
 // output of a hop: written once, not re-read by the kernel
-template <typename C>
-__device__ __forceinline__ void store12_stream(C *f, long Vh, long h, int r, int N, const C v[12]) {
+template <typename S, typename C>
+__device__ __forceinline__ void store12_stream(S *f, long Vh, long h, int r, int N, const C v[12]) {
 #if WC_NT_STORE
-  const wc_rsrc rs = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)N * (uint32_t)sizeof(C)));
-  const uint32_t vo = (uint32_t)(aidx_site(h, 12) * N + r) * (uint32_t)sizeof(C);
+  const wc_rsrc rs = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)N * (uint32_t)sizeof(S)));
+  const uint32_t vo = (uint32_t)(aidx_site(h, 12) * N + r) * (uint32_t)sizeof(S);
 #pragma unroll
-  for (int k = 0; k < 12; ++k) bstore_t<C, 2>::go(v[k], rs, vo, (uint32_t)(k * kBlk * N * sizeof(C)));
+  for (int k = 0; k < 12; ++k)
+    bstore_t<S, 2>::go(narrow<S>(v[k]), rs, vo, (uint32_t)(k * kBlk * N * sizeof(S)));
 #else
   store12(f, h, r, N, v);
 #endif

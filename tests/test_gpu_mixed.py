@@ -1,8 +1,9 @@
 """Mixed-precision reliable-update solve (FP32 inner CG, FP64 solution and
-true residuals): the true residual of the full system, recomputed here with
-the oracle's FP64 operator, meets the requested tolerance; the solution
-agrees with the FP64 CG one; a right-hand side gives the same result alone
-and inside a batch."""
+true residuals), with the inner state in FP32 or in 16-bit storage: the true
+residual of the full system, recomputed here with the oracle's FP64
+operator, meets the requested tolerance; the solution agrees with the FP64
+CG one to 1e-8; a right-hand side gives the same result alone and inside a
+batch (to 1e-8: both are solves to 1e-10)."""
 import numpy as np
 import pytest
 
@@ -40,12 +41,16 @@ def _dev(a, grid):
     return x
 
 
-def test_true_residual_and_fp64_agreement(setup):
+STORAGE = [None, "half"]
+
+
+@pytest.mark.parametrize("storage", STORAGE)
+def test_true_residual_and_fp64_agreement(setup, storage):
     fd, U, srcs = setup
     ref = W.wilson_clover(U, **P)
     w = g.qcd.fermion.wilson_clover(U, P)
     tol = 1e-10
-    mcg = inv.cg_mixed(eps=tol, maxiter=5000)
+    mcg = inv.cg_mixed(eps=tol, maxiter=5000, inner_storage=storage)
     slv = w.propagator(inv.preconditioned(pc.eo2_ne(), mcg))
     x = g(slv * _dev(srcs[0], w.F_grid)).to_numpy()
     assert mcg.converged and mcg.reliable_updates >= 2
@@ -58,16 +63,17 @@ def test_true_residual_and_fp64_agreement(setup):
     assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-8
 
 
-def test_alone_equals_in_batch(setup):
+@pytest.mark.parametrize("storage", STORAGE)
+def test_alone_equals_in_batch(setup, storage):
     fd, U, srcs = setup
     ref = W.wilson_clover(U, **P)
     tol = 1e-10
     w1 = g.qcd.fermion.wilson_clover(U, P)
     w3 = g.qcd.fermion.wilson_clover(U, P, n_rhs=3)
-    m1 = inv.cg_mixed(eps=tol, maxiter=5000)
+    m1 = inv.cg_mixed(eps=tol, maxiter=5000, inner_storage=storage)
     s1 = w1.propagator(inv.preconditioned(pc.eo2_ne(), m1))
     alone = [g(s1 * _dev(s, w1.F_grid)).to_numpy() for s in srcs]
-    m3 = inv.cg_mixed(eps=tol, maxiter=5000)
+    m3 = inv.cg_mixed(eps=tol, maxiter=5000, inner_storage=storage)
     s3 = w3.propagator(inv.preconditioned(pc.eo2_ne(), m3))
     batch = unpack(g(s3 * pack([_dev(s, w1.F_grid) for s in srcs])))
     assert m3.converged
@@ -77,3 +83,19 @@ def test_alone_equals_in_batch(setup):
         assert true <= 1.05 * tol
         # both meet the tolerance; they agree to the tolerance-level accuracy
         assert np.linalg.norm(b - alone[i]) / np.linalg.norm(alone[i]) < 1e-8
+
+
+def test_half_storage_point_source_and_errors(setup):
+    """a point source (wide dynamic range across sites) converges with 16-bit
+    inner storage; 16-bit storage needs the FP32 inner operator"""
+    fd, U, srcs = setup
+    ref = W.wilson_clover(U, **P)
+    w = g.qcd.fermion.wilson_clover(U, P)
+    pt = np.zeros_like(srcs[0])
+    pt[0, 0, 0, 0, 0, 0] = 1.0
+    mcg = inv.cg_mixed(eps=1e-10, maxiter=5000, inner_storage="half")
+    x = g(w.propagator(inv.preconditioned(pc.eo2_ne(), mcg)) * _dev(pt, w.F_grid)).to_numpy()
+    assert mcg.converged
+    assert np.linalg.norm(ref.M(x) - pt) / np.linalg.norm(pt) <= 1.05e-10
+    with pytest.raises(ValueError):
+        inv.cg_mixed(eps=1e-10, inner_precision=g.double, inner_storage="half")
