@@ -573,19 +573,18 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
   wck::zero<R>(n, op->tmp[0], s);
   wck::zero<R>(n, op->tmp[1], s);
   std::vector<wck::HopShape> cand;
+  // t-marching order (seg > 1) has never won on MI355X and is left to
+  // wc_set_hop_shape
   const int occs[] = {0, 2, 4, 8, 16};
-  const int segs[] = {1, 4, 8, 16};
-  for (int rb = 1; rb <= max_rb_prec(op->prec); ++rb)
+  for (int rb = 1; rb <= max_rb_prec(op->prec) && rb <= N; ++rb)
     for (int tpb = 1; tpb <= wck::kMaxTpb; tpb *= 2)
-      for (int oc : occs)
-        for (int sg : segs) {
-          wck::HopShape sh;
-          sh.rb = rb;
-          sh.tpb = tpb;
-          sh.occ = oc;
-          sh.seg = sg;
-          if (shape_valid(op, sh)) cand.push_back(sh);
-        }
+      for (int oc : occs) {
+        wck::HopShape sh;
+        sh.rb = rb;
+        sh.tpb = tpb;
+        sh.occ = oc;
+        if (shape_valid(op, sh)) cand.push_back(sh);
+      }
   hipEvent_t e0, e1;
   WC_HIP(hipEventCreate(&e0));
   WC_HIP(hipEventCreate(&e1));
@@ -629,7 +628,8 @@ static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *m
 }
 
 // Solvers tune the hop on first use for lattices where launch shapes matter
-// (>= 2^17 site-RHS pairs; WC_AUTOTUNE=0 turns it off).
+// (>= 2^17 site-RHS pairs; WC_AUTOTUNE=0 turns it off), once per lattice,
+// precision, link format and class of right-hand-side counts.
 int maybe_tune(wc_operator *op, int nrhs, hipStream_t s) {
   if (op->shape_set[nrhs]) return WC_OK;
   {

@@ -36,6 +36,12 @@ template <typename CS, typename CD> __global__ void k_to_slot(long n, int N, int
   long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e < n) dst[e * N + slot] = cvt<CS, CD>(src[e]);
 }
+// dst slot = narrow(a * src), src FP64 single-RHS, dst FP32 or 16-bit
+template <typename CD>
+__global__ void k_to_slot_scaled(long n, int N, int slot, double a, const double2 *src, CD *dst) {
+  long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) dst[e * N + slot] = narrow<CD>(make_float2((float)(a * src[e].x), (float)(a * src[e].y)));
+}
 template <typename C> __global__ void k_from_slot(long n, int N, int slot, const C *src, C *dst) {
   long e = (long)blockIdx.x * 256 + threadIdx.x;
   if (e < n) dst[e] = src[e * N + slot];
@@ -56,6 +62,7 @@ inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 struct wc_batch {
   wc_operator *hi = nullptr, *lo = nullptr;
   int N = 0, pd = 1;
+  int h16 = 0;  // 16-bit storage of the inner p and Schur intermediates
   double delta = 0.1;
   hipStream_t s = nullptr;
   // N-RHS half fields: FP64 y (solution of Mpc y = R src), R src, scratch
@@ -68,6 +75,9 @@ struct wc_batch {
   std::vector<int> occupied, active, finished, conv;
   std::vector<long> iters, maxiter;
   std::vector<double> tol, src2, maxr, tres;
+  // per-slot frame of the 16-bit inner state (1 otherwise) and the ratio
+  // |r_s|^2 / |r_h|^2 of the last reliable update (convergence predictor)
+  std::vector<double> scale, ratio;
   long updates = 0;
 };
 
@@ -93,8 +103,11 @@ static int ensure_single(wc_batch *b) {
   return ensure_red(b->lo, 1);
 }
 
-// FP64: y += x_inner for every slot, x_inner = 0, then r_s = b_s - Mpc y,
-// r_h = Mpc^dag r_s, r_inner = r_h; |r_s|^2 and |r_h|^2 per slot to the host
+// FP64: y += x_inner / s for every slot, x_inner = 0, then r_s = b_s - Mpc y,
+// r_h = Mpc^dag r_s, r_inner = s r_h in the slot's new frame s (1 without
+// 16-bit storage; p and A_D^-1 p carried over by s_new / s_old); |r_s|^2
+// and |r_h|^2 per slot to the host.  Scale rows of st: 11 s_new, 12
+// s_new / s_old, 13 1 / s.
 static int batch_reliable(wc_batch *b) {
   wc_operator *hi = b->hi, *lo = b->lo;
   const int N = b->N;
@@ -102,15 +115,14 @@ static int batch_reliable(wc_batch *b) {
   hipStream_t s = b->s;
   int rc;
   const bool single = lo->prec == WC_SINGLE;
-  if (single) wck::accumulate_f2d(n, b->xl, b->xh, s);
+  if (b->h16) wck::accumulate_f2d_scaled(n, N, b->st + 13 * N, b->xl, b->xh, s);
+  else if (single) wck::accumulate_f2d(n, b->xl, b->xh, s);
   else wck::axpy<double>(n, 1.0, 0.0, b->xl, b->xh, b->xh, s);
   if (single) wck::zero<float>(n, b->xl, s);
   else wck::zero<double>(n, b->xl, s);
   if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, nullptr, b->partials))) return rc;
   wck::axpy<double>(n, -1.0, 0.0, b->mp, b->bs, b->rs, s);
   if ((rc = mpc_impl<double>(hi, N, b->pd, 1, b->rh, b->rs, nullptr, s, nullptr, b->partials))) return rc;
-  if (single) wck::convert_d2f(n, b->rh, b->rl, s);
-  else WC_HIP(hipMemcpyAsync(b->rl, b->rh, hbytes(hi) * N, hipMemcpyDeviceToDevice, s));
   wck::dot_partials_rhs<double>(n, N, b->rs, b->rs, b->partials, nullptr, s);
   wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
   wck::dot_partials_rhs<double>(n, N, b->rh, b->rh, b->partials, nullptr, s);
@@ -120,26 +132,42 @@ static int batch_reliable(wc_batch *b) {
   WC_HIP(hipStreamSynchronize(s));
   ++b->updates;
   for (int i = 0; i < N; ++i) {
-    if (!b->active[i]) continue;
     const double t2 = b->host[8 * N + i], c = b->host[9 * N + i];
-    if (!std::isfinite(t2) || !std::isfinite(c))
-      WC_FAIL(WC_ERR_NUMERIC, "wc_batch: true residual of slot %d became %g", i, t2);
-    b->tres[i] = b->src2[i] > 0 ? std::sqrt(t2 / b->src2[i]) : 0.0;
-    b->maxr[i] = c;
-    if (t2 <= b->tol[i] * b->tol[i] * b->src2[i]) {
-      b->active[i] = 0;
-      b->finished[i] = 1;
-      b->conv[i] = 1;
-    } else if (b->iters[i] >= b->maxiter[i]) {
-      b->active[i] = 0;
-      b->finished[i] = 1;
-      b->conv[i] = 0;
+    double sn = 1.0;
+    if (b->active[i]) {
+      if (!std::isfinite(t2) || !std::isfinite(c))
+        WC_FAIL(WC_ERR_NUMERIC, "wc_batch: true residual of slot %d became %g", i, t2);
+      b->tres[i] = b->src2[i] > 0 ? std::sqrt(t2 / b->src2[i]) : 0.0;
+      b->ratio[i] = c > 0.0 ? t2 / c : 0.0;
+      if (t2 <= b->tol[i] * b->tol[i] * b->src2[i]) {
+        b->active[i] = 0;
+        b->finished[i] = 1;
+        b->conv[i] = 1;
+      } else if (b->iters[i] >= b->maxiter[i]) {
+        b->active[i] = 0;
+        b->finished[i] = 1;
+        b->conv[i] = 0;
+      }
+      if (b->h16 && b->active[i] && c > 0.0) sn = std::sqrt(12.0 * (double)hi->g.Vh / c);
     }
-  }
-  // c = |r_h|^2 for the slots that go on, active flags (st rows 0 and 5)
-  for (int i = 0; i < N; ++i) {
-    b->host[i] = b->active[i] ? b->host[9 * N + i] : 0.0;
+    b->host[11 * N + i] = sn;
+    b->host[12 * N + i] = sn / b->scale[i];
+    b->host[13 * N + i] = 1.0 / sn;
+    b->scale[i] = sn;
+    b->maxr[i] = c * sn * sn;
+    // c = s^2 |r_h|^2 for the slots that go on, active flags (st rows 0 and 5)
+    b->host[i] = b->active[i] ? b->maxr[i] : 0.0;
     b->host[5 * N + i] = b->active[i] ? 1.0 : 0.0;
+  }
+  if (b->h16) {
+    WC_HIP(hipMemcpyAsync(b->st + 11 * N, b->host + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
+    wck::convert_d2f_scaled(n, N, b->st + 11 * N, b->rh, b->rl, s);
+    wck::scale_rhs_half(n, N, b->st + 12 * N, b->pl, s);
+    wck::scale_rhs_half(n, N, b->st + 12 * N, b->aD, s);
+  } else if (single) {
+    wck::convert_d2f(n, b->rh, b->rl, s);
+  } else {
+    WC_HIP(hipMemcpyAsync(b->rl, b->rh, hbytes(hi) * N, hipMemcpyDeviceToDevice, s));
   }
   WC_HIP(hipMemcpyAsync(b->st, b->host, sizeof(double) * N, hipMemcpyHostToDevice, s));
   WC_HIP(hipMemcpyAsync(b->st + 5 * N, b->host + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
@@ -148,8 +176,8 @@ static int batch_reliable(wc_batch *b) {
 
 extern "C" {
 
-int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots, double delta, void *stream,
-                    wc_batch **out) {
+int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots, double delta, int inner_storage,
+                    void *stream, wc_batch **out) {
   if (!op || !inner || !out) WC_FAIL(WC_ERR_ARG, "wc_batch_create: null argument");
   *out = nullptr;
   if (op->prec != WC_DOUBLE) WC_FAIL(WC_ERR_ARG, "wc_batch_create: the outer operator must be FP64");
@@ -158,6 +186,10 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
     if (op->g.L[mu] != inner->g.L[mu]) WC_FAIL(WC_ERR_ARG, "wc_batch_create: operators of different lattices");
   if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_batch_create: parity must be 0 or 1");
   if (!(delta > 0.0 && delta < 1.0)) WC_FAIL(WC_ERR_ARG, "wc_batch_create: need 0 < delta < 1");
+  if (inner_storage != WC_STORAGE_NATIVE && inner_storage != WC_STORAGE_HALF)
+    WC_FAIL(WC_ERR_ARG, "wc_batch_create: inner_storage %d unknown", inner_storage);
+  if (inner_storage == WC_STORAGE_HALF && inner->prec != WC_SINGLE)
+    WC_FAIL(WC_ERR_ARG, "wc_batch_create: 16-bit storage needs an FP32 inner operator");
   int rc = check_nrhs(op, nslots, "wc_batch_create");
   if (rc) return rc;
   wc_batch *b = new wc_batch();
@@ -165,7 +197,10 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
   b->lo = inner;
   b->N = nslots;
   b->pd = parity;
-  b->delta = delta;
+  b->h16 = inner_storage == WC_STORAGE_HALF ? 1 : 0;
+  // 16-bit storage resolves about 5e-4 of a field's scale: reliable updates
+  // at least every 1e-2 of residual reduction
+  b->delta = b->h16 ? std::max(delta, 1e-2) : delta;
   b->s = (hipStream_t)stream;
   const int N = nslots;
   const size_t hh = hbytes(op) * N, hl = hbytes(inner) * N;
@@ -203,6 +238,7 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
     else wck::zero<double>(n, q, s);
   }
   std::memset(b->host, 0, sizeof(double) * 16 * N);
+  for (int i = 0; i < N; ++i) b->host[13 * N + i] = 1.0;  // 1 / s of the empty slots
   if (hipMemcpyAsync(b->st, b->host, sizeof(double) * 16 * N, hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     free_batch(b);
@@ -217,6 +253,8 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
   b->tol.assign(N, 0.0);
   b->src2.assign(N, 0.0);
   b->maxr.assign(N, 0.0);
+  b->scale.assign(N, 1.0);
+  b->ratio.assign(N, 0.0);
   b->tres.assign(N, 0.0);
   *out = b;
   return WC_OK;
@@ -258,10 +296,33 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
   if ((rc = mpc_impl<double>(hi, 1, pd, 0, mp1, y1, nullptr, s))) return rc;
   wck::axpy<double>(n1, -1.0, 0.0, mp1, bs1, rs1, s);
   if ((rc = mpc_impl<double>(hi, 1, pd, 1, rh1, rs1, nullptr, s))) return rc;
-  // into the slot: y, b_s; inner r = p = r_h, x = 0, A_D^-1 p
+  // |src|^2 (both parities), |r_s|^2, |r_h|^2
+  double *st1 = hi->st;
+  if ((rc = norm2_rhs<double>(hi, 1, src, st1 + 0, s))) return rc;
+  // the full field is two half fields: add the second half's norm
+  if ((rc = norm2_rhs<double>(hi, 1, half(src, 1), st1 + 1, s))) return rc;
+  if ((rc = norm2_rhs<double>(hi, 1, rs1, st1 + 2, s))) return rc;
+  if ((rc = norm2_rhs<double>(hi, 1, rh1, st1 + 3, s))) return rc;
+  double h4[4];
+  WC_HIP(hipMemcpyAsync(h4, st1, sizeof h4, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  const double s2 = h4[0] + h4[1], t2 = h4[2], c = h4[3];
+  if (!std::isfinite(t2) || !std::isfinite(c)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch_admit: initial residual is %g", t2);
+  const bool done = (t2 <= tol * tol * s2) || maxiter == 0;
+  // the slot's frame: unit RMS per component of the inner residual (16-bit storage)
+  const double sn = (b->h16 && !done && c > 0.0) ? std::sqrt(12.0 * (double)hi->g.Vh / c) : 1.0;
+  // into the slot: y, b_s; inner r = p = s r_h, x = 0, A_D^-1 p
   k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)y1, (double2 *)b->xh);
   k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)bs1, (double2 *)b->bs);
-  if (lo->prec == WC_SINGLE) {
+  if (b->h16) {
+    k_to_slot_scaled<float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, sn, (const double2 *)rh1, (float2 *)b->rl);
+    // p as 16-bit in a single-RHS field, its A_D^-1 from that (as the batch computes it)
+    k_to_slot_scaled<c2h><<<nblk(n1), 256, 0, s>>>(n1, 1, 0, sn, (const double2 *)rh1, (c2h *)pl1);
+    site_op<float>(lo, 1, pd, 1, pl1, aD1, nullptr, s, 1);
+    k_to_slot<c2h, c2h><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const c2h *)pl1, (c2h *)b->pl);
+    k_zero_slot<float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (float2 *)b->xl);
+    k_to_slot<c2h, c2h><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const c2h *)aD1, (c2h *)b->aD);
+  } else if (lo->prec == WC_SINGLE) {
     wck::convert_d2f(n1, rh1, pl1, s);
     site_op<float>(lo, 1, pd, 1, pl1, aD1, nullptr, s);
     k_to_slot<float2, float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const float2 *)pl1, (float2 *)b->rl);
@@ -276,20 +337,6 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
     k_to_slot<double2, double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)aD1, (double2 *)b->aD);
   }
   WC_LAUNCHED();
-  // |src|^2 (both parities), |r_s|^2, |r_h|^2
-  double *st1 = hi->st;
-  if ((rc = norm2_rhs<double>(hi, 1, src, st1 + 0, s))) return rc;
-  {
-    // the full field is two half fields: add the second half's norm
-    if ((rc = norm2_rhs<double>(hi, 1, half(src, 1), st1 + 1, s))) return rc;
-  }
-  if ((rc = norm2_rhs<double>(hi, 1, rs1, st1 + 2, s))) return rc;
-  if ((rc = norm2_rhs<double>(hi, 1, rh1, st1 + 3, s))) return rc;
-  double h4[4];
-  WC_HIP(hipMemcpyAsync(h4, st1, sizeof h4, hipMemcpyDeviceToHost, s));
-  WC_HIP(hipStreamSynchronize(s));
-  const double s2 = h4[0] + h4[1], t2 = h4[2], c = h4[3];
-  if (!std::isfinite(t2) || !std::isfinite(c)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch_admit: initial residual is %g", t2);
   b->occupied[slot] = 1;
   b->finished[slot] = 0;
   b->conv[slot] = 0;
@@ -297,18 +344,20 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
   b->maxiter[slot] = maxiter;
   b->tol[slot] = tol;
   b->src2[slot] = s2;
-  b->maxr[slot] = c;
+  b->scale[slot] = sn;
+  b->ratio[slot] = c > 0.0 ? t2 / c : 0.0;
+  b->maxr[slot] = c * sn * sn;
   b->tres[slot] = s2 > 0 ? std::sqrt(t2 / s2) : 0.0;
-  const bool done = (t2 <= tol * tol * s2) || maxiter == 0;
   b->active[slot] = done ? 0 : 1;
   if (done) {
     b->finished[slot] = 1;
     b->conv[slot] = (t2 <= tol * tol * s2) ? 1 : 0;
   }
-  // the slot's inner CG scalars: c = |r_h|^2, active flag
-  double cs[2] = {done ? 0.0 : c, done ? 0.0 : 1.0};
+  // the slot's inner CG scalars (c = s^2 |r_h|^2, active flag) and its frame (1 / s for the fold)
+  double cs[3] = {done ? 0.0 : c * sn * sn, done ? 0.0 : 1.0, 1.0 / sn};
   WC_HIP(hipMemcpyAsync(b->st + 0 * N + slot, &cs[0], sizeof(double), hipMemcpyHostToDevice, s));
   WC_HIP(hipMemcpyAsync(b->st + 5 * N + slot, &cs[1], sizeof(double), hipMemcpyHostToDevice, s));
+  WC_HIP(hipMemcpyAsync(b->st + 13 * N + slot, &cs[2], sizeof(double), hipMemcpyHostToDevice, s));
   WC_HIP(hipStreamSynchronize(s));
   return WC_OK;
 }
@@ -326,9 +375,10 @@ int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished) {
     return k;
   };
   for (int it = 0; it < iters && count_active() > 0; ++it) {
-    rc = lo->prec == WC_SINGLE
-             ? inner_iteration<float>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s, b->st, b->partials)
-             : inner_iteration<double>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s, b->st, b->partials);
+    rc = lo->prec == WC_SINGLE ? inner_iteration<float>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s, b->st,
+                                                        b->partials, b->h16)
+                               : inner_iteration<double>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s,
+                                                         b->st, b->partials);
     if (rc) return rc;
     WC_HIP(hipMemcpyAsync(b->host + 4 * N, b->st + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     WC_HIP(hipStreamSynchronize(s));
@@ -340,6 +390,8 @@ int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished) {
       if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch: residual of slot %d became %g", i, cp);
       if (cp > b->maxr[i]) b->maxr[i] = cp;
       if (cp < b->delta * b->delta * b->maxr[i] || b->iters[i] >= b->maxiter[i]) trigger = true;
+      // predicted to meet its tolerance: check it now
+      if (b->ratio[i] * cp / (b->scale[i] * b->scale[i]) <= b->tol[i] * b->tol[i] * b->src2[i]) trigger = true;
     }
     if (trigger && (rc = batch_reliable(b))) return rc;
   }

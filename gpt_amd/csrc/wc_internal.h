@@ -155,20 +155,33 @@ inline bool shape_valid(const wc_operator *op, const wck::HopShape &sh) {
          sh.occ <= wck::kMaxOcc && sh.seg >= 1 && (sh.seg == 1 || (op->bps > 0 && sh.seg <= op->bps));
 }
 
-// process-wide cache of tuned shapes: (dims, precision, link format, nrhs)
+// process-wide cache of tuned shapes: (dims, precision, link format, class of
+// nrhs: 1, 2-3, 4-7, 8-15, 16-31, 32-64)
 typedef std::tuple<int, int, int, int, int, int, int> ShapeKey;
+inline int nrhs_class(int nrhs) {
+  int c = 1;
+  while (2 * c <= nrhs && c < 32) c *= 2;
+  return c;
+}
 extern std::mutex g_shape_mu;
 extern std::map<ShapeKey, wck::HopShape> g_shape_cache;
 inline ShapeKey shape_key(const wc_operator *op, int nrhs) {
-  return ShapeKey(op->g.L[0], op->g.L[1], op->g.L[2], op->g.L[3], op->prec, op->nl, nrhs);
+  return ShapeKey(op->g.L[0], op->g.L[1], op->g.L[2], op->g.L[3], op->prec, op->nl, nrhs_class(nrhs));
 }
 
+// the shape set for this operator, else the cached shape of the same
+// lattice, precision, link format and right-hand-side class (rb kept within
+// what nrhs allows), else the default
 inline wck::HopShape shape_for(const wc_operator *op, int nrhs) {
   if (nrhs >= 1 && nrhs <= 64 && op->shape_set[nrhs]) return op->shape[nrhs];
   {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     auto it = g_shape_cache.find(shape_key(op, nrhs));
-    if (it != g_shape_cache.end()) return it->second;
+    if (it != g_shape_cache.end()) {
+      wck::HopShape sh = it->second;
+      if (sh.rb > nrhs) sh.rb = 1;
+      return sh;
+    }
   }
   return default_shape(op, nrhs);
 }

@@ -7,8 +7,9 @@ right-hand sides, a tolerance) arrive at any time and are served by
 continuous batching:
 
 * a *group* per (resident field, mass) owns an FP64 / FP32 operator pair and
-  a ``batch`` of ``nslots`` solve slots (``wc_batch_*``: the mixed-precision
-  eo2_ne CG advancing all occupied slots with the multi-RHS kernels);
+  a ``batch`` of solve slots (``wc_batch_*``: the mixed-precision eo2_ne CG
+  advancing all occupied slots with the multi-RHS kernels), sized to the
+  right-hand sides waiting when it starts (at most ``nslots``);
 * right-hand sides enter free slots of their group whenever slots free up,
   while the group keeps iterating, and leave when their FP64 true residual
   |src - M dst| / |src| meets the request's tolerance;
@@ -37,13 +38,18 @@ from .core import lattice
 
 class batch:
     """One wc_batch: nslots slots of the mixed-precision eo2_ne solve on the
-    operator pair (op FP64, inner in the inner precision)."""
+    operator pair (op FP64, inner in the inner precision).  inner_storage
+    "half" (the default for an FP32 inner operator) keeps the inner search
+    direction and Schur intermediates in 16-bit storage."""
 
-    def __init__(self, op, inner, nslots, parity=core.odd, delta=0.1):
+    def __init__(self, op, inner, nslots, parity=core.odd, delta=0.01, inner_storage="auto"):
         self.op, self.inner = op, inner
         self.nslots = int(nslots)
+        if inner_storage == "auto":
+            inner_storage = "half" if inner.precision is core.single else None
+        code = _lib.WC_STORAGE_HALF if inner_storage == "half" else _lib.WC_STORAGE_NATIVE
         h = ctypes.c_void_p()
-        _lib.check(_lib.load().wc_batch_create(op.handle, inner.handle, parity.tag, self.nslots, float(delta),
+        _lib.check(_lib.load().wc_batch_create(op.handle, inner.handle, parity.tag, self.nslots, float(delta), code,
                                                core.stream_ptr(), ctypes.byref(h)))
         self.handle = h.value
         self.free = list(range(self.nslots))
@@ -118,19 +124,20 @@ class request:
 
 
 def group_bytes(fdims, nslots):
-    """HBM held by one group: both operators (links, clover and inverse,
-    the FP64 gauge copy each keeps, solver scratch) and the batch state."""
+    """HBM held by one group: both operators (links -- 12-real in the inner
+    one --, clover and inverse, the FP64 gauge copy each keeps, solver
+    scratch) and the batch state."""
     V = int(np.prod(fdims))
     Vh = V // 2
     h64, h32 = Vh * 12 * 16, Vh * 12 * 8
     op64 = 2 * Vh * 8 * 9 * 16 + 4 * Vh * 36 * 16 + V * 4 * 9 * 16 + h64 * (3 * nslots + 9)
-    op32 = 2 * Vh * 8 * 9 * 8 + 4 * Vh * 36 * 8 + V * 4 * 9 * 16 + h32 * (3 * nslots + 5)
+    op32 = 2 * Vh * 8 * 6 * 8 + 4 * Vh * 36 * 8 + V * 4 * 9 * 16 + h32 * (3 * nslots + 5)
     state = 5 * h64 * nslots + 5 * h32 * nslots + 2 * h64 * nslots
     return op64 + op32 + state
 
 
 class _group:
-    def __init__(self, server, key, U, mass, stream):
+    def __init__(self, server, key, U, mass, stream, nslots):
         self.server = server
         self.key = key
         self.U = U
@@ -140,7 +147,8 @@ class _group:
         self.in_slot = {}  # slot -> (request, rhs index)
         self.batch = None
         self.op = self.inner = None
-        self.bytes = group_bytes(U[0].grid.fdimensions, server.nslots)
+        self.nslots = nslots
+        self.bytes = group_bytes(U[0].grid.fdimensions, nslots)
 
     def busy(self):
         return bool(self.queue) or bool(self.in_slot)
@@ -155,8 +163,12 @@ class _group:
             if self.batch is None:
                 p = dict(self.server.params, mass=self.mass)
                 self.op = fermion.wilson_clover(self.U, p)
-                self.inner = self.op.converted(self.server.inner_precision)
-                self.batch = batch(self.op, self.inner, self.server.nslots, delta=self.server.delta)
+                # the inner operator only drives the recursion between FP64
+                # reliable updates: 12-real links change the speed, not the result
+                self.inner = fermion.wilson_clover(self.U, dict(p, compress_links=True),
+                                                   precision=self.server.inner_precision)
+                self.batch = batch(self.op, self.inner, self.nslots, delta=self.server.delta,
+                                   inner_storage=self.server.inner_storage)
             while self.queue and self.batch.free:
                 req, j = self.queue.popleft()
                 slot = self.batch.admit(req.sources[j], req.tol, req.maxiter)
@@ -188,12 +200,13 @@ class server:
     of the device memory free at start)."""
 
     def __init__(self, params, nslots=12, streams=4, iters_per_step=25, hbm_budget=None,
-                 inner_precision=core.single, delta=0.1):
+                 inner_precision=core.single, delta=0.01, inner_storage="auto"):
         self.params = {k: v for k, v in dict(params).items() if k not in ("mass", "kappa")}
         self.nslots = int(nslots)
         self.iters_per_step = int(iters_per_step)
         self.inner_precision = inner_precision
         self.delta = float(delta)
+        self.inner_storage = inner_storage
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * free)
@@ -218,15 +231,20 @@ class server:
             U = gauge.from_numpy(core.grid(fd, core.double), U)
         self.fields[key] = U
 
+    # a group's batch has as many slots as the request that opens it has
+    # right-hand sides (at most nslots): a slot costs the same whether or not
+    # it is occupied; later requests of the same (field, mass) take slots as
+    # they free up
     def _route(self, req):
         key = (req.field, req.mass)
         g = self.groups.get(key)
         if g is None:
             U = self.fields[req.field]
-            need = group_bytes(U[0].grid.fdimensions, self.nslots)
+            nslots = max(1, min(self.nslots, len(req.sources)))
+            need = group_bytes(U[0].grid.fdimensions, nslots)
             if self.hbm_used + need > self.hbm_budget:
                 return False
-            g = _group(self, key, U, req.mass, self.streams[self._next_stream % len(self.streams)])
+            g = _group(self, key, U, req.mass, self.streams[self._next_stream % len(self.streams)], nslots)
             self._next_stream += 1
             self.groups[key] = g
             self.hbm_used += g.bytes

@@ -198,9 +198,11 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
  * wc_tune_hop times the hop of nrhs right-hand sides (reps calls per shape)
  * for every valid shape and keeps the fastest for this operator and, in a
  * process-wide cache, for every operator of the same dimensions, precision
- * and link format.  shape (4 ints) and ms_per_call may be NULL.  The CG and
- * solve entry points tune on first use when Vh * nrhs >= 2^17 (environment
- * WC_AUTOTUNE=0 turns that off).  wc_set_hop_shape fixes a shape (NULL:
+ * and link format, for right-hand-side counts of the same class (1, 2-3,
+ * 4-7, 8-15, 16-31, 32-64).  shape (4 ints) and ms_per_call may be NULL.
+ * The CG, solve and batch entry points tune on first use when
+ * Vh * nrhs >= 2^17, once per class (environment WC_AUTOTUNE=0 turns tuning
+ * off).  The tuner leaves seg at 1.  wc_set_hop_shape fixes a shape (NULL:
  * back to automatic); wc_get_hop_shape reports the one in use. */
 int wc_tune_hop(wc_operator *op, int nrhs, int reps, int *shape, float *ms_per_call, void *stream);
 int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape);
@@ -210,8 +212,8 @@ int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape);
  * sides one after another, matrix_operator.py:288-295).  A wc_batch holds
  * nslots slots of the mixed-precision eo2_ne solve on one operator pair
  * (op FP64, inner a separate operator object in the inner precision, as for
- * wc_solve_eo2_ne_mixed) and advances all occupied slots together with the
- * multi-RHS kernels.  Requests enter free slots at any time and leave when
+ * wc_solve_eo2_ne_mixed, including inner_storage) and advances all
+ * occupied slots together with the multi-RHS kernels.  Requests enter free slots at any time and leave when
  * done, so the batch keeps running while its membership changes.
  *
  *   wc_batch_admit   puts one right-hand side (src: full FP64 field; guess:
@@ -228,8 +230,8 @@ int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape);
  * Calls on one batch, and on its operators, must not overlap; batches on
  * different operator pairs and streams run concurrently. */
 typedef struct wc_batch wc_batch;
-int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots, double delta, void *stream,
-                    wc_batch **out);
+int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots, double delta, int inner_storage,
+                    void *stream, wc_batch **out);
 int wc_batch_destroy(wc_batch *b);
 int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter);
 int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);
