@@ -58,8 +58,11 @@ def parse():
     ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
     ap.add_argument("--nrhs", type=int, default=12)
     ap.add_argument("--serve", type=int, default=12, help="serving stream: number of requests (0: skip)")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 8^4 point-source propagator")
+    ap.add_argument("--c4", default="16x16x16x32:12,16x16x16x32:48,32x32x32x64:12,32x32x32x64:48",
+                    help="multi-RHS cases LXxLYxLZxLT:NRHS, comma-separated ('' to skip)")
     ap.add_argument("--shape", default=None,
-                    help="hop launch shape rb,tpb,occ,seg for the Dhop variants instead of tuning")
+                    help="hop launch shape rb,tpb,occ,seg,ls for the Dhop variants instead of tuning")
     return ap.parse_args()
 
 
@@ -101,6 +104,27 @@ def smooth_links_device(torch, fd, eps, seed, dev):
             H = torch.einsum("na,aij->nij", h.to(torch.complex128), T)
             out[s:s + n] = torch.linalg.matrix_exp(1j * eps * H)
         U.append(g.qcd.gauge.link_field(g.grid(fd, g.double), out))
+    return U
+
+
+def haar_links_device(torch, fd, seed, dev):
+    """Haar-random SU(3) links made on the GPU: QR of a complex Gaussian
+    matrix with the phases of R's diagonal moved into Q, divided by a cube
+    root of its determinant."""
+    import gpt_amd as g
+
+    V = int(np.prod(fd))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    U = []
+    for mu in range(4):
+        z = torch.complex(torch.randn((V, 3, 3), generator=gen, dtype=torch.float64, device=dev),
+                          torch.randn((V, 3, 3), generator=gen, dtype=torch.float64, device=dev))
+        q, r = torch.linalg.qr(z)
+        d = torch.diagonal(r, dim1=-2, dim2=-1)
+        q = q * (d / d.abs()).unsqueeze(-2)
+        q = q / torch.pow(torch.linalg.det(q), 1.0 / 3.0)[:, None, None]
+        U.append(g.qcd.gauge.link_field(g.grid(fd, g.double), q.contiguous()))
     return U
 
 
@@ -231,7 +255,7 @@ def main():
                      "GBps": round(bytes_site * Vh / t_ev / 1e9, 2), "nrhs": nrhs,
                      "GFLOPps": round(1320.0 * nrhs * Vh / t_ev / 1e9, 2), "bytes_per_site": bytes_site,
                      "shape": {"rhs_per_thread": shape[0], "threads_per_block": 64 * shape[1],
-                               "max_blocks_per_cu": shape[2], "t_runs": shape[3]},
+                               "max_blocks_per_cu": shape[2], "t_runs": shape[3], "lds_staging": shape[4]},
                      "reference_effective_GBps": round(2880.0 / 8 * real_b * nrhs * Vh / t_ev / 1e9, 2)}
         log(f"bench: {name}: {res[name]}")
         del w, pairs
@@ -243,6 +267,16 @@ def main():
     cg_res = None
     if not args.no_cg:
         cg_res = cg_measure(torch, g, dev, fd=[int(x) for x in args.cg_lattice.split(",")], nrhs=args.nrhs)
+
+    c1_res = None
+    if not args.no_c1:
+        c1_res = c1_measure(torch, g, dev)
+
+    c4_res = []
+    for case in [c for c in args.c4.split(",") if c]:
+        dims, nr = case.split(":")
+        c4_res.append(c4_measure(torch, g, dev, [int(x) for x in dims.split("x")], int(nr)))
+        torch.cuda.empty_cache()
 
     serve_res = None
     if args.serve > 0:
@@ -259,7 +293,7 @@ def main():
     if "fp64_18" in res and [int(x) for x in fd] == [32, 32, 32, 64]:
         sh = head["shape"]
         traffic = profiled_traffic((sh["rhs_per_thread"], sh["threads_per_block"] // 64, sh["max_blocks_per_cu"],
-                                    sh["t_runs"]))
+                                    sh["t_runs"], sh["lds_staging"]))
     out = {
         "metric": "EO Dhop GB/s (% HBM peak); mixed-prec CG solves/s + p99 latency at 1e-10",
         "value": achieved,
@@ -282,6 +316,8 @@ def main():
         "variants": res,
         "copy_kernel_GBps": round(copy_gbs, 1),
         "cg": cg_res,
+        "c1_point_source_8x8x8x8": c1_res,
+        "multi_rhs": c4_res,
         "serving": serve_res,
     }
     print(json.dumps(out), flush=True)
@@ -395,6 +431,132 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
     out["speedup_mixed_multi_vs_fp64"] = out["mixed_multi"]["solves_per_s"] / out["fp64_cg"]["solves_per_s"]
     out["speedup_mixed_multi_half_vs_fp64"] = (out["mixed_multi_half"]["solves_per_s"]
                                                / out["fp64_cg"]["solves_per_s"])
+    return out
+
+
+def _timed(torch, fn):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def c1_measure(torch, g, dev, reps=3):
+    """8^4 point-source propagator: Haar-random links (seed 1001), mass 0.1,
+    csw 1, boundary phases [1,1,1,-1]; the 12 spin-colour right-hand sides
+    solved with eo2_ne + CG to 1e-8, (a) one after another as gpt does and
+    (b) as one 12-RHS solve; each with the iterations run one at a time
+    (host test after each) and in captured graphs."""
+    from gpt_amd.qcd.fermion.operator import pack
+
+    fd = [8, 8, 8, 8]
+    U = haar_links_device(torch, fd, 1001, dev)
+    prm = dict(mass=0.1, csw_r=1.0, csw_t=1.0, boundary_phases=[1, 1, 1, -1])
+    inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
+    grid = g.grid(fd, g.double)
+    srcs = []
+    for s in range(4):
+        for c in range(3):
+            a = np.zeros(tuple(reversed(fd)) + (4, 3), complex)
+            a[0, 0, 0, 0, s, c] = 1.0
+            x = g.lattice(grid)
+            x.from_numpy(a)
+            srcs.append(x)
+    out = {"lattice": fd, "mass": 0.1, "eps": 1e-8}
+    for mode, chunk in [("one_at_a_time", 1), ("graphs", 0)]:
+        w = g.qcd.fermion.wilson_clover(U, prm)
+        w.set_iteration_chunk(chunk)
+        cg = inv.cg(eps=1e-8, maxiter=5000)
+        slv = w.propagator(inv.preconditioned(pc.eo2_ne(), cg))
+        its = []
+
+        def seq():
+            its.clear()
+            for x in srcs:
+                g(slv * x)
+                its.append(len(cg.history))
+
+        seq()
+        lat = [_timed(torch, seq)[1] for _ in range(reps)]
+        w12 = g.qcd.fermion.wilson_clover(U, prm, n_rhs=12)
+        w12.set_iteration_chunk(chunk)
+        cg12 = inv.cg(eps=1e-8, maxiter=5000)
+        slv12 = w12.propagator(inv.preconditioned(pc.eo2_ne(), cg12))
+        S = pack(srcs)
+        g(slv12 * S)
+        lat12 = [_timed(torch, lambda: g(slv12 * S))[1] for _ in range(reps)]
+        out[mode] = {"sequential_ms_per_propagator": round(1e3 * float(np.median(lat)), 3),
+                     "sequential_iterations": its[:],
+                     "rhs12_ms_per_propagator": round(1e3 * float(np.median(lat12)), 3),
+                     "rhs12_iterations": list(cg12.iterations),
+                     "graph_runs": w12.iteration_runs()[0]}
+        log(f"bench: c1 {mode} {out[mode]}")
+        w.release()
+        w12.release()
+    return out
+
+
+def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
+    """Multi-RHS: smoothed links (seed 4001), mass 0.08, csw 1; nrhs
+    complex-Gaussian sources (seeds 4100 + j) to a true relative residual
+    tol.  FP64 eo2_ne CG one RHS at a time against the mixed-precision
+    multi-RHS solve with 16-bit inner storage, in batches of at most 12 RHS
+    whose half fields stay within the 4 GB offset window of the kernels."""
+    from gpt_amd.qcd.fermion.operator import pack
+
+    U = smooth_links_device(torch, fd, 0.3, seed, dev)
+    prm = dict(mass=0.08, csw_r=1.0, csw_t=1.0, boundary_phases=[1, 1, 1, -1])
+    inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
+    grid = g.grid(fd, g.double)
+    Vh = int(np.prod(fd)) // 2
+    per_rhs = Vh * 12 * 16  # FP64 half field of one RHS
+    nb = min(nrhs, 12 if per_rhs * nrhs >= (1 << 32) else nrhs)
+    srcs = []
+    for j in range(nrhs):
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(4100 + j)
+        x = g.lattice(grid)
+        x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
+        srcs.append(x)
+    out = {"lattice": fd, "nrhs": nrhs, "mass": 0.08, "tol_true_residual": tol, "batch": nb}
+    w = g.qcd.fermion.wilson_clover(U, prm)
+    eps = tol
+    for _ in range(6):
+        cg = inv.cg(eps=eps, maxiter=20000)
+        slv = w.propagator(inv.preconditioned(pc.eo2_ne(), cg))
+        x0 = g(slv * srcs[0])
+        if true_residual(g, w, x0, srcs[0])[0] <= tol:
+            break
+        eps /= 2.0
+    lat, its = [], []
+    for i in range(2):
+        x, t = _timed(torch, lambda: g(slv * srcs[i]))
+        lat.append(t)
+        its.append(len(cg.history))
+    t64 = float(np.median(lat))
+    out["fp64_sequential"] = {"eps_normal_equation": eps, "iterations": its, "s_per_solve": t64,
+                              "solves_per_s": 1.0 / t64}
+    wN = g.qcd.fermion.wilson_clover(U, prm, n_rhs=nb)
+    mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half")
+    slvh = wN.propagator(inv.preconditioned(pc.eo2_ne(), mh))
+    batches = [pack(srcs[i:i + nb]) for i in range(0, nrhs, nb)]
+    g(slvh * batches[0])  # warm-up (tuning, graphs)
+
+    def run():
+        res, itr = [], []
+        for S in batches:
+            X = g(slvh * S)
+            res += true_residual(g, wN, X, S)
+            itr += list(mh.iterations)
+        return res, itr
+
+    (res, itr), t = _timed(torch, run)
+    out["mixed_multi_half"] = {"iterations_max": max(itr), "reliable_updates": mh.reliable_updates,
+                               "max_true_residual": max(res), "s_total": t, "solves_per_s": nrhs / t}
+    out["speedup_vs_fp64_sequential"] = out["mixed_multi_half"]["solves_per_s"] * t64
+    log(f"bench: multi-rhs {fd} N={nrhs}: {out}")
+    del w, wN, batches, srcs, U
     return out
 
 

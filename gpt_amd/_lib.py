@@ -70,6 +70,8 @@ SIGNATURES = {
     "wc_tune_hop": (_I, [_P, _I, _I, _P, _P, _P]),
     "wc_set_hop_shape": (_I, [_P, _I, _P]),
     "wc_get_hop_shape": (_I, [_P, _I, _P]),
+    "wc_set_iteration_chunk": (_I, [_P, _I]),
+    "wc_iteration_runs": (_I, [_P, _P, _P]),
     "wc_batch_create": (_I, [_P, _P, _I, _I, _D, _I, _P, ctypes.POINTER(_P)]),
     "wc_batch_destroy": (_I, [_P]),
     "wc_batch_admit": (_I, [_P, _I, _P, _P, _D, _L]),

@@ -268,9 +268,7 @@ static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, do
   if ((rc = ensure_red(op, nrhs))) return rc;
   const int N = nrhs;
   void *r = op->tmp[3], *p = op->tmp[4], *mmp = op->tmp[5], *aDp = op->tmp[9];
-  const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
   const long n = op->g.Vh * 12 * N;
-  const int hop_blocks = wck::hop_blocks(op->g);
   double *st = op->st, *hs = op->host;
   // mmp = A x ; r = b - mmp ; p = r ; c = |p|^2 ; ssq = |b|^2
   if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, x, nullptr, s))) return rc;
@@ -280,11 +278,12 @@ static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, do
   if ((rc = norm2_rhs<R>(op, N, b, st + 6 * N, s))) return rc;  // |b|^2
   WC_HIP(hipMemcpyAsync(hs, st, sizeof(double) * 7 * N, hipMemcpyDeviceToHost, s));
   WC_HIP(hipStreamSynchronize(s));
-  std::vector<double> rsq(N), active(N, 1.0);
+  std::vector<double> rsq(N), active(N, 1.0), c0(N);
   int nactive = 0;
   for (int i = 0; i < N; ++i) {
     const double ssq = hs[6 * N + i];
     rsq[i] = eps * eps * ssq;
+    c0[i] = hs[i];
     if (iters) iters[i] = 0;
     if (conv) conv[i] = 0;
     if (r2) r2[i] = hs[i];
@@ -305,40 +304,61 @@ static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, do
   WC_HIP(hipMemcpyAsync(st + 5 * N, hs, sizeof(double) * N, hipMemcpyHostToDevice, s));
   // A_D^-1 p, kept up to date by the fused x/p update
   site_op<R>(op, N, pd, 1, p, aDp, nullptr, s);
-  long k = 0;
-  for (; k < maxiter && nactive > 0; ++k) {
-    // mmp = A p with per-RHS partials of Re<p, mmp>
-    if ((rc = mpc_impl<R>(op, N, pd, 2, mmp, p, p, s, aDp))) return rc;
-    wck::cg_alpha(op->partials, hop_blocks, N, st, s);
-    wck::cg_update_r<R>(n, N, st, r, mmp, op->partials, s);
-    wck::cg_beta(op->partials, wck::kRedBlocks, N, st, s);
-    // x += alpha p ; p = beta p + r ; A_D^-1 p for the next iteration
-    wck::cg_update_xp<R>(op->g, N, pd, st, r, p, x, clovD, 1.0 / op->diag, aDp, op->open_bc ? 1 : 0, s);
+  // Iterations run in chunks (wck::iter_check): the device applies the
+  // stopping rule after every iteration and the host replays the residual
+  // history of each run, so the result and history do not depend on the
+  // chunk lengths.
+  IterRun &ir = op->run;
+  if ((rc = ensure_run(ir))) return rc;
+  for (int i = 0; i < N; ++i) ir.hdat[i] = rsq[i];
+  WC_HIP(hipMemcpyAsync(ir.dat, ir.hdat, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  std::vector<uintptr_t> key = {0, (uintptr_t)N, (uintptr_t)pd, (uintptr_t)x, (uintptr_t)r, (uintptr_t)p,
+                                (uintptr_t)mmp, (uintptr_t)aDp, (uintptr_t)st};
+  op_key(op, N, key);
+  auto one = [&](hipStream_t q) -> int {
+    op->halt_cur = ir.ctl + wck::kCtlHalt;
+    const int rc2 = inner_iteration<R>(op, N, pd, r, p, mmp, x, aDp, q);
+    op->halt_cur = nullptr;
+    if (rc2) return rc2;
+    wck::iter_check(N, 0, 0.0, st, ir.dat, ir.ctl, ir.hist, q);
     WC_LAUNCHED();
-    WC_HIP(hipMemcpyAsync(hs + 4 * N, st + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-    WC_HIP(hipStreamSynchronize(s));
-    bool changed = false;
+    return WC_OK;
+  };
+  const int cap = chunk_cap(op, N, op->chunk);
+  std::vector<double> clast(c0);
+  long k = 0;
+  while (k < maxiter && nactive > 0) {
+    // the run ends when the last right-hand side converges
+    double need = 0.0;
     for (int i = 0; i < N; ++i) {
       if (active[i] == 0.0) continue;
-      const double cp = hs[4 * N + i];
-      if (history) history[k * N + i] = cp;
-      if (r2) r2[i] = cp;
-      if (!std::isfinite(cp)) {
-        if (iters) iters[i] = k + 1;
-        WC_FAIL(WC_ERR_NUMERIC, "CG: residual of right-hand side %d became %g at iteration %ld", i, cp, k + 1);
-      }
-      if (std::fabs(cp) <= rsq[i]) {
-        active[i] = 0.0;
-        --nactive;
-        changed = true;
-        if (iters) iters[i] = k + 1;
-        if (conv) conv[i] = 1;
+      const double it = iters_to(clast[i], rsq[i], decay_rate(c0[i], clast[i], k, 0.0));
+      need = std::max(need, it < 0.0 ? (double)kFirstRun : it);
+    }
+    const int K = pow2_floor(std::min(need, (double)(maxiter - k)), cap);
+    if ((rc = run_iterations(ir, key, K, N, s, one))) return rc;
+    const int done = ir.hctl[wck::kCtlIter];
+    for (int j = 0; j < done; ++j) {
+      for (int i = 0; i < N; ++i) {
+        if (active[i] == 0.0) continue;
+        const double cp = ir.hhist[(size_t)j * N + i];
+        if (history) history[(k + j) * N + i] = cp;
+        if (r2) r2[i] = cp;
+        clast[i] = cp;
+        if (!std::isfinite(cp)) {
+          if (iters) iters[i] = k + j + 1;
+          WC_FAIL(WC_ERR_NUMERIC, "CG: residual of right-hand side %d became %g at iteration %ld", i, cp,
+                  k + j + 1);
+        }
+        if (std::fabs(cp) <= rsq[i]) {
+          active[i] = 0.0;
+          --nactive;
+          if (iters) iters[i] = k + j + 1;
+          if (conv) conv[i] = 1;
+        }
       }
     }
-    if (changed && nactive > 0) {
-      for (int i = 0; i < N; ++i) hs[5 * N + i] = active[i];
-      WC_HIP(hipMemcpyAsync(st + 5 * N, hs + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
-    }
+    k += done;
   }
   for (int i = 0; i < N; ++i)
     if (active[i] != 0.0 && iters) iters[i] = k;
@@ -442,6 +462,26 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   std::vector<double> src2(N), maxr(N), active(N, 1.0), scale(N, 1.0), ratio(N, 0.0);
   int nactive = N;
   long updates = 0;
+  // chunked inner iterations (wck::iter_check mode 1 decides on the device
+  // when a reliable update is due); cref / clast: inner |r|^2 at the last
+  // update and now, rate: its decay per iteration in the last cycle
+  IterRun &ir = lo->run;
+  if ((rc = ensure_run(ir))) return rc;
+  std::vector<double> cref(N, 0.0), clast(N, 0.0), rate(N, 0.0);
+  long since = 0;
+  auto upload = [&]() -> int {
+    double *d = ir.hdat;
+    for (int i = 0; i < N; ++i) {
+      d[0 * wck::kMaxRhs + i] = tol * tol * src2[i];
+      d[1 * wck::kMaxRhs + i] = maxr[i];
+      d[2 * wck::kMaxRhs + i] = ratio[i];
+      d[3 * wck::kMaxRhs + i] = scale[i];
+      d[4 * wck::kMaxRhs + i] = 0.0;
+      d[5 * wck::kMaxRhs + i] = HUGE_VAL;  // no per-RHS iteration limit
+    }
+    WC_HIP(hipMemcpyAsync(ir.dat, ir.hdat, IterRun::kDatBytes, hipMemcpyHostToDevice, s));
+    return WC_OK;
+  };
   // scale rows in hi->st: 11 s_new, 12 s_new / s_old, 13 1 / s (for the next fold)
   for (int i = 0; i < N; ++i) hsh[13 * N + i] = 1.0;
   WC_HIP(hipMemcpyAsync(sth + 13 * N, hsh + 13 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
@@ -504,7 +544,12 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     WC_HIP(hipMemcpyAsync(stl + 0 * N, hsl, sizeof(double) * N, hipMemcpyHostToDevice, s));
     for (int i = 0; i < N; ++i) hsl[5 * N + i] = active[i];
     WC_HIP(hipMemcpyAsync(stl + 5 * N, hsl + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
-    return WC_OK;
+    for (int i = 0; i < N; ++i) {
+      if (since >= 2) rate[i] = decay_rate(cref[i], clast[i], since, rate[i]);
+      cref[i] = clast[i] = maxr[i];
+    }
+    since = 0;
+    return upload();
   };
   for (int i = 0; i < N; ++i) {
     if (iters) iters[i] = 0;
@@ -521,25 +566,55 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     if (lo_single) site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s);
     else site_op<double>(lo, N, pd, 1, pl, aDl, nullptr, s);
   }
+  // A reliable update is due when a right-hand side's iterated |r|^2 has
+  // fallen below delta^2 times its maximum since the last update, or when
+  // it predicts convergence (ratio |r|^2 / s^2 <= tol^2 |src|^2): then the
+  // true residual is checked at once rather than at the next delta step.
+  const double d2 = delta * delta;
+  auto one = [&](hipStream_t q) -> int {
+    lo->halt_cur = ir.ctl + wck::kCtlHalt;
+    const int rc2 = lo_single ? inner_iteration<float>(lo, N, pd, rl, pl, ml, xl, aDl, q, nullptr, nullptr, h16)
+                              : inner_iteration<double>(lo, N, pd, rl, pl, ml, xl, aDl, q);
+    lo->halt_cur = nullptr;
+    if (rc2) return rc2;
+    wck::iter_check(N, 1, d2, stl, ir.dat, ir.ctl, ir.hist, q);
+    WC_LAUNCHED();
+    return WC_OK;
+  };
+  std::vector<uintptr_t> key = {1, (uintptr_t)N, (uintptr_t)pd, (uintptr_t)h16, (uintptr_t)rl, (uintptr_t)pl,
+                                (uintptr_t)ml, (uintptr_t)xl, (uintptr_t)aDl, (uintptr_t)stl};
+  op_key(lo, N, key);
+  const int cap = chunk_cap(lo, N, hi->chunk);
   long k = 0;
-  for (; k < maxiter && nactive > 0; ++k) {
-    rc = lo_single ? inner_iteration<float>(lo, N, pd, rl, pl, ml, xl, aDl, s, nullptr, nullptr, h16)
-                   : inner_iteration<double>(lo, N, pd, rl, pl, ml, xl, aDl, s);
-    if (rc) return rc;
-    WC_HIP(hipMemcpyAsync(hsl + 4 * N, stl + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-    WC_HIP(hipStreamSynchronize(s));
-    bool trigger = false;
+  while (k < maxiter && nactive > 0) {
+    // the run ends at the first right-hand side that triggers an update
+    double need = 1e30;
     for (int i = 0; i < N; ++i) {
       if (active[i] == 0.0) continue;
-      const double cp = hsl[4 * N + i];
-      if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "mixed CG: residual of right-hand side %d became %g", i, cp);
-      if (cp > maxr[i]) maxr[i] = cp;
-      if (cp < delta * delta * maxr[i]) trigger = true;
-      // predicted to meet tol: check it now rather than at the next delta step
-      if (ratio[i] * cp / (scale[i] * scale[i]) <= tol * tol * src2[i]) trigger = true;
+      const double r_ = decay_rate(cref[i], clast[i], since, rate[i]);
+      double it = iters_to(clast[i], d2 * maxr[i], r_);
+      if (ratio[i] > 0.0) {
+        const double it2 = iters_to(clast[i], tol * tol * src2[i] * scale[i] * scale[i] / ratio[i], r_);
+        if (it2 >= 0.0 && (it < 0.0 || it2 < it)) it = it2;
+      }
+      need = std::min(need, it < 0.0 ? (double)kFirstRun : it);
     }
+    const int K = pow2_floor(std::min(need, (double)(maxiter - k)), cap);
+    if ((rc = run_iterations(ir, key, K, N, s, one))) return rc;
+    const int done = ir.hctl[wck::kCtlIter];
+    for (int j = 0; j < done; ++j)
+      for (int i = 0; i < N; ++i) {
+        if (active[i] == 0.0) continue;
+        const double cp = ir.hhist[(size_t)j * N + i];
+        if (!std::isfinite(cp))
+          WC_FAIL(WC_ERR_NUMERIC, "mixed CG: residual of right-hand side %d became %g", i, cp);
+        if (cp > maxr[i]) maxr[i] = cp;
+        clast[i] = cp;
+      }
+    k += done;
+    since += done;
     // fold the inner solution into the FP64 one, restart its accumulation
-    if (trigger && (rc = reliable(k + 1, false))) return rc;
+    if (ir.hctl[wck::kCtlTrig] && (rc = reliable(k, false))) return rc;
   }
   if (nactive > 0) {
     // not converged within maxiter: fold what the inner solve has and report
@@ -574,26 +649,32 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
   wck::zero<R>(n, op->tmp[1], s);
   std::vector<wck::HopShape> cand;
   // t-marching order (seg > 1) has never won on MI355X and is left to
-  // wc_set_hop_shape
+  // wc_set_hop_shape; LDS staging (ls) only pays when a site has several
+  // items (more right-hand sides than rb)
   const int occs[] = {0, 2, 4, 8, 16};
   for (int rb = 1; rb <= max_rb_prec(op->prec) && rb <= N; ++rb)
-    for (int tpb = 1; tpb <= wck::kMaxTpb; tpb *= 2)
-      for (int oc : occs) {
-        wck::HopShape sh;
-        sh.rb = rb;
-        sh.tpb = tpb;
-        sh.occ = oc;
-        if (shape_valid(op, sh)) cand.push_back(sh);
-      }
+    for (int ls = 0; ls <= (N > rb ? 1 : 0); ++ls)
+      for (int tpb = 1; tpb <= wck::kMaxTpb; tpb *= 2)
+        for (int oc : occs) {
+          wck::HopShape sh;
+          sh.rb = rb;
+          sh.tpb = tpb;
+          sh.occ = oc;
+          sh.ls = ls;
+          if (shape_valid(op, sh)) cand.push_back(sh);
+        }
   hipEvent_t e0, e1;
   WC_HIP(hipEventCreate(&e0));
   WC_HIP(hipEventCreate(&e1));
   best_ms = -1.0f;
   for (const wck::HopShape &sh : cand) {
-    HopLaunch a = hop_args<R>(op, N, 0, 0, 0, op->tmp[1], op->tmp[0]);
+    // the hop of the normal operator's first step (clover-inverse epilogue),
+    // whose blocks the staging shapes also stage
+    HopLaunch a = hop_args<R>(op, N, 0, 0, op->has_clover ? wck::EPI_CLOV : 0, op->tmp[1], op->tmp[0]);
+    a.clov = op->has_clover ? op->Ai[0] : nullptr;
     a.shape = sh;
-    WC_HOP(R, a, s);  // warm-up
-    if (hipGetLastError() != hipSuccess) continue;  // a shape this device cannot launch
+    // shapes this device cannot launch, or whose staging area is too large
+    if (!wck::hop<R>(a, s) || hipGetLastError() != hipSuccess) continue;
     WC_HIP(hipEventRecord(e0, s));
     for (int k = 0; k < reps; ++k) WC_HOP(R, a, s);
     WC_HIP(hipEventRecord(e1, s));
@@ -655,6 +736,7 @@ int wc_tune_hop(wc_operator *op, int nrhs, int reps, int *shape, float *ms_per_c
     shape[1] = op->shape[nrhs].tpb;
     shape[2] = op->shape[nrhs].occ;
     shape[3] = op->shape[nrhs].seg;
+    shape[4] = op->shape[nrhs].ls;
   }
   return WC_OK;
 }
@@ -672,11 +754,13 @@ int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape) {
   sh.tpb = shape[1];
   sh.occ = shape[2];
   sh.seg = shape[3];
-  if (!shape_valid(op, sh))
+  sh.ls = shape[4];
+  if (!shape_valid(op, sh) || !stage_fits(op, nrhs, sh))
     WC_FAIL(WC_ERR_ARG,
-            "wc_set_hop_shape: rb=%d tpb=%d occ=%d seg=%d invalid (rb 1..%d, tpb 1..%d, occ 0..%d, seg 1 or "
-            "2..%d)",
-            sh.rb, sh.tpb, sh.occ, sh.seg, max_rb_prec(op->prec), wck::kMaxTpb, wck::kMaxOcc, op->bps);
+            "wc_set_hop_shape: rb=%d tpb=%d occ=%d seg=%d ls=%d invalid (rb 1..%d, tpb 1..%d, occ 0..%d, seg 1 or "
+            "2..%d, ls 0 or 1 with a staging area of at most %d bytes)",
+            sh.rb, sh.tpb, sh.occ, sh.seg, sh.ls, max_rb_prec(op->prec), wck::kMaxTpb, wck::kMaxOcc, op->bps,
+            wck::kMaxStageBytes);
   op->shape[nrhs] = sh;
   op->shape_set[nrhs] = true;
   return WC_OK;
@@ -690,6 +774,21 @@ int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape) {
   shape[1] = sh.tpb;
   shape[2] = sh.occ;
   shape[3] = sh.seg;
+  shape[4] = sh.ls;
+  return WC_OK;
+}
+
+int wc_set_iteration_chunk(wc_operator *op, int chunk) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_set_iteration_chunk: null operator");
+  if (chunk < 0 || chunk > 32) WC_FAIL(WC_ERR_ARG, "wc_set_iteration_chunk: chunk = %d must be in [0, 32]", chunk);
+  op->chunk = chunk;
+  return WC_OK;
+}
+
+int wc_iteration_runs(const wc_operator *op, long *graph_runs, long *direct_runs) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_iteration_runs: null operator");
+  if (graph_runs) *graph_runs = op->run.graph_runs;
+  if (direct_runs) *direct_runs = op->run.direct_runs;
   return WC_OK;
 }
 

@@ -79,6 +79,14 @@ struct wc_batch {
   // |r_s|^2 / |r_h|^2 of the last reliable update (convergence predictor)
   std::vector<double> scale, ratio;
   long updates = 0;
+  // chunked iterations (wck::iter_check mode 1): device trigger data to be
+  // re-sent after an admission or a reliable update; per slot the inner
+  // |r|^2 at its last restart and now, the decay rate per iteration of its
+  // last cycle, and the iterations since its restart
+  IterRun run;
+  bool dirty = true;
+  std::vector<double> cref, clast, rate;
+  std::vector<long> since;
 };
 
 static size_t hbytes(const wc_operator *op) { return half_bytes(op); }
@@ -90,6 +98,7 @@ static void free_batch(wc_batch *b) {
   (void)hipFree(b->st);
   (void)hipFree(b->partials);
   if (b->host) (void)hipHostFree(b->host);
+  free_run(b->run);
   delete b;
 }
 
@@ -155,6 +164,11 @@ static int batch_reliable(wc_batch *b) {
     b->host[13 * N + i] = 1.0 / sn;
     b->scale[i] = sn;
     b->maxr[i] = c * sn * sn;
+    if (b->active[i]) {
+      if (b->since[i] >= 2) b->rate[i] = decay_rate(b->cref[i], b->clast[i], b->since[i], b->rate[i]);
+      b->cref[i] = b->clast[i] = b->maxr[i];
+      b->since[i] = 0;
+    }
     // c = s^2 |r_h|^2 for the slots that go on, active flags (st rows 0 and 5)
     b->host[i] = b->active[i] ? b->maxr[i] : 0.0;
     b->host[5 * N + i] = b->active[i] ? 1.0 : 0.0;
@@ -171,6 +185,7 @@ static int batch_reliable(wc_batch *b) {
   }
   WC_HIP(hipMemcpyAsync(b->st, b->host, sizeof(double) * N, hipMemcpyHostToDevice, s));
   WC_HIP(hipMemcpyAsync(b->st + 5 * N, b->host + 5 * N, sizeof(double) * N, hipMemcpyHostToDevice, s));
+  b->dirty = true;
   return WC_OK;
 }
 
@@ -256,6 +271,10 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
   b->scale.assign(N, 1.0);
   b->ratio.assign(N, 0.0);
   b->tres.assign(N, 0.0);
+  b->cref.assign(N, 0.0);
+  b->clast.assign(N, 0.0);
+  b->rate.assign(N, 0.0);
+  b->since.assign(N, 0);
   *out = b;
   return WC_OK;
 }
@@ -347,6 +366,10 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
   b->scale[slot] = sn;
   b->ratio[slot] = c > 0.0 ? t2 / c : 0.0;
   b->maxr[slot] = c * sn * sn;
+  b->cref[slot] = b->clast[slot] = b->maxr[slot];
+  b->rate[slot] = 0.0;
+  b->since[slot] = 0;
+  b->dirty = true;
   b->tres[slot] = s2 > 0 ? std::sqrt(t2 / s2) : 0.0;
   b->active[slot] = done ? 0 : 1;
   if (done) {
@@ -374,26 +397,76 @@ int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished) {
     for (int i = 0; i < N; ++i) k += b->active[i];
     return k;
   };
-  for (int it = 0; it < iters && count_active() > 0; ++it) {
-    rc = lo->prec == WC_SINGLE ? inner_iteration<float>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s, b->st,
-                                                        b->partials, b->h16)
-                               : inner_iteration<double>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, s,
-                                                         b->st, b->partials);
-    if (rc) return rc;
-    WC_HIP(hipMemcpyAsync(b->host + 4 * N, b->st + 4 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-    WC_HIP(hipStreamSynchronize(s));
-    bool trigger = false;
+  // Iterations run in chunks: wck::iter_check (mode 1) decides on the
+  // device when a reliable update is due -- a slot's iterated |r|^2 below
+  // delta^2 times its maximum since its restart, its iteration limit, or a
+  // predicted convergence (ratio |r|^2 / s^2 <= tol^2 |src|^2) -- and the
+  // rest of the run then does nothing.
+  IterRun &ir = b->run;
+  if ((rc = ensure_run(ir))) return rc;
+  const double d2 = b->delta * b->delta;
+  auto one = [&](hipStream_t q) -> int {
+    lo->halt_cur = ir.ctl + wck::kCtlHalt;
+    const int rc2 = lo->prec == WC_SINGLE
+                        ? inner_iteration<float>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, q, b->st,
+                                                 b->partials, b->h16)
+                        : inner_iteration<double>(lo, N, b->pd, b->rl, b->pl, b->ml, b->xl, b->aD, q, b->st,
+                                                  b->partials);
+    lo->halt_cur = nullptr;
+    if (rc2) return rc2;
+    wck::iter_check(N, 1, d2, b->st, ir.dat, ir.ctl, ir.hist, q);
+    WC_LAUNCHED();
+    return WC_OK;
+  };
+  std::vector<uintptr_t> key = {2, (uintptr_t)N, (uintptr_t)b->pd, (uintptr_t)b->h16, (uintptr_t)b->rl,
+                                (uintptr_t)b->pl, (uintptr_t)b->ml, (uintptr_t)b->xl, (uintptr_t)b->aD,
+                                (uintptr_t)b->st, (uintptr_t)b->partials};
+  op_key(lo, N, key);
+  const int cap = chunk_cap(lo, N, b->hi->chunk);
+  int it = 0;
+  while (it < iters && count_active() > 0) {
+    if (b->dirty) {
+      double *d = ir.hdat;
+      for (int i = 0; i < N; ++i) {
+        d[0 * wck::kMaxRhs + i] = b->tol[i] * b->tol[i] * b->src2[i];
+        d[1 * wck::kMaxRhs + i] = b->maxr[i];
+        d[2 * wck::kMaxRhs + i] = b->ratio[i];
+        d[3 * wck::kMaxRhs + i] = b->scale[i];
+        d[4 * wck::kMaxRhs + i] = (double)b->iters[i];
+        d[5 * wck::kMaxRhs + i] = (double)b->maxiter[i];
+      }
+      WC_HIP(hipMemcpyAsync(ir.dat, ir.hdat, IterRun::kDatBytes, hipMemcpyHostToDevice, s));
+      b->dirty = false;
+    }
+    // the run ends at the first slot that triggers an update
+    double need = 1e30;
     for (int i = 0; i < N; ++i) {
       if (!b->active[i]) continue;
-      ++b->iters[i];
-      const double cp = b->host[4 * N + i];
-      if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch: residual of slot %d became %g", i, cp);
-      if (cp > b->maxr[i]) b->maxr[i] = cp;
-      if (cp < b->delta * b->delta * b->maxr[i] || b->iters[i] >= b->maxiter[i]) trigger = true;
-      // predicted to meet its tolerance: check it now
-      if (b->ratio[i] * cp / (b->scale[i] * b->scale[i]) <= b->tol[i] * b->tol[i] * b->src2[i]) trigger = true;
+      const double r_ = decay_rate(b->cref[i], b->clast[i], b->since[i], b->rate[i]);
+      double t = iters_to(b->clast[i], d2 * b->maxr[i], r_);
+      if (b->ratio[i] > 0.0) {
+        const double t2 = iters_to(b->clast[i], b->tol[i] * b->tol[i] * b->src2[i] * b->scale[i] * b->scale[i] /
+                                                    b->ratio[i], r_);
+        if (t2 >= 0.0 && (t < 0.0 || t2 < t)) t = t2;
+      }
+      if (t < 0.0) t = kFirstRun;
+      need = std::min({need, t, (double)(b->maxiter[i] - b->iters[i])});
     }
-    if (trigger && (rc = batch_reliable(b))) return rc;
+    const int K = pow2_floor(std::min(need, (double)(iters - it)), cap);
+    if ((rc = run_iterations(ir, key, K, N, s, one))) return rc;
+    const int done = ir.hctl[wck::kCtlIter];
+    for (int j = 0; j < done; ++j)
+      for (int i = 0; i < N; ++i) {
+        if (!b->active[i]) continue;
+        ++b->iters[i];
+        ++b->since[i];
+        const double cp = ir.hhist[(size_t)j * N + i];
+        if (!std::isfinite(cp)) WC_FAIL(WC_ERR_NUMERIC, "wc_batch: residual of slot %d became %g", i, cp);
+        if (cp > b->maxr[i]) b->maxr[i] = cp;
+        b->clast[i] = cp;
+      }
+    it += done;
+    if (ir.hctl[wck::kCtlTrig] && (rc = batch_reliable(b))) return rc;
   }
   int k = 0;
   for (int i = 0; i < N; ++i) {

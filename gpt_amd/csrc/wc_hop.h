@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "wc_kernels.h"
@@ -121,10 +122,13 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], 
 // wave-uniform switch.  Neighbour offsets come from uniform strides of the
 // half-site index h = ((t*Lz + z)*Ly + y)*Lxh + x/2.  rl[j] is the
 // right-hand side a spinor is read for (clamped into range for padded lanes).
-template <typename R, int NL, int DAG, int RB, typename SI>
+//
+// LS: the links are staged in LDS, entry e of the site at lw[e * ns].
+template <typename R, int NL, int DAG, int RB, typename SI, bool LS>
 __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int rl[RB], const int cx,
                                          const int cy, const int cz, const int ct,
-                                         typename c2<R>::t acc[RB][12]) {
+                                         typename c2<R>::t acc[RB][12], const typename c2<R>::t *lw,
+                                         int ns) {
   // coordinates as scalars: a select among array elements would become a
   // dynamically indexed (scratch) array
   const int c[4] = {cx, cy, cz, ct};
@@ -135,6 +139,7 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
   const wc_rsrc in = make_rsrc(a.in, 12u * (uint32_t)g.Vh * N * (uint32_t)sizeof(SI));
   // a wave's sites are consecutive and lie in at most two aligned 64-site
   // blocks: its links are addressed from the first active lane's block
+  // (unused, and removed by the compiler, when the links are staged)
   const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
   const wc_rsrc rw = make_rsrc((const C *)a.W + blk * (8 * NL * kBlk), 2u * 8u * NL * kBlk * (uint32_t)sizeof(C));
   const uint32_t vw = (uint32_t)(aidx_site(h, 8 * NL) - blk * (8 * NL * kBlk)) * (uint32_t)sizeof(C);
@@ -159,9 +164,15 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
         nb = (cm == 0) ? hh + wrap : hh - stride;
     }
     C u[9];
-    const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
+    if constexpr (LS) {
 #pragma unroll
-    for (int e = 0; e < NL; ++e) u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
+      for (int e = 0; e < NL; ++e) u[e] = lw[(dir * NL + e) * ns];
+    } else {
+      const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
+#pragma unroll
+      for (int e = 0; e < NL; ++e)
+        u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
+    }
     const uint32_t vs = (uint32_t)aidx_site(nb, 12) * N;
     C psi[RB][12];
 #pragma unroll
@@ -213,19 +224,46 @@ template <typename R, int RB> constexpr int hop_min_waves() { return (sizeof(R) 
 // writes each (site, right-hand side) product to dots[h * N + r]; dot_blocks
 // sums them per 64-site block in site order, so the sums do not depend on
 // the launch shape.
-template <typename R, int NL, int DAG, int EPI, int RB, typename SI, typename SO>
+template <typename R, int NL, int DAG, int EPI, int RB, typename SI, typename SO, bool LS>
 __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const HopLaunch a) {
   using C = typename c2<R>::t;
+  if (a.halt && *a.halt) return;
   const Geom &g = a.g;
   const int N = a.nrhs;
   const int Gt = (N + RB - 1) / RB;
-  const long q = xcd_tile(blockIdx.x, gridDim.x) * (long)blockDim.x + threadIdx.x;
+  const long q0 = xcd_tile(blockIdx.x, gridDim.x) * (long)blockDim.x;
+  const long q = q0 + threadIdx.x;
   const long v = q / Gt;  // site in visiting order
-  if (v >= g.Vh) return;
-  const int rb = (int)(q - v * Gt) * RB;
   // a 64-site block has 64 * Gt items, a whole number of waves: a wave's
   // sites lie in one block, and blocks can be visited in any order
-  const long h = a.shape.seg > 1 ? block_order(v >> 6, a.shape.seg, a.bps, g.L[3]) * kBlk + (v & 63) : v;
+  auto site_of = [&](long vv) {
+    return a.shape.seg > 1 ? block_order(vv >> 6, a.shape.seg, a.bps, g.L[3]) * kBlk + (vv & 63) : vv;
+  };
+  // ls: the workgroup's sites v0 .. v0 + ns - 1 get their links (and the
+  // EPI_CLOV blocks) copied to LDS once, entry e of site v0 + s at
+  // stage[e * ns + s]; the Gt items of a site then read them from there
+  // instead of each loading them through the texture path
+  extern __shared__ __align__(16) char wc_dyn_lds[];
+  C *stage = (C *)wc_dyn_lds;
+  const long v0 = q0 / Gt;
+  int ns = 0;
+  const bool clov_staged = LS && (EPI & EPI_CLOV) && a.clov;
+  if constexpr (LS) {
+    long v1 = (q0 + (long)blockDim.x - 1) / Gt;
+    if (v1 > g.Vh - 1) v1 = g.Vh - 1;
+    ns = v1 >= v0 ? (int)(v1 - v0 + 1) : 0;
+    const int nw = 8 * NL, ne = nw + (clov_staged ? 36 : 0);
+    for (int i = threadIdx.x; i < ne * ns; i += blockDim.x) {
+      const int e = i / ns, sidx = i - e * ns;
+      const long hv = site_of(v0 + sidx);
+      stage[i] = e < nw ? ((const C *)a.W)[aidx(e, hv, nw)] : ((const C *)a.clov)[aidx(e - nw, hv, 36)];
+    }
+    __syncthreads();
+  }
+  if (v >= g.Vh) return;
+  const int rb = (int)(q - v * Gt) * RB;
+  const long h = site_of(v);
+  const int sv = (int)(v - v0);
   int rl[RB];
 #pragma unroll
   for (int j = 0; j < RB; ++j) rl[j] = min(rb + j, N - 1);
@@ -236,7 +274,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   for (int j = 0; j < RB; ++j)
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
-  hop_site<R, NL, DAG, RB, SI>(a, h, rl, c[0], c[1], c[2], c[3], acc);
+  hop_site<R, NL, DAG, RB, SI, LS>(a, h, rl, c[0], c[1], c[2], c[3], acc, stage + sv, ns);
   // epilogue
   const bool zb = a.open_bc && (c[3] == 0 || c[3] == g.L[3] - 1);
   if (zb) {
@@ -245,7 +283,9 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
 #pragma unroll
       for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
   } else if (EPI & EPI_CLOV) {
-    if (a.clov) {
+    if (LS && clov_staged) {
+      clov_apply_inplace_at<R, RB>(stage + 8 * NL * ns + sv, (long)ns, acc);
+    } else if (a.clov) {
       clov_apply_inplace<R, RB>((const C *)a.clov, h, acc);
     } else {
 #pragma unroll
@@ -310,16 +350,28 @@ template <typename R, typename SI, typename SO> constexpr bool epi_instantiated(
 
 template <typename R, int NL, int DAG, int RB, typename SI, typename SO>
 static bool hop_epi(const HopLaunch &a, hipStream_t s) {
+  using C = typename c2<R>::t;
   const long threads = kBlk * a.shape.tpb;
-  const long items = a.g.Vh * (long)((a.nrhs + RB - 1) / RB);
+  const long G = (a.nrhs + RB - 1) / RB;
+  const long items = a.g.Vh * G;
   const unsigned grid = (unsigned)((items + threads - 1) / threads);
-  // LDS reserved only to cap the workgroups per CU (the kernel uses none)
-  const size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
+  // LDS: the staging area (shape.ls), at least the share of a CU that caps
+  // the workgroups per CU at shape.occ
+  size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
+  if (a.shape.ls) {
+    const size_t st = (size_t)(8 * NL + ((a.epi & EPI_CLOV) && a.clov ? 36 : 0)) *
+                      (size_t)stage_sites(threads, G) * sizeof(C);
+    if (st > (size_t)kMaxStageBytes) return false;
+    lds = std::max(lds, st);
+  }
   switch (a.epi) {
 #define WC_EPI(E)                                                                        \
   case E:                                                                                \
     if constexpr (epi_instantiated<R, SI, SO>(E)) {                                     \
-      k_hop<R, NL, DAG, E, RB, SI, SO><<<grid, (unsigned)threads, lds, s>>>(a);          \
+      if (a.shape.ls)                                                                    \
+        k_hop<R, NL, DAG, E, RB, SI, SO, true><<<grid, (unsigned)threads, lds, s>>>(a);  \
+      else                                                                               \
+        k_hop<R, NL, DAG, E, RB, SI, SO, false><<<grid, (unsigned)threads, lds, s>>>(a); \
       return true;                                                                       \
     }                                                                                    \
     return false;

@@ -5,7 +5,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -37,6 +39,54 @@ extern thread_local std::string g_err;  // message of the last failure (wc_api.h
 // launches are asynchronous: surface launch-configuration errors right away
 #define WC_LAUNCHED() WC_HIP(hipGetLastError())
 
+// Control state of chunked CG iterations (see wck::iter_check): the device
+// control block, per-RHS data and residual history with pinned host mirrors,
+// and the graphs captured for runs of 2, 4, ..., 32 iterations.
+struct IterRun {
+  static constexpr int kGraphs = 6;  // run lengths 1 << j, j = 1 .. 5
+  static constexpr size_t kCtlBytes = sizeof(int) * wck::kCtlInts;
+  static constexpr size_t kDatBytes = sizeof(double) * wck::kDatRows * wck::kMaxRhs;
+  static constexpr size_t kBytes = kCtlBytes + kDatBytes + sizeof(double) * wck::kMaxChunk * wck::kMaxRhs;
+  char *dev = nullptr, *host = nullptr;
+  int *ctl = nullptr, *hctl = nullptr;
+  double *dat = nullptr, *hdat = nullptr, *hist = nullptr, *hhist = nullptr;
+  hipStream_t cap = nullptr;  // capture stream (graphs are launched on the caller's stream)
+  hipGraphExec_t exec[kGraphs] = {};
+  std::vector<uintptr_t> key;  // what the graphs were captured for
+  long graph_runs = 0, direct_runs = 0;
+};
+
+inline void drop_graphs(IterRun &ir) {
+  for (hipGraphExec_t &e : ir.exec)
+    if (e) {
+      (void)hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  ir.key.clear();
+}
+
+inline void free_run(IterRun &ir) {
+  drop_graphs(ir);
+  if (ir.cap) (void)hipStreamDestroy(ir.cap);
+  (void)hipFree(ir.dev);
+  if (ir.host) (void)hipHostFree(ir.host);
+  ir = IterRun();
+}
+
+inline int ensure_run(IterRun &ir) {
+  if (ir.dev) return WC_OK;
+  WC_HIP(hipMalloc((void **)&ir.dev, IterRun::kBytes));
+  WC_HIP(hipHostMalloc((void **)&ir.host, IterRun::kBytes, hipHostMallocDefault));
+  memset(ir.host, 0, IterRun::kBytes);
+  ir.ctl = (int *)ir.dev;
+  ir.hctl = (int *)ir.host;
+  ir.dat = (double *)(ir.dev + IterRun::kCtlBytes);
+  ir.hdat = (double *)(ir.host + IterRun::kCtlBytes);
+  ir.hist = (double *)(ir.dev + IterRun::kCtlBytes + IterRun::kDatBytes);
+  ir.hhist = (double *)(ir.host + IterRun::kCtlBytes + IterRun::kDatBytes);
+  return WC_OK;
+}
+
 struct wc_operator {
   wc_params p{};
   Geom g{};
@@ -62,6 +112,9 @@ struct wc_operator {
   int bps = 0;                  // 64-site blocks per time slice (0: not a whole number)
   wck::HopShape shape[65];      // hop launch shape per right-hand-side count
   bool shape_set[65] = {};      // shape chosen (by the tuner or the caller)
+  int chunk = 0;                // iterations per run: 0 automatic, 1 one at a time, else at most this many
+  const int *halt_cur = nullptr;  // halt word of the run being enqueued (null outside runs)
+  IterRun run;                  // chunked iterations of this operator's CG and mixed solves
 };
 
 inline size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
@@ -125,6 +178,7 @@ inline void free_op(wc_operator *op) {
   (void)hipFree(op->dots);
   (void)hipFree(op->st);
   if (op->host) (void)hipHostFree(op->host);
+  free_run(op->run);
   delete op;
 }
 
@@ -152,7 +206,17 @@ inline wck::HopShape default_shape(const wc_operator *op, int nrhs) {
 
 inline bool shape_valid(const wc_operator *op, const wck::HopShape &sh) {
   return sh.tpb >= 1 && sh.tpb <= wck::kMaxTpb && sh.rb >= 1 && sh.rb <= max_rb_prec(op->prec) && sh.occ >= 0 &&
-         sh.occ <= wck::kMaxOcc && sh.seg >= 1 && (sh.seg == 1 || (op->bps > 0 && sh.seg <= op->bps));
+         sh.occ <= wck::kMaxOcc && sh.seg >= 1 && (sh.seg == 1 || (op->bps > 0 && sh.seg <= op->bps)) &&
+         (sh.ls == 0 || sh.ls == 1);
+}
+
+// the LDS staging area of a shape (links and clover blocks of the sites a
+// workgroup covers) fits the per-workgroup limit
+inline bool stage_fits(const wc_operator *op, int nrhs, const wck::HopShape &sh) {
+  if (!sh.ls) return true;
+  const long G = (nrhs + sh.rb - 1) / sh.rb;
+  const size_t bytes = (size_t)(8 * op->nl + 36) * (size_t)wck::stage_sites((long)kBlk * sh.tpb, G) * op->csz;
+  return bytes <= (size_t)wck::kMaxStageBytes;
 }
 
 // process-wide cache of tuned shapes: (dims, precision, link format, class of
@@ -180,6 +244,7 @@ inline wck::HopShape shape_for(const wc_operator *op, int nrhs) {
     if (it != g_shape_cache.end()) {
       wck::HopShape sh = it->second;
       if (sh.rb > nrhs) sh.rb = 1;
+      if (!stage_fits(op, nrhs, sh)) sh.ls = 0;
       return sh;
     }
   }
@@ -210,6 +275,7 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
   a.clov_scalar = 1.0 / op->diag;
   a.shape = shape_for(op, nrhs);
   a.bps = op->bps;
+  a.halt = op->halt_cur;
   return a;
 }
 
@@ -217,8 +283,8 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
 #define WC_HOP(R, a, s)                                                                        \
   do {                                                                                         \
     if (!wck::hop<R>((a), (s)))                                                                \
-      WC_FAIL(WC_ERR_ARG, "hop: launch shape rb=%d tpb=%d occ=%d seg=%d is outside the limits",   \
-              (a).shape.rb, (a).shape.tpb, (a).shape.occ, (a).shape.seg);                      \
+      WC_FAIL(WC_ERR_ARG, "hop: launch shape rb=%d tpb=%d occ=%d seg=%d ls=%d is outside the limits", \
+              (a).shape.rb, (a).shape.tpb, (a).shape.occ, (a).shape.seg, (a).shape.ls);        \
   } while (0)
 
 // out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
@@ -268,7 +334,7 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
     b.dots = op->dots;
     b.half_in = b.half_out = half;
     WC_HOP(R, b, s);
-    if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s);
+    if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s, op->halt_cur);
     src = dst_mpc;
   }
   if (which == 1 || which == 2) {
@@ -284,7 +350,7 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
     b.dots = op->dots;
     b.half_in = half;  // the product itself stays in the working precision
     WC_HOP(R, b, s);
-    if (dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s);
+    if (dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s, op->halt_cur);
   }
   WC_LAUNCHED();
   return WC_OK;
@@ -311,15 +377,129 @@ inline int inner_iteration(wc_operator *lo, int N, int pd, void *r, void *p, voi
   const long n = lo->g.Vh * 12 * N;
   if (!st) st = lo->st;
   if (!partials) partials = lo->partials;
+  const int *halt = lo->halt_cur;
   if ((rc = mpc_impl<R>(lo, N, pd, 2, mmp, p, p, s, aD, partials, half))) return rc;
-  wck::cg_alpha(partials, wck::hop_blocks(lo->g), N, st, s);
-  wck::cg_update_r<R>(n, N, st, r, mmp, partials, s);
-  wck::cg_beta(partials, wck::kRedBlocks, N, st, s);
+  wck::cg_alpha(partials, wck::hop_blocks(lo->g), N, st, s, halt);
+  wck::cg_update_r<R>(n, N, st, r, mmp, partials, s, halt);
+  wck::cg_beta(partials, wck::kRedBlocks, N, st, s, halt);
   wck::cg_update_xp<R>(lo->g, N, pd, st, r, p, x, lo->has_clover ? lo->Ai[pd] : nullptr, 1.0 / lo->diag, aD,
-                       lo->open_bc ? 1 : 0, s, half);
+                       lo->open_bc ? 1 : 0, s, half, halt);
   WC_LAUNCHED();
   return WC_OK;
 }
+
+// ------------------------------------------------- chunked CG iterations
+// What a captured run depends on beyond the caller's pointers: the
+// operator's links, clover, coefficients and hop launch shape.
+inline void op_key(const wc_operator *op, int nrhs, std::vector<uintptr_t> &key) {
+  auto bits = [](double d) {
+    uint64_t u;
+    memcpy(&u, &d, sizeof u);
+    return (uintptr_t)u;
+  };
+  const wck::HopShape sh = shape_for(op, nrhs);
+  auto tmp = [&](size_t i) { return i < op->tmp.size() ? (uintptr_t)op->tmp[i] : (uintptr_t)0; };
+  for (uintptr_t v : {(uintptr_t)op, (uintptr_t)op->W[0], (uintptr_t)op->W[1], (uintptr_t)op->Ai[0],
+                      (uintptr_t)op->Ai[1], tmp(0), tmp(1), tmp(2), (uintptr_t)op->dots, (uintptr_t)op->partials, bits(op->diag), bits(op->cs), bits(op->ct), (uintptr_t)op->open_bc,
+                      (uintptr_t)op->has_clover, (uintptr_t)sh.rb, (uintptr_t)sh.tpb, (uintptr_t)sh.occ,
+                      (uintptr_t)sh.seg, (uintptr_t)sh.ls, (uintptr_t)op->bps})
+    key.push_back(v);
+  for (int mu = 0; mu < 4; ++mu) {
+    key.push_back(bits(op->p.boundary_phases[mu][0]));
+    key.push_back(bits(op->p.boundary_phases[mu][1]));
+  }
+}
+
+// Iterations per run: the operator's setting, or automatically from the
+// estimated time of one iteration (HBM bytes of Mpc^dag Mpc and the updates
+// at ~5 TB/s): short iterations are dominated by launch and synchronisation
+// costs, which a captured run of many iterations removes.
+inline int chunk_cap(const wc_operator *op, int nrhs, int setting) {
+  if (setting > 0) return std::min(setting, 32);
+  const double bytes = (double)op->g.Vh * (double)op->csz * (32.0 * op->nl + 240.0 * nrhs + 144.0);
+  const double us = bytes / 5e6;
+  return us < 100.0 ? 32 : us < 400.0 ? 16 : 8;
+}
+
+// Largest power of two <= n within [1, cap].
+inline int pow2_floor(double n, int cap) {
+  int k = 1;
+  while (2 * k <= cap && 2 * k <= n) k *= 2;
+  return k;
+}
+
+// Runs up to K (a power of two <= 32) iterations: enqueue(q) enqueues one
+// iteration followed by iter_check on stream q (with the operator's halt_cur
+// set to the run's halt word).  K > 1 replays a graph captured once per K and
+// key.  On return (the stream synchronised) hctl and the first
+// hctl[kCtlIter] rows of hhist (nrhs each) hold the run's outcome.
+template <typename F>
+inline int run_iterations(IterRun &ir, const std::vector<uintptr_t> &key, int K, int nrhs, hipStream_t s,
+                          F &&enqueue) {
+  int rc;
+  if ((rc = ensure_run(ir))) return rc;
+  if (ir.key != key) {
+    drop_graphs(ir);
+    ir.key = key;
+  }
+  ir.hctl[wck::kCtlHalt] = 0;
+  ir.hctl[wck::kCtlIter] = 0;
+  ir.hctl[wck::kCtlMax] = K;
+  ir.hctl[wck::kCtlErr] = 0;
+  ir.hctl[wck::kCtlTrig] = 0;
+  WC_HIP(hipMemcpyAsync(ir.ctl, ir.hctl, IterRun::kCtlBytes, hipMemcpyHostToDevice, s));
+  if (K <= 1) {
+    if ((rc = enqueue(s))) return rc;
+    ++ir.direct_runs;
+  } else {
+    int j = 0;
+    while ((2 << j) < K) ++j;
+    if (!ir.exec[j]) {
+      if (!ir.cap) WC_HIP(hipStreamCreateWithFlags(&ir.cap, hipStreamNonBlocking));
+      WC_HIP(hipStreamBeginCapture(ir.cap, hipStreamCaptureModeRelaxed));
+      for (int i = 0; i < K && !rc; ++i) rc = enqueue(ir.cap);
+      hipGraph_t gr = nullptr;
+      const hipError_t e = hipStreamEndCapture(ir.cap, &gr);
+      if (rc || e != hipSuccess) {
+        if (gr) (void)hipGraphDestroy(gr);
+        if (rc) return rc;
+        WC_FAIL(WC_ERR_HIP, "capturing a run of %d CG iterations failed: %s", K, hipGetErrorString(e));
+      }
+      const hipError_t ei = hipGraphInstantiate(&ir.exec[j], gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (ei != hipSuccess) {
+        ir.exec[j] = nullptr;
+        WC_FAIL(WC_ERR_HIP, "instantiating a run of %d CG iterations failed: %s", K, hipGetErrorString(ei));
+      }
+    }
+    WC_HIP(hipGraphLaunch(ir.exec[j], s));
+    ++ir.graph_runs;
+  }
+  WC_HIP(hipMemcpyAsync(ir.hctl, ir.ctl, IterRun::kCtlBytes, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipMemcpyAsync(ir.hhist, ir.hist, sizeof(double) * (size_t)K * nrhs, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  const int done = ir.hctl[wck::kCtlIter];
+  if (done < 1 || done > K) WC_FAIL(WC_ERR_HIP, "CG run: %d iterations reported for a run of %d", done, K);
+  return WC_OK;
+}
+
+// average decay of log |r|^2 per iteration from c_ref to c_now over n
+// iterations (negative), or fallback when that is not known
+inline double decay_rate(double c_ref, double c_now, long n, double fallback) {
+  if (n >= 2 && c_ref > 0.0 && c_now > 0.0) {
+    const double r = std::log(c_now / c_ref) / (double)n;
+    if (r < 0.0) return r;
+  }
+  return fallback;
+}
+// iterations from c_now to target at that rate; -1 when unknown
+inline double iters_to(double c_now, double target, double rate) {
+  if (!(rate < 0.0) || !(c_now > 0.0) || !(target > 0.0)) return -1.0;
+  if (c_now <= target) return 0.0;
+  return std::log(target / c_now) / rate;
+}
+// run length when no prediction is available
+constexpr int kFirstRun = 4;
 
 inline int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
   if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);

@@ -26,8 +26,12 @@ struct HopShape {
   int tpb = 1;
   int occ = 0;
   int seg = 1;  // > 1: t-marching order, seg runs per time slice (block_order)
+  int ls = 0;   // 1: a workgroup stages its sites' links (and EPI_CLOV blocks) in LDS once
 };
 constexpr int kLdsPerCU = 160 * 1024;
+constexpr int kMaxStageBytes = 80 * 1024;  // largest LDS staging area of one workgroup
+// sites a workgroup of `threads` items covers, G items per site
+inline long stage_sites(long threads, long G) { return (threads + G - 1) / G + 1; }
 constexpr int kMaxOcc = 32;
 constexpr int kMaxTpb = 8;  // up to 512 threads per workgroup
 // largest RB instantiated for the precision (beyond it the accumulators and
@@ -59,6 +63,7 @@ struct HopLaunch {
   int half_out;      // FP32 hops: out stored as 16-bit c2h
   HopShape shape;    // launch shape (1 <= tpb <= kMaxTpb, 1 <= rb <= max_rb, 0 <= occ <= kMaxOcc)
   int bps;           // 64-site blocks per time slice (0: not a whole number; then seg must be 1)
+  const int *halt;   // non-null and *halt != 0: the launch does nothing (chunked iterations)
 };
 
 // number of blocks a site launch uses for Vh sites
@@ -67,7 +72,8 @@ int site_blocks(long Vh);
 inline int hop_blocks(const Geom &g) { return (int)(g.Vh / 64); }
 // partial[r * (Vh / 64) + b] = sum over the 64 sites of block b, in site
 // order, of dots[h * nrhs + r]
-void dot_blocks(const Geom &g, int nrhs, const double *dots, double *partial, hipStream_t s);
+void dot_blocks(const Geom &g, int nrhs, const double *dots, double *partial, hipStream_t s,
+                const int *halt = nullptr);
 #ifndef WC_SITE_THREADS
 #define WC_SITE_THREADS 64
 #endif
@@ -140,18 +146,52 @@ void build_clover(const Geom &g, const void *Unat, const CloverParams &cp, void 
 
 // CG on nrhs independent right-hand sides with device scalars:
 // st[j * nrhs + r], j = 0 c, 1 d, 2 alpha, 3 beta, 4 cp, 5 active (1/0);
-// partials[r * nblocks + b]
-void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s);
-void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s);
+// partials[r * nblocks + b].  halt (optional, here and below): a device
+// word; while it is non-zero the launch does nothing.
+void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s, const int *halt = nullptr);
+void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s, const int *halt = nullptr);
 // r -= alpha_r mmp with per-RHS partials of |r|^2 (kRedBlocks blocks)
 template <typename R>
-void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s);
+void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s,
+                 const int *halt = nullptr);
 // x += alpha_r p ; p = beta_r p + r ; t = A p with A the clover-inverse
 // blocks of parity par (or diag when null), zero on an open boundary.
 // half: FP32 only, p and t stored as 16-bit c2h.
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
-                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half = 0);
+                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half = 0,
+                  const int *halt = nullptr);
+
+// Chunked CG iterations: a run of up to K iterations is enqueued (or
+// replayed from a captured graph) without the host looking in between.
+// After each iteration iter_check decides on the device, from the iterated
+// |r|^2 of every right-hand side (st row 4), whether the run goes on; once
+// it sets ctl[kCtlHalt] every later kernel of the run returns at once, so the
+// state is exactly the one after the deciding iteration.
+enum {
+  kCtlHalt = 0,   // set: the rest of the run does nothing
+  kCtlIter = 1,   // iterations done in this run
+  kCtlMax = 2,    // iterations allowed in this run
+  kCtlErr = 3,    // 1 + first right-hand side with a non-finite |r|^2, or 0
+  kCtlTrig = 4,   // mode 1: a reliable update is due
+  kCtlInts = 16,  // ints of the control block (64 B)
+};
+// dat rows (kMaxRhs entries each): 0 stop limit (mode 0: eps^2 |b|^2;
+// mode 1: tol^2 |src|^2 for the predictor), 1 running max of |r|^2 since the
+// last reliable update, 2 ratio |true r|^2 / |iterated r|^2 of that update,
+// 3 frame scale s, 4 iterations so far, 5 iteration limit
+constexpr int kMaxRhs = 64;
+constexpr int kDatRows = 6;
+constexpr int kMaxChunk = 64;  // iterations per run (rows of hist)
+// mode 0 (CG, cg.py:95-112): an active right-hand side whose |r|^2 <= limit
+// is made inactive (st row 5 = 0); the run halts when none is left.
+// mode 1 (reliable-update trigger): the run halts when any active right-hand
+// side has |r|^2 < delta2 * max, reaches its iteration limit, or is
+// predicted to meet its tolerance (ratio |r|^2 / s^2 <= limit).
+// Every mode: hist[k * nrhs + r] = |r|^2 of iteration k of the run; a
+// non-finite |r|^2 halts with kCtlErr; iteration kCtlMax halts.
+void iter_check(int nrhs, int mode, double delta2, double *st, double *dat, int *ctl, double *hist,
+                hipStream_t s);
 
 // per-right-hand-side scalings of the 16-bit-storage inner solve (n complex
 // entries, RHS innermost): dst = (float) (src * f[r]); x16 *= f[r];

@@ -551,7 +551,8 @@ __device__ __forceinline__ double rhs_total(const double *p, int nb, int r, doub
 }
 
 // per-(block, RHS) sums of per-site products, sites in order
-__global__ void k_dot_blocks(long nb, int N, const double *dots, double *partial) {
+__global__ void k_dot_blocks(long nb, int N, const double *dots, double *partial, const int *halt) {
+  if (halt && *halt) return;
   const long t = (long)blockIdx.x * 256 + threadIdx.x;
   if (t >= nb * N) return;
   const long b = t / N;
@@ -561,9 +562,9 @@ __global__ void k_dot_blocks(long nb, int N, const double *dots, double *partial
   for (int i = 0; i < kBlk; ++i) s += p[(long)i * N];
   partial[(long)r * nb + b] = s;
 }
-void dot_blocks(const Geom &g, int nrhs, const double *dots, double *partial, hipStream_t s) {
+void dot_blocks(const Geom &g, int nrhs, const double *dots, double *partial, hipStream_t s, const int *halt) {
   const long nb = hop_blocks(g);
-  k_dot_blocks<<<(unsigned)((nb * nrhs + 255) / 256), 256, 0, s>>>(nb, nrhs, dots, partial);
+  k_dot_blocks<<<(unsigned)((nb * nrhs + 255) / 256), 256, 0, s>>>(nb, nrhs, dots, partial, halt);
 }
 
 // out[r] = sum_b partials[r * nb + b]; one block per RHS
@@ -617,7 +618,8 @@ template void scale_rhs<float>(long, int, const double *, void *, hipStream_t);
 
 // ------------------------------------------------------------ CG scalars
 // st[j * N + r]: j = 0 c, 1 d, 2 alpha, 3 beta, 4 cp, 5 active
-__global__ __launch_bounds__(256) void k_cg_alpha(const double *p, int nb, int N, double *st) {
+__global__ __launch_bounds__(256) void k_cg_alpha(const double *p, int nb, int N, double *st, const int *halt) {
+  if (halt && *halt) return;
   __shared__ double red[256 / 64 + 1];
   const int r = blockIdx.x;
   const double s = rhs_total(p, nb, r, red);
@@ -626,7 +628,8 @@ __global__ __launch_bounds__(256) void k_cg_alpha(const double *p, int nb, int N
     st[2 * N + r] = (st[5 * N + r] != 0.0 && s != 0.0) ? st[0 * N + r] / s : 0.0;
   }
 }
-__global__ __launch_bounds__(256) void k_cg_beta(const double *p, int nb, int N, double *st) {
+__global__ __launch_bounds__(256) void k_cg_beta(const double *p, int nb, int N, double *st, const int *halt) {
+  if (halt && *halt) return;
   __shared__ double red[256 / 64 + 1];
   const int r = blockIdx.x;
   const double s = rhs_total(p, nb, r, red);
@@ -640,18 +643,20 @@ __global__ __launch_bounds__(256) void k_cg_beta(const double *p, int nb, int N,
     }
   }
 }
-void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s) {
-  k_cg_alpha<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st);
+void cg_alpha(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s, const int *halt) {
+  k_cg_alpha<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st, halt);
 }
-void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s) {
-  k_cg_beta<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st);
+void cg_beta(const double *partials, int nblocks, int nrhs, double *st, hipStream_t s, const int *halt) {
+  k_cg_beta<<<nrhs, 256, 0, s>>>(partials, nblocks, nrhs, st, halt);
 }
 
 // r -= alpha_r mmp; partial |r|^2 per RHS
 template <typename R>
 __global__ __launch_bounds__(256) void k_cg_update_r(long n, int N, const double *st, typename c2<R>::t *r,
-                                                     const typename c2<R>::t *mmp, double *pre) {
+                                                     const typename c2<R>::t *mmp, double *pre,
+                                                     const int *halt) {
   using C = typename c2<R>::t;
+  if (halt && *halt) return;
   __shared__ double lds[256];
   const long stride = (long)gridDim.x * blockDim.x;
   const long i0 = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -669,12 +674,16 @@ __global__ __launch_bounds__(256) void k_cg_update_r(long n, int N, const double
   if ((int)threadIdx.x < N) pre[(long)threadIdx.x * gridDim.x + blockIdx.x] = seg_sum(lds, blockDim.x, N);
 }
 template <typename R>
-void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s) {
+void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, double *partials, hipStream_t s,
+                 const int *halt) {
   using C = typename c2<R>::t;
-  k_cg_update_r<R><<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, st, (C *)r, (const C *)mmp, partials);
+  k_cg_update_r<R><<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, st, (C *)r, (const C *)mmp, partials,
+                                                                  halt);
 }
-template void cg_update_r<double>(long, int, const double *, void *, const void *, double *, hipStream_t);
-template void cg_update_r<float>(long, int, const double *, void *, const void *, double *, hipStream_t);
+template void cg_update_r<double>(long, int, const double *, void *, const void *, double *, hipStream_t,
+                                  const int *);
+template void cg_update_r<float>(long, int, const double *, void *, const void *, double *, hipStream_t,
+                                 const int *);
 
 // x += alpha_r p ; p = beta_r p + r ; t = A p (site-local, parity par: the
 // clover inverse blocks A, or diag when A is null; zero on an open boundary).
@@ -683,8 +692,10 @@ template <typename R, typename SP>
 __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, int par, const double *st,
                                                               const typename c2<R>::t *r, SP *p,
                                                               typename c2<R>::t *x, const typename c2<R>::t *A,
-                                                              double diag, SP *t, int open_bc) {
+                                                              double diag, SP *t, int open_bc,
+                                                              const int *halt) {
   using C = typename c2<R>::t;
+  if (halt && *halt) return;
   const long tid = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
   const long h = (N == 1) ? tid : tid / N;
   const int rr = (int)(tid - h * N);
@@ -722,23 +733,68 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
 }
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
-                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half) {
+                  const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half,
+                  const int *halt) {
   using C = typename c2<R>::t;
   const int nb = site_blocks(g.Vh * nrhs);
   if constexpr (sizeof(R) == 4) {
     if (half) {
       k_cg_update_xp<R, c2h><<<nb, kSiteThreads, 0, s>>>(g, nrhs, par, st, (const C *)r, (c2h *)p, (C *)x,
-                                                         (const C *)blocks, diag, (c2h *)t, open_bc);
+                                                         (const C *)blocks, diag, (c2h *)t, open_bc, halt);
       return;
     }
   }
   k_cg_update_xp<R, C><<<nb, kSiteThreads, 0, s>>>(g, nrhs, par, st, (const C *)r, (C *)p, (C *)x,
-                                                   (const C *)blocks, diag, (C *)t, open_bc);
+                                                   (const C *)blocks, diag, (C *)t, open_bc, halt);
 }
 template void cg_update_xp<double>(const Geom &, int, int, const double *, const void *, void *, void *,
-                                   const void *, double, void *, int, hipStream_t, int);
+                                   const void *, double, void *, int, hipStream_t, int, const int *);
 template void cg_update_xp<float>(const Geom &, int, int, const double *, const void *, void *, void *,
-                                  const void *, double, void *, int, hipStream_t, int);
+                                  const void *, double, void *, int, hipStream_t, int, const int *);
+
+// one wave; lane r handles right-hand side r (nrhs <= 64)
+__global__ __launch_bounds__(64) void k_iter_check(int N, int mode, double delta2, double *st, double *dat,
+                                                   int *ctl, double *hist) {
+  if (ctl[kCtlHalt]) return;
+  const int r = threadIdx.x;
+  const int k = ctl[kCtlIter];
+  if (k >= kMaxChunk) {  // the history holds kMaxChunk iterations
+    if (r == 0) ctl[kCtlHalt] = 1;
+    return;
+  }
+  const bool act = r < N && st[5 * N + r] != 0.0;
+  bool bad = false, trig = false, left = false;
+  if (act) {
+    const double cp = st[4 * N + r];
+    hist[(long)k * N + r] = cp;
+    bad = !isfinite(cp);
+    if (mode == 0) {
+      if (fabs(cp) <= dat[0 * kMaxRhs + r]) st[5 * N + r] = 0.0;
+      else left = true;
+    } else {
+      double mx = dat[1 * kMaxRhs + r];
+      if (cp > mx) mx = cp;
+      dat[1 * kMaxRhs + r] = mx;
+      const double it = dat[4 * kMaxRhs + r] + 1.0;
+      dat[4 * kMaxRhs + r] = it;
+      const double sc = dat[3 * kMaxRhs + r];
+      trig = cp < delta2 * mx || it >= dat[5 * kMaxRhs + r] ||
+             dat[2 * kMaxRhs + r] * cp / (sc * sc) <= dat[0 * kMaxRhs + r];
+      left = true;
+    }
+  }
+  const unsigned long long badm = __ballot(bad), trigm = __ballot(trig), leftm = __ballot(left);
+  if (r == 0) {
+    ctl[kCtlIter] = k + 1;
+    if (badm) ctl[kCtlErr] = 1 + __builtin_ctzll(badm);
+    if (trigm) ctl[kCtlTrig] = 1;
+    if (badm || trigm || !leftm || k + 1 >= ctl[kCtlMax]) ctl[kCtlHalt] = 1;
+  }
+}
+void iter_check(int nrhs, int mode, double delta2, double *st, double *dat, int *ctl, double *hist,
+                hipStream_t s) {
+  k_iter_check<<<1, 64, 0, s>>>(nrhs, mode, delta2, st, dat, ctl, hist);
+}
 
 // ------------------------------------------- 16-bit-storage inner solve
 __global__ void k_d2f_scaled(long n, int N, const double *f, const double2 *src, float2 *dst) {

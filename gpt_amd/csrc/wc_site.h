@@ -16,13 +16,18 @@ template <typename C> struct CloverBlock {
   C L[15];
 };
 
+// block b of a site whose 36 slots lie at p[k * stride]
+template <typename C>
+__device__ __forceinline__ void clov_load_at(const C *p, long stride, int b, CloverBlock<C> &B) {
+  const C *Ab = p + (long)(b * 18) * stride;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) B.d[k] = Ab[(long)k * stride];
+#pragma unroll
+  for (int k = 0; k < 15; ++k) B.L[k] = Ab[(long)(3 + k) * stride];
+}
 template <typename C>
 __device__ __forceinline__ void clov_load(const C *A, long h, int b, CloverBlock<C> &B) {
-  const C *Ab = A + aidx(b * 18, h, 36);
-#pragma unroll
-  for (int k = 0; k < 3; ++k) B.d[k] = Ab[(long)k * kBlk];
-#pragma unroll
-  for (int k = 0; k < 15; ++k) B.L[k] = Ab[(long)(3 + k) * kBlk];
+  clov_load_at(A + aidx(0, h, 36), (long)kBlk, b, B);
 }
 
 // o = B v on the 6 entries of one chiral block
@@ -52,13 +57,14 @@ __device__ __forceinline__ void clov_apply(const C *A, long h, const C v[12], C 
   }
 }
 
-// v[j] = A v[j] for RB spinors of site h (each block loaded once)
+// v[j] = A v[j] for RB spinors of a site whose slots lie at p[k * stride]
+// (each block loaded once)
 template <typename R, int RB, typename C>
-__device__ __forceinline__ void clov_apply_inplace(const C *A, long h, C v[RB][12]) {
+__device__ __forceinline__ void clov_apply_inplace_at(const C *p, long stride, C v[RB][12]) {
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     CloverBlock<C> B;
-    clov_load(A, h, b, B);
+    clov_load_at(p, stride, b, B);
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
       C o[6];
@@ -67,6 +73,11 @@ __device__ __forceinline__ void clov_apply_inplace(const C *A, long h, C v[RB][1
       for (int k = 0; k < 6; ++k) v[j][6 * b + k] = o[k];
     }
   }
+}
+// the same for site h of packed blocks A
+template <typename R, int RB, typename C>
+__device__ __forceinline__ void clov_apply_inplace(const C *A, long h, C v[RB][12]) {
+  clov_apply_inplace_at<R, RB>(A + aidx(0, h, 36), (long)kBlk, v);
 }
 
 // spinor of site h, right-hand side r of N: entry k at aidx(k, h, 12) * N + r,

@@ -59,6 +59,7 @@ class interface:
             # set mass needs to precede update for clover-type fermions (interface.py:43-45)
             _lib.check(lib.wc_set_mass(self.obj, float(params["mass"]), stream))
             _lib.check(lib.wc_update(self.obj, Ut.data_ptr(), stream))
+            _lib.check(lib.wc_set_iteration_chunk(self.obj, 0))  # a new operator's default schedule
         else:
             p = _lib.wc_params()
             for i in range(4):
@@ -195,16 +196,17 @@ class fine_operator(matrix_operator):
             self.interface._setup(*self.interface.setup_arguments)
         return self.interface.obj
 
-    # hop launch shapes (rb, tpb, occ, seg): rb right-hand sides per thread,
-    # 64 * tpb threads per workgroup, at most occ workgroups per CU (0: no cap),
-    # seg t-marching runs per time slice (1: natural order)
+    # hop launch shapes (rb, tpb, occ, seg, ls): rb right-hand sides per
+    # thread, 64 * tpb threads per workgroup, at most occ workgroups per CU (0:
+    # no cap), seg t-marching runs per time slice (1: natural order), ls 1: a
+    # workgroup stages its sites' links and clover blocks in LDS
     # (wc_tune_hop / wc_set_hop_shape / wc_get_hop_shape); every shape gives
     # bitwise the same result
     def tune_hop(self, n_rhs=None, reps=5):
         """time every launch shape of the hop for n_rhs right-hand sides and keep
         the fastest; returns (shape, milliseconds per call)"""
         n = self.n_rhs if n_rhs is None else int(n_rhs)
-        sh = (ctypes.c_int * 4)()
+        sh = (ctypes.c_int * 5)()
         ms = ctypes.c_float()
         _lib.check(_lib.load().wc_tune_hop(self.handle, n, int(reps), ctypes.addressof(sh),
                                            ctypes.addressof(ms), core.stream_ptr()))
@@ -212,18 +214,32 @@ class fine_operator(matrix_operator):
 
     def hop_shape(self, n_rhs=None):
         n = self.n_rhs if n_rhs is None else int(n_rhs)
-        sh = (ctypes.c_int * 4)()
+        sh = (ctypes.c_int * 5)()
         _lib.check(_lib.load().wc_get_hop_shape(self.handle, n, ctypes.addressof(sh)))
         return tuple(sh)
 
     def set_hop_shape(self, shape, n_rhs=None):
-        """shape = (rb, tpb, occ, seg), or None for automatic"""
+        """shape = (rb, tpb, occ, seg, ls), or None for automatic"""
         n = self.n_rhs if n_rhs is None else int(n_rhs)
         if shape is None:
             _lib.check(_lib.load().wc_set_hop_shape(self.handle, n, None))
             return
-        sh = (ctypes.c_int * 4)(*[int(x) for x in shape])
+        sh = (ctypes.c_int * 5)(*[int(x) for x in shape])
         _lib.check(_lib.load().wc_set_hop_shape(self.handle, n, ctypes.addressof(sh)))
+
+    # CG iteration schedule (wc_set_iteration_chunk): iterations run in
+    # captured graphs of up to `chunk` iterations with the stopping test on the
+    # device; 0 automatic, 1 one at a time (host test after each iteration)
+    def set_iteration_chunk(self, chunk):
+        _lib.check(_lib.load().wc_set_iteration_chunk(self.handle, int(chunk)))
+
+    def iteration_runs(self):
+        """(runs replayed from graphs, runs of one iteration) of this
+        operator's CG solves, and of the mixed solves it is the inner operator
+        of"""
+        g_, d_ = ctypes.c_long(), ctypes.c_long()
+        _lib.check(_lib.load().wc_iteration_runs(self.handle, ctypes.addressof(g_), ctypes.addressof(d_)))
+        return int(g_.value), int(d_.value)
 
     def kappa(self):
         p = self.params

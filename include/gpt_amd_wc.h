@@ -187,19 +187,22 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
                           int *converged, double *true_residual, long *reliable_updates, void *stream);
 
 /* Hop launch shapes (no cgpt counterpart: Grid fixes its kernel launch at
- * compile time).  A shape is four ints {rb, tpb, occ, seg}: rb right-hand
- * sides per thread (1 in FP64, 1..2 in FP32), 64 * tpb threads per
- * workgroup (tpb 1..8), occ (0..32) caps the workgroups per CU by reserving
- * LDS (0: no cap), and seg > 1 visits the 64-site blocks in t-marching order
- * with seg runs per time slice (needs the time slice to be a whole number
- * of blocks, seg <= that number).  Every shape gives bitwise the same
- * result; only the speed differs.
+ * compile time).  A shape is five ints {rb, tpb, occ, seg, ls}: rb
+ * right-hand sides per thread (1 in FP64, 1..2 in FP32), 64 * tpb threads
+ * per workgroup (tpb 1..8), occ (0..32) caps the workgroups per CU by
+ * reserving LDS (0: no cap), seg > 1 visits the 64-site blocks in
+ * t-marching order with seg runs per time slice (needs the time slice to be
+ * a whole number of blocks, seg <= that number), and ls = 1 makes a
+ * workgroup copy the links (and clover blocks) of its sites to LDS once, so
+ * that the several items of a site read them from there (at most 80 KB of
+ * LDS per workgroup).  Every shape gives bitwise the same result; only the
+ * speed differs.
  *
  * wc_tune_hop times the hop of nrhs right-hand sides (reps calls per shape)
  * for every valid shape and keeps the fastest for this operator and, in a
  * process-wide cache, for every operator of the same dimensions, precision
  * and link format, for right-hand-side counts of the same class (1, 2-3,
- * 4-7, 8-15, 16-31, 32-64).  shape (4 ints) and ms_per_call may be NULL.
+ * 4-7, 8-15, 16-31, 32-64).  shape (5 ints) and ms_per_call may be NULL.
  * The CG, solve and batch entry points tune on first use when
  * Vh * nrhs >= 2^17, once per class (environment WC_AUTOTUNE=0 turns tuning
  * off).  The tuner leaves seg at 1.  wc_set_hop_shape fixes a shape (NULL:
@@ -207,6 +210,23 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
 int wc_tune_hop(wc_operator *op, int nrhs, int reps, int *shape, float *ms_per_call, void *stream);
 int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape);
 int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape);
+
+/* CG iteration schedule (no cgpt counterpart: gpt's cg.py reads the
+ * residual on the host after every iteration).  The CG, solve, mixed and
+ * batch entry points run their iterations in runs of up to `chunk`: a run is
+ * one HIP graph (captured once per run length, operator state and field
+ * set, then replayed), in which a device-side check after every iteration
+ * applies the stopping rule (CG) or the reliable-update trigger (mixed,
+ * batch) and turns the rest of the run into no-ops; the host then replays
+ * the run's residual history.  Results, iteration counts and histories do
+ * not depend on the schedule.  chunk: 0 automatic (run lengths predicted
+ * from the residual decay, capped by the iteration's size), 1 one iteration
+ * at a time without graphs, 2..32 a fixed cap.  The setting of the outer
+ * (FP64) operator governs its mixed solves and batches.
+ * wc_iteration_runs reports how many runs this operator's CG and mixed
+ * solves (as the inner operator) made as graphs and one at a time. */
+int wc_set_iteration_chunk(wc_operator *op, int chunk);
+int wc_iteration_runs(const wc_operator *op, long *graph_runs, long *direct_runs);
 
 /* Continuous-batching solver (no cgpt counterpart: gpt solves right-hand
  * sides one after another, matrix_operator.py:288-295).  A wc_batch holds

@@ -117,19 +117,27 @@ def test_every_hop_shape_gives_identical_bits(setup):
         H = pack([g.pick_checkerboard(g.odd, x) for x in single])
         mpc = pc.eo2_ne()(w).Mpc
         rbs = (1, 2) if pr is g.single else (1,)
-        shapes = [(rb, tpb, occ, seg) for rb in rbs for tpb in (1, 3, 8) for occ in (0, 4) for seg in (1, 2, 4)]
+        shapes = [(rb, tpb, occ, seg, ls) for rb in rbs for tpb in (1, 3, 8) for occ in (0, 4) for seg in (1, 2, 4)
+                  for ls in (0, 1)]
         ref = None
+        staged = 0
         for sh in shapes:
-            w.set_hop_shape(sh)
+            try:
+                w.set_hop_shape(sh)
+            except RuntimeError as e:  # a staging area above the LDS limit
+                assert sh[4] == 1 and "staging area" in str(e), (sh, str(e))
+                continue
+            staged += sh[4]
             assert w.hop_shape() == sh
             out = (w.Meooe(H).data.clone(), mpc(H).data.clone())
             if ref is None:
                 ref = out
             else:
                 assert torch.equal(out[0], ref[0]) and torch.equal(out[1], ref[1]), (pr.__name__, sh)
+        assert staged >= 6
         # a CG (hop partials feed alpha) is identical under two shapes
         res = []
-        for sh in [(1, 1, 0, 1), (rbs[-1], 3, 8, 4)]:
+        for sh in [(1, 1, 0, 1, 0), (rbs[-1], 3, 8, 4, 1)]:
             w.set_hop_shape(sh)
             cgN = inv.cg(eps=1e-6 if pr is g.single else 1e-10, maxiter=500)
             X = g(w.propagator(inv.preconditioned(pc.eo2_ne(), cgN)) * pack(single))
@@ -137,11 +145,13 @@ def test_every_hop_shape_gives_identical_bits(setup):
         assert res[0][1] == res[1][1] and torch.equal(res[0][0], res[1][0])
         w.set_hop_shape(None)
         with pytest.raises(RuntimeError, match="invalid"):
-            w.set_hop_shape((3, 1, 0, 1))
+            w.set_hop_shape((3, 1, 0, 1, 0))
         with pytest.raises(RuntimeError, match="invalid"):
-            w.set_hop_shape((1, 9, 0, 1))
+            w.set_hop_shape((1, 9, 0, 1, 0))
         with pytest.raises(RuntimeError, match="invalid"):
-            w.set_hop_shape((1, 1, 0, 5))  # 8^3 x 16: 4 blocks per time slice
+            w.set_hop_shape((1, 1, 0, 5, 0))  # 8^3 x 16: 4 blocks per time slice
+        with pytest.raises(RuntimeError, match="invalid"):
+            w.set_hop_shape((1, 1, 0, 1, 2))
 
 
 def test_tuner_picks_a_valid_shape(setup):
@@ -149,4 +159,5 @@ def test_tuner_picks_a_valid_shape(setup):
     w = g.qcd.fermion.wilson_clover(U, P, precision=g.single, n_rhs=4)
     sh, ms = w.tune_hop(reps=2)
     assert ms > 0 and sh[0] in (1, 2) and sh[1] in (1, 2, 4, 8) and sh[2] in (0, 2, 4, 8, 16) and sh[3] == 1
+    assert sh[4] in (0, 1)
     assert w.hop_shape() == sh
