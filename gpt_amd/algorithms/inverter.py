@@ -6,6 +6,11 @@
   scalars); on any other matrix_operator the same recursion with the HIP
   BLAS-1 kernels.  ``history`` holds |r|^2 per iteration as gpt's
   base_iterative.log_convergence does (lib/gpt/algorithms/base.py:76-106).
+* ``block_cg``          -- lib/gpt/algorithms/inverter/block_cg.py:36-149: the
+  right-hand sides of an ``n_rhs`` field solved together over their block
+  Krylov space (wc_block_cg_eo2_ne); on the eo2_ne normal operator of a
+  gpt_amd fermion, alone or inside ``preconditioned``.  ``history`` holds the
+  largest relative |r|^2 of the block per iteration.
 * ``preconditioned``    -- lib/gpt/algorithms/inverter/preconditioned.py:24-54
 * ``defect_correcting`` -- lib/gpt/algorithms/inverter/defect_correcting.py:74-142
 * ``mixed_precision``   -- lib/gpt/algorithms/inverter/mixed_precision.py:23-32
@@ -188,6 +193,59 @@ class cg_mixed:
         raise TypeError("cg_mixed runs inside preconditioned(eo2_ne(), cg_mixed(...))")
 
 
+class block_cg:
+    """Block CG (block_cg.py:36-149) with the reference's parameters eps,
+    maxiter, eps_abs and miniter."""
+
+    def __init__(self, params=None, **kw):
+        p = {"eps": 1e-15, "maxiter": 1000000, "eps_abs": None, "miniter": 0, "fail_if_not_converged": False}
+        p.update(params or {})
+        p.update(kw)
+        self.params = p
+        self.eps = p["eps"]
+        self.eps_abs = p["eps_abs"]
+        self.maxiter = p["maxiter"]
+        self.miniter = p["miniter"]
+        self.fail_if_not_converged = p["fail_if_not_converged"]
+        self.history = []
+        self.iterations = 0
+        self.converged = None
+
+    def modified(self, **params):
+        return block_cg({**self.params, **params})
+
+    def _call(self, fn, op, parity, dst, src):
+        N = src.grid.nrhs
+        hist = np.zeros(max(self.maxiter, 1))
+        it = np.zeros(1, dtype=np.int64)
+        conv = np.zeros(1, dtype=np.int32)
+        _lib.check(fn(op.handle, parity.tag, N, dst.ptr(), src.ptr(), float(self.eps),
+                      float(self.eps_abs or 0.0), int(self.miniter), int(self.maxiter), it.ctypes.data,
+                      conv.ctypes.data, hist.ctypes.data, core.stream_ptr()))
+        self.iterations = int(it[0])
+        self.history = list(hist[: self.iterations])
+        self.converged = bool(conv[0])
+        if not self.converged and self.fail_if_not_converged:
+            raise ValueError("FATAL error: block CG not converged")
+
+    def solve(self, op, parity, dst, src):
+        """dst = M^-1 src through eo2_ne (full fields)"""
+        self._call(_lib.load().wc_solve_eo2_ne_block_cg, op, parity, dst, src)
+
+    def __call__(self, mat):
+        pc = getattr(mat, "eo2_ne", None)
+        if pc is None:
+            raise TypeError("block_cg runs on the eo2_ne normal operator of a gpt_amd fermion")
+
+        def inv(psi, src):
+            if src is psi:
+                raise ValueError("src must differ from psi")
+            self._call(_lib.load().wc_block_cg_eo2_ne, pc.op, pc.D, psi, src)
+            psi.checkerboard(pc.D)
+
+        return matrix_operator(inv, inv_mat=mat.mat, make_dst=mat.make_dst, accept_guess=(True, False))
+
+
 class preconditioned:
     """M^-1 = L Mpc^-1 R + S (preconditioned.py:24-54)."""
 
@@ -197,6 +255,15 @@ class preconditioned:
 
     def __call__(self, mat):
         matrix = self.preconditioner(mat)
+        if isinstance(self.inverter, block_cg):
+            if getattr(matrix.Mpc, "eo2_ne", None) is None:
+                raise TypeError("block_cg needs the eo2_ne preconditioner of a gpt_amd fermion")
+
+            def inv_block(dst, src):
+                self.inverter.solve(matrix.op, matrix.D, dst, src)
+
+            return matrix_operator(inv_block, inv_mat=mat.mat, make_dst=mat.make_dst,
+                                   accept_guess=(True, False))
         if isinstance(self.inverter, cg_mixed):
             if getattr(matrix.Mpc, "eo2_ne", None) is None:
                 raise TypeError("cg_mixed needs the eo2_ne preconditioner of a gpt_amd fermion")

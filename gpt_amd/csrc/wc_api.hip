@@ -369,37 +369,9 @@ static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, do
 template <typename R>
 static int solve_impl(wc_operator *op, int nrhs, int pd, void *dst, const void *src, double eps, long maxiter,
                       double *history, long *iters, int *conv, double *r2, double *b2, hipStream_t s) {
-  const int pc = 1 - pd;
-  int rc = ensure_tmp(op, 9, nrhs);
-  if (rc) return rc;
-  const size_t hb = half_bytes(op) * (size_t)nrhs;
-  auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
-  void *tR = op->tmp[6], *pb = op->tmp[7], *px = op->tmp[8];
-  void *tC = op->tmp[1];
-  // R src = src_D - Meooe A_C^-1 src_C
-  site_op<R>(op, nrhs, pc, 1, half(src, pc), tC, nullptr, s);
-  {
-    HopLaunch a = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY, tC, tR);
-    a.y = half(src, pd);
-    WC_HOP(R, a, s);
-  }
-  // R_ne src = Mpc^dag R src
-  if ((rc = mpc_impl<R>(op, nrhs, pd, 1, pb, tR, nullptr, s))) return rc;
-  // guess L^-1 dst = A_D dst_D
-  site_op<R>(op, nrhs, pd, 0, half(dst, pd), px, nullptr, s);
-  WC_LAUNCHED();
-  if ((rc = cg_impl<R>(op, nrhs, pd, px, pb, eps, maxiter, history, iters, conv, r2, b2, s))) return rc;
-  // dst = L x + S src: dst_D = A_D^-1 x ; dst_C = A_C^-1 src_C - A_C^-1 Meooe dst_D
-  site_op<R>(op, nrhs, pd, 1, px, half(dst, pd), nullptr, s);
-  site_op<R>(op, nrhs, pc, 1, half(src, pc), tR, nullptr, s);
-  {
-    HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
-    a.clov = op->has_clover ? op->Ai[pc] : nullptr;
-    a.y = tR;
-    WC_HOP(R, a, s);
-  }
-  WC_LAUNCHED();
-  return WC_OK;
+  return schur_solve<R>(op, nrhs, pd, dst, src, s, [&](void *px, const void *pb) {
+    return cg_impl<R>(op, nrhs, pd, px, pb, eps, maxiter, history, iters, conv, r2, b2, s);
+  });
 }
 
 // Mixed-precision solve with reliable updates for nrhs right-hand sides.

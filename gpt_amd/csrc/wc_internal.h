@@ -186,11 +186,24 @@ inline void free_op(wc_operator *op) {
 template <typename R> inline int max_rb_of() { return wck::max_rb<R>(); }
 inline int max_rb_prec(int prec) { return prec == WC_DOUBLE ? max_rb_of<double>() : max_rb_of<float>(); }
 
-// Shape used until the tuner (or the caller) picks one: rb right-hand sides
-// per thread with the least padding (ties: the larger rb), 64 threads.
+// the LDS staging area of a shape (links and clover blocks of the sites a
+// workgroup covers) fits the per-workgroup limit
+inline bool stage_fits(const wc_operator *op, int nrhs, const wck::HopShape &sh) {
+  if (!sh.ls) return true;
+  const long G = (nrhs + sh.rb - 1) / sh.rb;
+  const size_t bytes = (size_t)(8 * op->nl + 36) * (size_t)wck::stage_sites((long)kBlk * sh.tpb, G) * op->csz;
+  return bytes <= (size_t)wck::kMaxStageBytes;
+}
+
+// Shape used until the tuner (or the caller) picks one: at most 4
+// workgroups per CU; with several right-hand sides, rb per thread with the
+// least padding (ties: the larger rb), 256 threads per workgroup and the
+// links staged in LDS where the staging area fits.
 inline wck::HopShape default_shape(const wc_operator *op, int nrhs) {
   wck::HopShape sh;
+  sh.occ = 4;
   if (nrhs <= 1) return sh;
+  sh.tpb = 4;
   const int mrb = max_rb_prec(op->prec);
   int best = 1, waste_best = 1 << 30;
   for (int rb = mrb; rb >= 1; --rb) {
@@ -201,6 +214,8 @@ inline wck::HopShape default_shape(const wc_operator *op, int nrhs) {
     }
   }
   sh.rb = best;
+  sh.ls = 1;
+  if (!stage_fits(op, nrhs, sh)) sh.ls = 0;
   return sh;
 }
 
@@ -208,15 +223,6 @@ inline bool shape_valid(const wc_operator *op, const wck::HopShape &sh) {
   return sh.tpb >= 1 && sh.tpb <= wck::kMaxTpb && sh.rb >= 1 && sh.rb <= max_rb_prec(op->prec) && sh.occ >= 0 &&
          sh.occ <= wck::kMaxOcc && sh.seg >= 1 && (sh.seg == 1 || (op->bps > 0 && sh.seg <= op->bps)) &&
          (sh.ls == 0 || sh.ls == 1);
-}
-
-// the LDS staging area of a shape (links and clover blocks of the sites a
-// workgroup covers) fits the per-workgroup limit
-inline bool stage_fits(const wc_operator *op, int nrhs, const wck::HopShape &sh) {
-  if (!sh.ls) return true;
-  const long G = (nrhs + sh.rb - 1) / sh.rb;
-  const size_t bytes = (size_t)(8 * op->nl + 36) * (size_t)wck::stage_sites((long)kBlk * sh.tpb, G) * op->csz;
-  return bytes <= (size_t)wck::kMaxStageBytes;
 }
 
 // process-wide cache of tuned shapes: (dims, precision, link format, class of
@@ -500,6 +506,44 @@ inline double iters_to(double c_now, double target, double rate) {
 }
 // run length when no prediction is available
 constexpr int kFirstRun = 4;
+
+// dst = L x + S src (preconditioned.py:32-54, schur_complement_two.py) with
+// x from inner(px, pb), which solves Mpc^dag Mpc x = pb = R_ne src starting
+// from the guess px = L^-1 dst = A_D dst_D.  Scratch: tmp[1], tmp[6..8].
+template <typename R, typename F>
+inline int schur_solve(wc_operator *op, int nrhs, int pd, void *dst, const void *src, hipStream_t s, F &&inner) {
+  const int pc = 1 - pd;
+  int rc = ensure_tmp(op, 9, nrhs);
+  if (rc) return rc;
+  const size_t hb = half_bytes(op) * (size_t)nrhs;
+  auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
+  void *tR = op->tmp[6], *pb = op->tmp[7], *px = op->tmp[8];
+  void *tC = op->tmp[1];
+  // R src = src_D - Meooe A_C^-1 src_C
+  site_op<R>(op, nrhs, pc, 1, half(src, pc), tC, nullptr, s);
+  {
+    HopLaunch a = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY, tC, tR);
+    a.y = half(src, pd);
+    WC_HOP(R, a, s);
+  }
+  // R_ne src = Mpc^dag R src
+  if ((rc = mpc_impl<R>(op, nrhs, pd, 1, pb, tR, nullptr, s))) return rc;
+  // guess L^-1 dst = A_D dst_D
+  site_op<R>(op, nrhs, pd, 0, half(dst, pd), px, nullptr, s);
+  WC_LAUNCHED();
+  if ((rc = inner(px, (const void *)pb))) return rc;
+  // dst = L x + S src: dst_D = A_D^-1 x ; dst_C = A_C^-1 src_C - A_C^-1 Meooe dst_D
+  site_op<R>(op, nrhs, pd, 1, px, half(dst, pd), nullptr, s);
+  site_op<R>(op, nrhs, pc, 1, half(src, pc), tR, nullptr, s);
+  {
+    HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
+    a.clov = op->has_clover ? op->Ai[pc] : nullptr;
+    a.y = tR;
+    WC_HOP(R, a, s);
+  }
+  WC_LAUNCHED();
+  return WC_OK;
+}
 
 inline int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
   if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);

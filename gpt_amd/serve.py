@@ -220,16 +220,46 @@ class server:
         self.completed = []
         self.peak_groups = 0
         self.waited_for_memory = 0
+        self.warmed = set()
 
-    def add_field(self, key, U):
+    def add_field(self, key, U, warmup=True):
         """keep a gauge field resident: 4 link fields (gpt_amd.qcd.gauge) or
-        natural-layout numpy arrays, held in HBM in FP64"""
+        natural-layout numpy arrays, held in HBM in FP64.  warmup: tune the
+        hop launch shapes of this lattice size now, for batches of every size
+        class up to nslots, so that no request waits for the tuner"""
         from .qcd import gauge
 
         if not isinstance(U[0], gauge.link_field):
             fd = list(reversed(np.asarray(U[0]).shape[:4]))
             U = gauge.from_numpy(core.grid(fd, core.double), U)
         self.fields[key] = U
+        if warmup:
+            self.warm(U)
+
+    def warm(self, U):
+        """tune the FP64 and inner operators' hops for this lattice size and
+        batches of 1, 2-3, 4-7, ... right-hand sides (up to nslots); the tuned
+        shapes are kept process-wide per lattice size, precision, link format
+        and class, so it runs once per lattice size"""
+        from .qcd import fermion
+
+        fd = tuple(U[0].grid.fdimensions)
+        if fd in self.warmed:
+            return
+        Vh = int(np.prod(fd)) // 2
+        p = dict(self.params, mass=0.1)
+        ops = [fermion.wilson_clover(U, p),
+               fermion.wilson_clover(U, dict(p, compress_links=True), precision=self.inner_precision)]
+        n = 1
+        while n <= self.nslots:
+            # below 2^17 site-RHS pairs solves do not tune (launch shapes matter little)
+            if Vh * n >= (1 << 17):
+                for op in ops:
+                    op.tune_hop(n_rhs=n, reps=2)
+            n *= 2
+        for op in ops:
+            op.release()
+        self.warmed.add(fd)
 
     # a group's batch has as many slots as the request that opens it has
     # right-hand sides (at most nslots): a slot costs the same whether or not

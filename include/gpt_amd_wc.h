@@ -186,6 +186,26 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
                           double tol, long maxiter, double delta, int inner_storage, long *iterations,
                           int *converged, double *true_residual, long *reliable_updates, void *stream);
 
+/* Block CG on Mpc^dag Mpc (lib/gpt/algorithms/inverter/block_cg.py:36-149,
+ * which gpt runs in Python on lists of fields): the nrhs right-hand sides of
+ * one multi-RHS half field b are solved together over their block Krylov
+ * space, with the Cholesky-QR factorisation of the residual block each
+ * iteration.  Stops when every right-hand side has |r|^2 <= eps^2 |b|^2, or
+ * (eps_abs > 0) |r|^2 <= eps_abs^2 for all, after at least miniter
+ * iterations.  x holds the guess and gets the solution.  history (may be
+ * NULL; maxiter doubles) gets the largest relative |r|^2 of the block per
+ * iteration.  A block whose residuals are not linearly independent (a zero
+ * right-hand side beside non-zero ones, or right-hand sides that converge
+ * at very different rates) makes the Cholesky factorisation fail, as in the
+ * reference: WC_ERR_NUMERIC.  An all-zero block gives x = 0.
+ * wc_solve_eo2_ne_block_cg is the same inside the eo2_ne Schur solve
+ * (dst, src: full fields), as wc_solve_eo2_ne_block is for CG. */
+int wc_block_cg_eo2_ne(wc_operator *op, int parity, int nrhs, void *x, const void *b, double eps, double eps_abs,
+                       long miniter, long maxiter, long *iterations, int *converged, double *history, void *stream);
+int wc_solve_eo2_ne_block_cg(wc_operator *op, int parity, int nrhs, void *dst, const void *src, double eps,
+                             double eps_abs, long miniter, long maxiter, long *iterations, int *converged,
+                             double *history, void *stream);
+
 /* Hop launch shapes (no cgpt counterpart: Grid fixes its kernel launch at
  * compile time).  A shape is five ints {rb, tpb, occ, seg, ls}: rb
  * right-hand sides per thread (1 in FP64, 1..2 in FP32), 64 * tpb threads

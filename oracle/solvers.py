@@ -134,38 +134,46 @@ def defect_correcting(outer_mat, inner_inv, src, eps=1e-15, maxiter=1000000, psi
     return psi, maxiter, False
 
 
-def block_cg(mat, srcs, eps=1e-15, maxiter=1000000):
-    """Block CG with Cholesky QR (block_cg.py:36-149); returns (X, iterations)."""
+def _qc_is_r(R):
+    """Q C = R with C upper triangular from the Cholesky factor of the
+    Hermitised Gram matrix R^dag R (block_cg.py:24-32); returns (Q, C)."""
+    G = R.conj().T @ R
+    G = 0.5 * (G + G.conj().T)
+    L = np.linalg.cholesky(G)  # G = L L^dag
+    C = L.conj().T
+    return R @ np.linalg.inv(C), C
+
+
+def block_cg(mat, srcs, eps=1e-15, maxiter=1000000, eps_abs=None, miniter=0, guess=None, history=None):
+    """Block CG with Cholesky QR (block_cg.py:36-149) on the columns srcs;
+    returns (X, iterations, converged).  history (optional list) gets the
+    largest relative |r|^2 of the block per iteration."""
     n = len(srcs)
     shp = srcs[0].shape
     B = np.stack([s.reshape(-1) for s in srcs], axis=1)
-    X = np.zeros_like(B)
-    R = B - np.stack([mat(X[:, j].reshape(shp)).reshape(-1) for j in range(n)], axis=1)
     ssq = np.array([np.vdot(B[:, j], B[:, j]).real for j in range(n)])
+    if ssq.sum() == 0.0:
+        return [np.zeros(shp, B.dtype) for _ in range(n)], 0, True
+    X = np.zeros_like(B) if guess is None else np.stack([x.reshape(-1) for x in guess], axis=1)
 
-    def qr(M):
-        G = M.conj().T @ M
-        C = np.linalg.cholesky(G)  # G = C C^dag
-        Q = M @ np.linalg.inv(C.conj().T)
-        return Q, C.conj().T
+    def A(Y):
+        return np.stack([mat(Y[:, j].reshape(shp)).reshape(-1) for j in range(n)], axis=1)
 
-    Q, Cr = qr(R)
-    S = np.eye(n, dtype=np.complex128)
-    P = Q.copy()
-    it = 0
-    for it in range(1, maxiter + 1):
-        AP = np.stack([mat(P[:, j].reshape(shp)).reshape(-1) for j in range(n)], axis=1)
-        alpha = np.linalg.solve(P.conj().T @ AP, S.conj().T)
-        X = X + P @ (alpha @ Cr)
-        Qn, Sn = qr(Q - AP @ alpha)
-        # residual norms: columns of Q_{k+1} S_{k+1} ... C
-        S = Sn
-        Cr_new = S @ Cr
-        res = np.linalg.norm(Cr_new, axis=0) ** 2
-        Q = Qn
-        P = Q + P @ S.conj().T
-        Cr = Cr_new
-        S = np.eye(n, dtype=np.complex128)
-        if np.all(res <= eps**2 * ssq):
-            break
-    return [X[:, j].reshape(shp) for j in range(n)], it
+    Q, C = _qc_is_r(B - A(X))
+    D = Q.copy()
+    k = -1
+    for k in range(maxiter):
+        Z = A(D)
+        M = np.linalg.inv(D.conj().T @ Z)
+        X = X + D @ (M @ C)
+        Q, S = _qc_is_r(Q - Z @ M)
+        D = Q + D @ S.conj().T
+        C = S @ C
+        rr = np.real(np.diag(C.conj().T @ C))
+        rel, ab = (rr / ssq).max(), rr.max()
+        if history is not None:
+            history.append(rel)
+        if k + 1 >= miniter:
+            if (eps_abs is not None and ab <= eps_abs**2) or rel <= eps**2:
+                return [X[:, j].reshape(shp) for j in range(n)], k + 1, True
+    return [X[:, j].reshape(shp) for j in range(n)], k + 1, False

@@ -123,3 +123,34 @@ def test_correlator_known_answer(kats):
             assert it > 0
             corr += (np.abs(dst) ** 2).sum(axis=(1, 2, 3, 4, 5))
     assert np.linalg.norm(corr - np.array(k["reference"])) < k["tol_norm"]
+
+
+def test_block_cg_pinned(tiny_op):
+    """block_cg of the oracle: one right-hand side is CG (same iterations +-1,
+    same solution); on the tiny lattice four right-hand sides match the dense
+    solve of the normal operator, within fewer iterations than CG needs for
+    any of them (block Krylov space)."""
+    w, g = tiny_op
+    pc = S.eo2_ne(w, parity=1)
+    par = cv.parity(g.shape).reshape(-1)
+    selD = np.repeat(par == 1, 12)
+    Mne = _dense(pc.Mpc_ne, g.shape)[np.ix_(selD, selD)]
+    rng = R.random("block")
+    srcs = [pc.R_ne(rng.vspincolor(g)) for _ in range(4)]
+    x1, it1, conv1 = S.cg(pc.Mpc_ne, srcs[0], eps=1e-12, maxiter=500)
+    (xb,), itb, convb = S.block_cg(pc.Mpc_ne, srcs[:1], eps=1e-12, maxiter=500)
+    assert conv1 and convb and abs(it1 - itb) <= 1
+    assert np.linalg.norm(xb - x1) / np.linalg.norm(x1) < 1e-10
+    hist = []
+    X, it4, conv4 = S.block_cg(pc.Mpc_ne, srcs, eps=1e-12, maxiter=500, history=hist)
+    assert conv4 and len(hist) == it4 and hist[-1] <= 1e-24
+    its = []
+    for s_, x in zip(srcs, X):
+        exact = np.linalg.solve(Mne, s_.reshape(-1)[selD])
+        assert np.linalg.norm(x.reshape(-1)[selD] - exact) / np.linalg.norm(exact) < 1e-10
+        assert np.abs(x.reshape(-1)[~selD]).max() == 0.0
+        its.append(S.cg(pc.Mpc_ne, s_, eps=1e-12, maxiter=500)[1])
+    assert it4 < min(its)
+    # a zero block is solved at once
+    Z, itz, convz = S.block_cg(pc.Mpc_ne, [np.zeros_like(srcs[0])] * 2, eps=1e-12)
+    assert convz and itz == 0 and all(np.abs(z).max() == 0 for z in Z)
