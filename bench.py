@@ -555,6 +555,23 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     out["mixed_multi_half"] = {"iterations_max": max(itr), "reliable_updates": mh.reliable_updates,
                                "max_true_residual": max(res), "s_total": t, "solves_per_s": nrhs / t}
     out["speedup_vs_fp64_sequential"] = out["mixed_multi_half"]["solves_per_s"] * t64
+    # FP64 block CG (block_cg.py) on the same batches, eps as the FP64 CG's
+    wB = wN
+    bcg = inv.block_cg(eps=eps, maxiter=20000)
+    slvb = wB.propagator(inv.preconditioned(pc.eo2_ne(), bcg))
+    g(slvb * batches[0])
+
+    def run_block():
+        res, itr = [], []
+        for S in batches:
+            X = g(slvb * S)
+            res += true_residual(g, wB, X, S)
+            itr.append(bcg.iterations)
+        return res, itr
+
+    (res, itr), t = _timed(torch, run_block)
+    out["block_cg_fp64"] = {"iterations": itr, "max_true_residual": max(res), "s_total": t,
+                            "solves_per_s": nrhs / t}
     log(f"bench: multi-rhs {fd} N={nrhs}: {out}")
     del w, wN, batches, srcs, U
     return out

@@ -62,26 +62,24 @@ __device__ __forceinline__ void accum_lower(C &a2, C &a3, const C c0, const C c1
   }
 }
 
-// direction (MU, FB): the hop coefficient (and the boundary phase on links
-// that wrap around) is folded into the link once, then every spinor is
-// projected, multiplied and accumulated
-template <typename R, int NL, int DAG, int MU, int FB, int RB>
-__device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], typename c2<R>::t u[9],
-                                            const typename c2<R>::t psi[RB][12],
-                                            typename c2<R>::t acc[RB][12]) {
+// the link of direction (mu, fb) as the hop applies it: the third row
+// rebuilt for 12-real storage, the hop coefficient (and the boundary phase on
+// links that wrap around) folded in
+template <typename R, int NL>
+__device__ __forceinline__ void prepare_link(const HopLaunch &a, int mu, int fb, const int c[4],
+                                             typename c2<R>::t u[9]) {
   using C = typename c2<R>::t;
   const Geom &g = a.g;
-  const bool bnd = (FB == 0) ? (c[MU] == g.L[MU] - 1) : (c[MU] == 0);
-  constexpr int S = ((FB == 0) != (DAG != 0)) ? 1 : -1;
+  const bool bnd = (fb == 0) ? (c[mu] == g.L[mu] - 1) : (c[mu] == 0);
   if (NL == 6) {
     // third row = conj(row0 x row1) for SU(3)
     u[6] = cconj(csub(cmul(u[1], u[5]), cmul(u[2], u[4])));
     u[7] = cconj(csub(cmul(u[2], u[3]), cmul(u[0], u[5])));
     u[8] = cconj(csub(cmul(u[0], u[4]), cmul(u[1], u[3])));
   }
-  const R coef = (R)(MU == 3 ? a.ct : a.cs);
+  const R coef = (R)(mu == 3 ? a.ct : a.cs);
   if (bnd) {
-    C f = cmk<C>((R)(a.ph[MU][0]), (R)(FB == 0 ? a.ph[MU][1] : -a.ph[MU][1]));
+    C f = cmk<C>((R)(a.ph[mu][0]), (R)(fb == 0 ? a.ph[mu][1] : -a.ph[mu][1]));
     f = cscale(f, coef);
 #pragma unroll
     for (int e = 0; e < 9; ++e) u[e] = cmul(f, u[e]);
@@ -89,6 +87,17 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], 
 #pragma unroll
     for (int e = 0; e < 9; ++e) u[e] = cscale(u[e], coef);
   }
+}
+
+// direction (MU, FB): the link is prepared (unless PRE: staged prepared),
+// then every spinor is projected, multiplied and accumulated
+template <typename R, int NL, int DAG, int MU, int FB, int RB, bool PRE>
+__device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], typename c2<R>::t u[9],
+                                            const typename c2<R>::t psi[RB][12],
+                                            typename c2<R>::t acc[RB][12]) {
+  using C = typename c2<R>::t;
+  constexpr int S = ((FB == 0) != (DAG != 0)) ? 1 : -1;
+  if constexpr (!PRE) prepare_link<R, NL>(a, MU, FB, c, u);
 #pragma unroll
   for (int j = 0; j < RB; ++j) {
     C hs[2][3];
@@ -123,7 +132,8 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], 
 // half-site index h = ((t*Lz + z)*Ly + y)*Lxh + x/2.  rl[j] is the
 // right-hand side a spinor is read for (clamped into range for padded lanes).
 //
-// LS: the links are staged in LDS, entry e of the site at lw[e * ns].
+// LS: the links are staged in LDS, prepared (prepare_link), entry e of
+// direction dir of the site at lw[(dir * 9 + e) * ns].
 template <typename R, int NL, int DAG, int RB, typename SI, bool LS>
 __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int rl[RB], const int cx,
                                          const int cy, const int cz, const int ct,
@@ -166,7 +176,7 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
     C u[9];
     if constexpr (LS) {
 #pragma unroll
-      for (int e = 0; e < NL; ++e) u[e] = lw[(dir * NL + e) * ns];
+      for (int e = 0; e < 9; ++e) u[e] = lw[(dir * 9 + e) * ns];
     } else {
       const uint32_t dsoff = (uint32_t)(dir * NL) * (uint32_t)(kBlk * sizeof(C));
 #pragma unroll
@@ -182,14 +192,14 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
       for (int k = 0; k < 12; ++k) psi[j][k] = widen(bload<SI, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride));
     }
     switch (dir) {
-      case 0: compute_dir<R, NL, DAG, 0, 0, RB>(a, c, u, psi, acc); break;
-      case 1: compute_dir<R, NL, DAG, 0, 1, RB>(a, c, u, psi, acc); break;
-      case 2: compute_dir<R, NL, DAG, 1, 0, RB>(a, c, u, psi, acc); break;
-      case 3: compute_dir<R, NL, DAG, 1, 1, RB>(a, c, u, psi, acc); break;
-      case 4: compute_dir<R, NL, DAG, 2, 0, RB>(a, c, u, psi, acc); break;
-      case 5: compute_dir<R, NL, DAG, 2, 1, RB>(a, c, u, psi, acc); break;
-      case 6: compute_dir<R, NL, DAG, 3, 0, RB>(a, c, u, psi, acc); break;
-      default: compute_dir<R, NL, DAG, 3, 1, RB>(a, c, u, psi, acc); break;
+      case 0: compute_dir<R, NL, DAG, 0, 0, RB, LS>(a, c, u, psi, acc); break;
+      case 1: compute_dir<R, NL, DAG, 0, 1, RB, LS>(a, c, u, psi, acc); break;
+      case 2: compute_dir<R, NL, DAG, 1, 0, RB, LS>(a, c, u, psi, acc); break;
+      case 3: compute_dir<R, NL, DAG, 1, 1, RB, LS>(a, c, u, psi, acc); break;
+      case 4: compute_dir<R, NL, DAG, 2, 0, RB, LS>(a, c, u, psi, acc); break;
+      case 5: compute_dir<R, NL, DAG, 2, 1, RB, LS>(a, c, u, psi, acc); break;
+      case 6: compute_dir<R, NL, DAG, 3, 0, RB, LS>(a, c, u, psi, acc); break;
+      default: compute_dir<R, NL, DAG, 3, 1, RB, LS>(a, c, u, psi, acc); break;
     }
   }
 }
@@ -239,10 +249,11 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   auto site_of = [&](long vv) {
     return a.shape.seg > 1 ? block_order(vv >> 6, a.shape.seg, a.bps, g.L[3]) * kBlk + (vv & 63) : vv;
   };
-  // ls: the workgroup's sites v0 .. v0 + ns - 1 get their links (and the
-  // EPI_CLOV blocks) copied to LDS once, entry e of site v0 + s at
-  // stage[e * ns + s]; the Gt items of a site then read them from there
-  // instead of each loading them through the texture path
+  // ls: the workgroup's sites v0 .. v0 + ns - 1 get their links, prepared
+  // (prepare_link: 9 entries per direction), and the EPI_CLOV blocks (36
+  // entries) copied to LDS once, entry e of site v0 + s at stage[e * ns + s];
+  // the Gt items of a site then read them from there instead of each loading
+  // them through the texture path and preparing the links again
   extern __shared__ __align__(16) char wc_dyn_lds[];
   C *stage = (C *)wc_dyn_lds;
   const long v0 = q0 / Gt;
@@ -252,12 +263,24 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
     long v1 = (q0 + (long)blockDim.x - 1) / Gt;
     if (v1 > g.Vh - 1) v1 = g.Vh - 1;
     ns = v1 >= v0 ? (int)(v1 - v0 + 1) : 0;
-    const int nw = 8 * NL, ne = nw + (clov_staged ? 36 : 0);
-    for (int i = threadIdx.x; i < ne * ns; i += blockDim.x) {
-      const int e = i / ns, sidx = i - e * ns;
+    const int nw = 8 * NL;
+    for (int i = threadIdx.x; i < 8 * ns; i += blockDim.x) {
+      const int dir = i / ns, sidx = i - dir * ns;
       const long hv = site_of(v0 + sidx);
-      stage[i] = e < nw ? ((const C *)a.W)[aidx(e, hv, nw)] : ((const C *)a.clov)[aidx(e - nw, hv, 36)];
+      int cs[4];
+      half_coords(g, hv, a.pout, cs);
+      C u[9];
+#pragma unroll
+      for (int e = 0; e < NL; ++e) u[e] = ((const C *)a.W)[aidx(dir * NL + e, hv, nw)];
+      prepare_link<R, NL>(a, dir >> 1, dir & 1, cs, u);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) stage[(dir * 9 + e) * ns + sidx] = u[e];
     }
+    if (clov_staged)
+      for (int i = threadIdx.x; i < 36 * ns; i += blockDim.x) {
+        const int e = i / ns, sidx = i - e * ns;
+        stage[72 * ns + i] = ((const C *)a.clov)[aidx(e, site_of(v0 + sidx), 36)];
+      }
     __syncthreads();
   }
   if (v >= g.Vh) return;
@@ -284,7 +307,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
       for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
   } else if (EPI & EPI_CLOV) {
     if (LS && clov_staged) {
-      clov_apply_inplace_at<R, RB>(stage + 8 * NL * ns + sv, (long)ns, acc);
+      clov_apply_inplace_at<R, RB>(stage + 72 * ns + sv, (long)ns, acc);
     } else if (a.clov) {
       clov_apply_inplace<R, RB>((const C *)a.clov, h, acc);
     } else {
@@ -359,7 +382,7 @@ static bool hop_epi(const HopLaunch &a, hipStream_t s) {
   // the workgroups per CU at shape.occ
   size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
   if (a.shape.ls) {
-    const size_t st = (size_t)(8 * NL + ((a.epi & EPI_CLOV) && a.clov ? 36 : 0)) *
+    const size_t st = (size_t)(72 + ((a.epi & EPI_CLOV) && a.clov ? 36 : 0)) *
                       (size_t)stage_sites(threads, G) * sizeof(C);
     if (st > (size_t)kMaxStageBytes) return false;
     lds = std::max(lds, st);

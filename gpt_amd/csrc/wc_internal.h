@@ -186,12 +186,13 @@ inline void free_op(wc_operator *op) {
 template <typename R> inline int max_rb_of() { return wck::max_rb<R>(); }
 inline int max_rb_prec(int prec) { return prec == WC_DOUBLE ? max_rb_of<double>() : max_rb_of<float>(); }
 
-// the LDS staging area of a shape (links and clover blocks of the sites a
-// workgroup covers) fits the per-workgroup limit
+// the LDS staging area of a shape (prepared links, 9 complex per direction,
+// and clover blocks of the sites a workgroup covers) fits the per-workgroup
+// limit
 inline bool stage_fits(const wc_operator *op, int nrhs, const wck::HopShape &sh) {
   if (!sh.ls) return true;
   const long G = (nrhs + sh.rb - 1) / sh.rb;
-  const size_t bytes = (size_t)(8 * op->nl + 36) * (size_t)wck::stage_sites((long)kBlk * sh.tpb, G) * op->csz;
+  const size_t bytes = (size_t)(72 + 36) * (size_t)wck::stage_sites((long)kBlk * sh.tpb, G) * op->csz;
   return bytes <= (size_t)wck::kMaxStageBytes;
 }
 

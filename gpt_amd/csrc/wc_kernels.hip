@@ -693,12 +693,30 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
                                                               const typename c2<R>::t *r, SP *p,
                                                               typename c2<R>::t *x, const typename c2<R>::t *A,
                                                               double diag, SP *t, int open_bc,
-                                                              const int *halt) {
+                                                              const int *halt, int stage) {
   using C = typename c2<R>::t;
   if (halt && *halt) return;
-  const long tid = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
+  const long t0 = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads;
+  const long tid = t0 + threadIdx.x;
   const long h = (N == 1) ? tid : tid / N;
   const int rr = (int)(tid - h * N);
+  // stage: the block's sites s0 .. s0 + ns - 1 get their clover blocks copied
+  // to LDS once (slot k of site s0 + i at cl[k * ns + i]), read by the N items
+  // of each site
+  extern __shared__ __align__(16) char upd_lds[];
+  C *cl = (C *)upd_lds;
+  const long s0 = t0 / N;
+  int ns = 0;
+  if (stage) {
+    long s1 = (t0 + kSiteThreads - 1) / N;
+    if (s1 > g.Vh - 1) s1 = g.Vh - 1;
+    ns = s1 >= s0 ? (int)(s1 - s0 + 1) : 0;
+    for (int i = threadIdx.x; i < 36 * ns; i += kSiteThreads) {
+      const int k = i / ns, si = i - k * ns;
+      cl[i] = A[aidx(k, s0 + si, 36)];
+    }
+    __syncthreads();
+  }
   if (h >= g.Vh) return;
   const R a = (R)st[2 * N + rr], b = (R)st[3 * N + rr];
   C pv[12], v[12];
@@ -724,7 +742,8 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = cmk<C>(0, 0);
   } else if (A) {
-    clov_apply<R>(A, h, pv, v);
+    if (stage) clov_apply_at<R>(cl + (h - s0), (long)ns, pv, v);
+    else clov_apply<R>(A, h, pv, v);
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = cscale(pv[k], (R)diag);
@@ -737,15 +756,20 @@ void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void
                   const int *halt) {
   using C = typename c2<R>::t;
   const int nb = site_blocks(g.Vh * nrhs);
+  // with several right-hand sides the clover blocks are staged (the items of
+  // a site would each load them through the texture path)
+  const int stage = (blocks && nrhs >= 4) ? 1 : 0;
+  const size_t lds = stage ? sizeof(C) * 36 * (size_t)stage_sites(kSiteThreads, nrhs) : 0;
   if constexpr (sizeof(R) == 4) {
     if (half) {
-      k_cg_update_xp<R, c2h><<<nb, kSiteThreads, 0, s>>>(g, nrhs, par, st, (const C *)r, (c2h *)p, (C *)x,
-                                                         (const C *)blocks, diag, (c2h *)t, open_bc, halt);
+      k_cg_update_xp<R, c2h><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (c2h *)p, (C *)x,
+                                                           (const C *)blocks, diag, (c2h *)t, open_bc, halt,
+                                                           stage);
       return;
     }
   }
-  k_cg_update_xp<R, C><<<nb, kSiteThreads, 0, s>>>(g, nrhs, par, st, (const C *)r, (C *)p, (C *)x,
-                                                   (const C *)blocks, diag, (C *)t, open_bc, halt);
+  k_cg_update_xp<R, C><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (C *)p, (C *)x,
+                                                     (const C *)blocks, diag, (C *)t, open_bc, halt, stage);
 }
 template void cg_update_xp<double>(const Geom &, int, int, const double *, const void *, void *, void *,
                                    const void *, double, void *, int, hipStream_t, int, const int *);

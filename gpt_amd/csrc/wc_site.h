@@ -46,15 +46,20 @@ __device__ __forceinline__ void clov_mul6(const CloverBlock<C> &B, const C *v, C
   }
 }
 
-// o = A v for one spinor of site h
+// o = A v for one spinor of a site whose 36 slots lie at p[k * stride]
 template <typename R, typename C>
-__device__ __forceinline__ void clov_apply(const C *A, long h, const C v[12], C o[12]) {
+__device__ __forceinline__ void clov_apply_at(const C *p, long stride, const C v[12], C o[12]) {
 #pragma unroll
   for (int b = 0; b < 2; ++b) {
     CloverBlock<C> B;
-    clov_load(A, h, b, B);
+    clov_load_at(p, stride, b, B);
     clov_mul6<R>(B, v + 6 * b, o + 6 * b);
   }
+}
+// o = A v for one spinor of site h
+template <typename R, typename C>
+__device__ __forceinline__ void clov_apply(const C *A, long h, const C v[12], C o[12]) {
+  clov_apply_at<R>(A + aidx(0, h, 36), (long)kBlk, v, o);
 }
 
 // v[j] = A v[j] for RB spinors of a site whose slots lie at p[k * stride]
