@@ -6,6 +6,12 @@
 * ``eo2_ne``              -- lib/gpt/qcd/fermion/preconditioner/even_odd_sites.py:61-72
   over lib/gpt/algorithms/preconditioner/schur_complement_two.py:58-152 and
   normal_equation.py:35-65 with D = ``parity`` sites (odd by default)
+* ``eo1_ne``              -- even_odd_sites.py:35-46 over
+  lib/gpt/algorithms/preconditioner/schur_complement_one.py and
+  normal_equation.py (N = 1 - DD^-1 DC CC^-1 CD on D)
+* ``eo2_kappa_ne``        -- even_odd_sites.py:75-89: schur_complement_two under
+  similarity_transformation.py with the scalar V = kappa, then the normal
+  equation
 * ``preconditioned``      -- lib/gpt/algorithms/inverter/preconditioned.py:32-54
   (dst = L Mpc^-1 R src + S src, initial guess L^-1 dst = DD P_D dst)
 * ``defect_correcting``   -- lib/gpt/algorithms/inverter/defect_correcting.py:74-142
@@ -107,9 +113,78 @@ class eo2_ne:
         return self.op.MooeeInv(self._mask(src, self.pC), self.pC)
 
 
-def preconditioned(op, src, parity=1, guess=None, **cg_params):
-    """Solve M dst = src through eo2_ne + CG; returns (dst, iterations, converged)."""
-    pc = eo2_ne(op, parity)
+class eo1_ne:
+    """Schur complement one with D = ``parity`` and its normal equation."""
+
+    def __init__(self, op, parity=1):
+        self.op = op
+        self.pD = parity
+        self.pC = 1 - parity
+
+    def _mask(self, x, p):
+        return self.op._mask(x, p)
+
+    def Mpc(self, i_d):
+        op, pD, pC = self.op, self.pD, self.pC
+        t = op.Meooe(i_d, pD)
+        t = op.MooeeInv(t, pC)
+        t = op.Meooe(t, pC)
+        t = op.MooeeInv(t, pD)
+        return i_d - t
+
+    def MpcDag(self, i_d):
+        op, pD, pC = self.op, self.pD, self.pC
+        t = op.MooeeInvDag(i_d, pD)
+        t = op.MeooeDag(t, pD)
+        t = op.MooeeInvDag(t, pC)
+        t = op.MeooeDag(t, pC)
+        return i_d - t
+
+    def Mpc_ne(self, i_d):
+        return self.MpcDag(self.Mpc(i_d))
+
+    def R(self, src):
+        op, pD, pC = self.op, self.pD, self.pC
+        t = op.Meooe(op.MooeeInv(self._mask(src, pC), pC), pC)
+        return op.MooeeInv(self._mask(src, pD) - t, pD)
+
+    def R_ne(self, src):
+        return self.MpcDag(self.R(src))
+
+    def L(self, pc_dst):
+        op, pD, pC = self.op, self.pD, self.pC
+        return pc_dst - op.MooeeInv(op.Meooe(pc_dst, pD), pC)
+
+    def L_inv(self, dst):
+        return self._mask(dst, self.pD)
+
+    def S(self, src):
+        return self.op.MooeeInv(self._mask(src, self.pC), self.pC)
+
+
+class eo2_kappa_ne(eo2_ne):
+    """eo2_ne under the similarity transformation by the scalar kappa:
+    Mpc -> kappa Mpc kappa^-1 = Mpc, R -> kappa R, L -> L kappa^-1,
+    L^-1 -> kappa L^-1 (similarity_transformation.py:22-60)."""
+
+    def __init__(self, op, parity=1):
+        super().__init__(op, parity)
+        self.kappa = op.kappa
+
+    def R_ne(self, src):
+        return self.MpcDag(self.kappa * self.R(src))
+
+    def L(self, pc_dst):
+        return super().L(pc_dst / self.kappa)
+
+    def L_inv(self, dst):
+        return self.kappa * super().L_inv(dst)
+
+
+def preconditioned(op, src, parity=1, guess=None, pc_class=None, **cg_params):
+    """Solve M dst = src through eo2_ne (or pc_class) + CG; returns (dst,
+    iterations, converged)."""
+    pc = (pc_class or eo2_ne)(op, parity)
     pc_src = pc.R_ne(src)
     guess = np.zeros_like(src) if guess is None else guess
     pc_dst0 = pc.L_inv(guess)

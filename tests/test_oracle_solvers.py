@@ -154,3 +154,28 @@ def test_block_cg_pinned(tiny_op):
     # a zero block is solved at once
     Z, itz, convz = S.block_cg(pc.Mpc_ne, [np.zeros_like(srcs[0])] * 2, eps=1e-12)
     assert convz and itz == 0 and all(np.abs(z).max() == 0 for z in Z)
+
+
+def test_eo1_ne_and_eo2_kappa_ne_pinned(tiny_op):
+    """eo1_ne (Schur complement one, normal equation) and eo2_kappa_ne of the
+    oracle: det M = det DD det CC det N for N = 1 - DD^-1 DC CC^-1 CD, both
+    preconditioned solves equal the dense solve, and the kappa-scaled eo2
+    needs the iterations of eo2_ne (+-1: only the scale of its recursion
+    differs)."""
+    w, g = tiny_op
+    par = cv.parity(g.shape).reshape(-1)
+    sel = {p: np.repeat(par == p, 12) for p in (0, 1)}
+    M = _dense(w.M, g.shape)
+    N1 = _dense(S.eo1_ne(w, parity=1).Mpc, g.shape)[np.ix_(sel[1], sel[1])]
+    lhs = np.linalg.slogdet(M)
+    rhs = [np.linalg.slogdet(a) for a in (M[np.ix_(sel[1], sel[1])], M[np.ix_(sel[0], sel[0])], N1)]
+    assert abs(lhs[1] - sum(r[1] for r in rhs)) < 1e-10
+    b = R.random("rhs1").vspincolor(g)
+    x_exact = np.linalg.solve(M, b.reshape(-1)).reshape(b.shape)
+    its = {}
+    for name, cls in [("eo1_ne", S.eo1_ne), ("eo2_ne", S.eo2_ne), ("eo2_kappa_ne", S.eo2_kappa_ne)]:
+        x, it, conv = S.preconditioned(w, b, pc_class=cls, eps=1e-13, maxiter=500)
+        assert conv, name
+        assert np.linalg.norm(x - x_exact) / np.linalg.norm(x_exact) < 1e-11, name
+        its[name] = it
+    assert abs(its["eo2_kappa_ne"] - its["eo2_ne"]) <= 1
