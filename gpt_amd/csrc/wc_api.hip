@@ -750,6 +750,22 @@ int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape) {
   return WC_OK;
 }
 
+int wc_trim(wc_operator *op) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_trim: null operator");
+  WC_HIP(hipDeviceSynchronize());
+  drop_graphs(op->run);
+  for (void *q : op->tmp) (void)hipFree(q);
+  op->tmp.clear();
+  op->tmp_bytes = 0;
+  (void)hipFree(op->partials);
+  op->partials = nullptr;
+  op->npartials = 0;
+  (void)hipFree(op->dots);
+  op->dots = nullptr;
+  op->ndots = 0;
+  return WC_OK;
+}
+
 int wc_set_iteration_chunk(wc_operator *op, int chunk) {
   if (!op) WC_FAIL(WC_ERR_ARG, "wc_set_iteration_chunk: null operator");
   if (chunk < 0 || chunk > 32) WC_FAIL(WC_ERR_ARG, "wc_set_iteration_chunk: chunk = %d must be in [0, 32]", chunk);

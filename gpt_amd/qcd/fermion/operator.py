@@ -91,6 +91,11 @@ class interface:
 
     def suspend(self):
         if self.obj is not None:
+            # a parked operator keeps its links and clover, not its solver scratch
+            try:
+                _lib.check(_lib.load().wc_trim(self.obj))
+            except Exception:  # interpreter shutdown: the runtime may be gone
+                pass
             operator_limbo.setdefault(operator_tag[self.obj], []).append(self.obj)
             self.obj = None
 

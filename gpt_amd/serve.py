@@ -167,6 +167,8 @@ class _group:
                 # reliable updates: 12-real links change the speed, not the result
                 self.inner = fermion.wilson_clover(self.U, dict(p, compress_links=True),
                                                    precision=self.server.inner_precision)
+                if self.server.iteration_chunk is not None:
+                    self.op.set_iteration_chunk(self.server.iteration_chunk)
                 self.batch = batch(self.op, self.inner, self.nslots, delta=self.server.delta,
                                    inner_storage=self.server.inner_storage)
             while self.queue and self.batch.free:
@@ -200,13 +202,15 @@ class server:
     of the device memory free at start)."""
 
     def __init__(self, params, nslots=12, streams=4, iters_per_step=25, hbm_budget=None,
-                 inner_precision=core.single, delta=0.01, inner_storage="auto"):
+                 inner_precision=core.single, delta=0.01, inner_storage="auto", iteration_chunk=None):
         self.params = {k: v for k, v in dict(params).items() if k not in ("mass", "kappa")}
         self.nslots = int(nslots)
         self.iters_per_step = int(iters_per_step)
         self.inner_precision = inner_precision
         self.delta = float(delta)
         self.inner_storage = inner_storage
+        # cap on the inner iterations per captured graph run (None: automatic; 1: no graphs)
+        self.iteration_chunk = iteration_chunk
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.8 * free)

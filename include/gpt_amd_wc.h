@@ -246,6 +246,12 @@ int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape);
  * wc_iteration_runs reports how many runs this operator's CG and mixed
  * solves (as the inner operator) made as graphs and one at a time. */
 int wc_set_iteration_chunk(wc_operator *op, int chunk);
+/* Free the operator's solver scratch (half fields, reduction buffers and
+ * captured iteration graphs; reallocated on next use), keeping its links and
+ * clover.  gpt parks unused operators for reuse (interface.py:23-80); a
+ * parked multi-RHS operator would otherwise hold many GB of scratch.  No
+ * cgpt counterpart. */
+int wc_trim(wc_operator *op);
 int wc_iteration_runs(const wc_operator *op, long *graph_runs, long *direct_runs);
 
 /* Continuous-batching solver (no cgpt counterpart: gpt solves right-hand
