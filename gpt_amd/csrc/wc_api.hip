@@ -468,9 +468,18 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     else wck::zero<double>(n, xl, s);
     if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s))) return rc2;
     wck::axpy<double>(n, -1.0, 0.0, mp, bs, rs, s);
-    if ((rc2 = mpc_impl<double>(hi, N, pd, 1, rh, rs, nullptr, s))) return rc2;
     if ((rc2 = norm2_rhs<double>(hi, N, rs, sth + 9 * N, s))) return rc2;
-    if ((rc2 = norm2_rhs<double>(hi, N, rh, sth + 10 * N, s))) return rc2;
+    if (lo_single) {
+      // Mpc^dag r_s only restarts the inner recursion, which runs in FP32:
+      // it is applied by the inner operator (into ml, free between iterations);
+      // r_s, its norm and the stopping test stay FP64
+      wck::convert_d2f(n, rs, rl, s);
+      if ((rc2 = mpc_impl<float>(lo, N, pd, 1, ml, rl, nullptr, s))) return rc2;
+      if ((rc2 = norm2_rhs<float>(lo, N, ml, sth + 10 * N, s))) return rc2;
+    } else {
+      if ((rc2 = mpc_impl<double>(hi, N, pd, 1, rh, rs, nullptr, s))) return rc2;
+      if ((rc2 = norm2_rhs<double>(hi, N, rh, sth + 10 * N, s))) return rc2;
+    }
     WC_HIP(hipMemcpyAsync(hsh + 8 * N, sth + 8 * N, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, s));
     WC_HIP(hipStreamSynchronize(s));
     ++updates;
@@ -501,13 +510,14 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     }
     if (h16) {
       WC_HIP(hipMemcpyAsync(sth + 11 * N, hsh + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
-      wck::convert_d2f_scaled(n, N, sth + 11 * N, rh, rl, s);
+      WC_HIP(hipMemcpyAsync(rl, ml, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
+      wck::scale_rhs<float>(n, N, sth + 11 * N, rl, s);
       if (!first) {
         wck::scale_rhs_half(n, N, sth + 12 * N, pl, s);
         wck::scale_rhs_half(n, N, sth + 12 * N, aDl, s);
       }
     } else if (lo_single) {
-      wck::convert_d2f(n, rh, rl, s);
+      WC_HIP(hipMemcpyAsync(rl, ml, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
     } else {
       WC_HIP(hipMemcpyAsync(rl, rh, hb, hipMemcpyDeviceToDevice, s));
     }

@@ -131,10 +131,18 @@ static int batch_reliable(wc_batch *b) {
   else wck::zero<double>(n, b->xl, s);
   if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, nullptr, b->partials))) return rc;
   wck::axpy<double>(n, -1.0, 0.0, b->mp, b->bs, b->rs, s);
-  if ((rc = mpc_impl<double>(hi, N, b->pd, 1, b->rh, b->rs, nullptr, s, nullptr, b->partials))) return rc;
   wck::dot_partials_rhs<double>(n, N, b->rs, b->rs, b->partials, nullptr, s);
   wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
-  wck::dot_partials_rhs<double>(n, N, b->rh, b->rh, b->partials, nullptr, s);
+  if (single) {
+    // Mpc^dag r_s only restarts the inner recursion, which runs in FP32: it is
+    // applied by the inner operator (into ml, free between iterations)
+    wck::convert_d2f(n, b->rs, b->rl, s);
+    if ((rc = mpc_impl<float>(lo, N, b->pd, 1, b->ml, b->rl, nullptr, s, nullptr, b->partials))) return rc;
+    wck::dot_partials_rhs<float>(n, N, b->ml, b->ml, b->partials, nullptr, s);
+  } else {
+    if ((rc = mpc_impl<double>(hi, N, b->pd, 1, b->rh, b->rs, nullptr, s, nullptr, b->partials))) return rc;
+    wck::dot_partials_rhs<double>(n, N, b->rh, b->rh, b->partials, nullptr, s);
+  }
   wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 9 * N, s);
   WC_LAUNCHED();
   WC_HIP(hipMemcpyAsync(b->host + 8 * N, b->st + 8 * N, sizeof(double) * 2 * N, hipMemcpyDeviceToHost, s));
@@ -175,11 +183,12 @@ static int batch_reliable(wc_batch *b) {
   }
   if (b->h16) {
     WC_HIP(hipMemcpyAsync(b->st + 11 * N, b->host + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
-    wck::convert_d2f_scaled(n, N, b->st + 11 * N, b->rh, b->rl, s);
+    WC_HIP(hipMemcpyAsync(b->rl, b->ml, hbytes(lo) * N, hipMemcpyDeviceToDevice, s));
+    wck::scale_rhs<float>(n, N, b->st + 11 * N, b->rl, s);
     wck::scale_rhs_half(n, N, b->st + 12 * N, b->pl, s);
     wck::scale_rhs_half(n, N, b->st + 12 * N, b->aD, s);
   } else if (single) {
-    wck::convert_d2f(n, b->rh, b->rl, s);
+    WC_HIP(hipMemcpyAsync(b->rl, b->ml, hbytes(lo) * N, hipMemcpyDeviceToDevice, s));
   } else {
     WC_HIP(hipMemcpyAsync(b->rl, b->rh, hbytes(hi) * N, hipMemcpyDeviceToDevice, s));
   }
