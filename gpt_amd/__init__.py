@@ -23,6 +23,7 @@ from .core import (  # noqa: F401
 )
 from . import qcd  # noqa: F401
 from . import algorithms  # noqa: F401
+from .io import format, load, save  # noqa: F401
 
 
 class _callable_module(_types.ModuleType):

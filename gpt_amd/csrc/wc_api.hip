@@ -1043,6 +1043,26 @@ int wc_field_import(const int fdims[4], int prec, const void *natural, void *eo,
   return WC_OK;
 }
 
+int wc_gauge_observables(const int fdims[4], const void *U, double *plaquette, double *link_trace, void *stream) {
+  Geom g;
+  int rc = geom_of(fdims, &g);
+  if (rc) return rc;
+  if (!U || !plaquette || !link_trace) WC_FAIL(WC_ERR_ARG, "wc_gauge_observables: null argument");
+  if ((rc = red_buffers())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  double *pp = g_red, *pt = g_red + wck::kRedBlocks, *res = g_red + 2 * wck::kRedBlocks;
+  wck::gauge_obs_partials(g, U, pp, pt, s);
+  wck::reduce_final(pp, wck::kRedBlocks, res, s);
+  wck::reduce_final(pt, wck::kRedBlocks, res + 1, s);
+  WC_LAUNCHED();
+  WC_HIP(hipMemcpyAsync(g_red_host, res, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  // gpt's normalisation: averages of Re tr / 3 over the 6 planes and the 4 directions
+  *plaquette = g_red_host[0] / (6.0 * 3.0 * (double)g.V);
+  *link_trace = g_red_host[1] / (4.0 * 3.0 * (double)g.V);
+  return WC_OK;
+}
+
 int wc_field_export(const int fdims[4], int prec, const void *eo, void *natural, void *stream) {
   WC_PREC_CHECK(prec);
   Geom g;

@@ -118,6 +118,11 @@ void axpy_norm2_partials(long n, double ar, double ai, const void *x, const void
 // out[0] = sum of partials (fixed order, one block)
 void reduce_final(const double *partials, int n, double *out, hipStream_t s);
 
+// gauge observables of natural FP64 links U[4][V][3][3] (periodic): per block
+// of a fixed grid (kRedBlocks), the sum over sites of Re tr of the 6
+// plaquettes and of Re tr of the 4 links
+void gauge_obs_partials(const Geom &g, const void *U, double *pplaq, double *ptrace, hipStream_t s);
+
 // precision conversion of n complex entries
 void convert_d2f(long n, const void *src, void *dst, hipStream_t s);
 void convert_f2d(long n, const void *src, void *dst, hipStream_t s);

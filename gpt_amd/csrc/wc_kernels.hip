@@ -174,6 +174,92 @@ void reduce_final(const double *partials, int n, double *out, hipStream_t s) {
   k_reduce_final<<<1, 1024, 0, s>>>(partials, n, out);
 }
 
+// ------------------------------------------------------- gauge observables
+// U: natural FP64 links [4][V][3][3]; per site, Re tr of the 6 plaquettes
+// U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag (mu < nu) and of the 4 links,
+// summed per block (fixed grid: deterministic)
+__device__ __forceinline__ void su3_load(const double2 *U, long V, int mu, long site, double2 m[9]) {
+  const double2 *p = U + ((long)mu * V + site) * 9;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) m[k] = p[k];
+}
+__global__ __launch_bounds__(256) void k_gauge_obs(Geom g, const double2 *U, double *pplaq, double *ptrace) {
+  __shared__ double red[256 / 64 + 1];
+  const long V = 2 * g.Vh;
+  double sp = 0.0, st = 0.0;
+  for (long x = (long)blockIdx.x * 256 + threadIdx.x; x < V; x += (long)gridDim.x * 256) {
+    int c[4];
+    long r = x;
+    for (int mu = 0; mu < 4; ++mu) {
+      c[mu] = (int)(r % g.L[mu]);
+      r /= g.L[mu];
+    }
+    long stride[4];
+    stride[0] = 1;
+    for (int mu = 1; mu < 4; ++mu) stride[mu] = stride[mu - 1] * g.L[mu - 1];
+    auto shift = [&](long site, int mu) {  // site + mu, periodic
+      const int cm = (int)((site / stride[mu]) % g.L[mu]);
+      return cm == g.L[mu] - 1 ? site - (long)(g.L[mu] - 1) * stride[mu] : site + stride[mu];
+    };
+    (void)c;
+    for (int mu = 0; mu < 4; ++mu) {
+      double2 a[9];
+      su3_load(U, V, mu, x, a);
+      st += a[0].x + a[4].x + a[8].x;
+      const long xm = shift(x, mu);
+      for (int nu = mu + 1; nu < 4; ++nu) {
+        double2 b[9], cc[9], d[9];
+        su3_load(U, V, nu, xm, b);               // U_nu(x+mu)
+        su3_load(U, V, mu, shift(x, nu), cc);    // U_mu(x+nu)
+        su3_load(U, V, nu, x, d);                // U_nu(x)
+        // e = a b
+        double2 e[9], f[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const double2 u = a[3 * i + k], v = b[3 * k + j];
+              t.x = fma(u.x, v.x, fma(-u.y, v.y, t.x));
+              t.y = fma(u.x, v.y, fma(u.y, v.x, t.y));
+            }
+            e[3 * i + j] = t;
+          }
+        // f = d cc (then Re tr(e f^dag) = Re tr(a b cc^dag d^dag))
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const double2 u = d[3 * i + k], v = cc[3 * k + j];
+              t.x = fma(u.x, v.x, fma(-u.y, v.y, t.x));
+              t.y = fma(u.x, v.y, fma(u.y, v.x, t.y));
+            }
+            f[3 * i + j] = t;
+          }
+        // Re tr(e f^dag) = sum_ij Re(e_ij conj(f_ij))
+        double tr = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) tr = fma(e[k].x, f[k].x, fma(e[k].y, f[k].y, tr));
+        sp += tr;
+      }
+    }
+  }
+  sp = block_sum<256>(sp, red);
+  st = block_sum<256>(st, red);
+  if (threadIdx.x == 0) {
+    pplaq[blockIdx.x] = sp;
+    ptrace[blockIdx.x] = st;
+  }
+}
+void gauge_obs_partials(const Geom &g, const void *U, double *pplaq, double *ptrace, hipStream_t s) {
+  k_gauge_obs<<<kRedBlocks, 256, 0, s>>>(g, (const double2 *)U, pplaq, ptrace);
+}
+
 // ------------------------------------------------------------- conversions
 __global__ void k_d2f(long n, const double2 *a, float2 *b) {
   long i = (long)blockIdx.x * 256 + threadIdx.x;

@@ -284,6 +284,15 @@ int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);
 int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double *true_residual, int *converged);
 int wc_batch_slots(const wc_batch *b, int *occupied, int *active);
 
+/* Gauge observables of natural-layout FP64 links U[4][V][3][3] (site
+ * ((t*Lz + z)*Ly + y)*Lx + x, periodic): the average plaquette
+ * (1/(18 V)) sum_x sum_{mu<nu} Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+ * and the average link trace (1/(12 V)) sum_{x,mu} Re tr U_mu(x), as gpt's
+ * g.qcd.gauge.plaquette and the NERSC reader's LINK_TRACE check compute them
+ * (lib/gpt/qcd/gauge/loops.py, lib/gpt/core/io/nersc_io.py).  Used by the
+ * NERSC configuration reader and writer (gpt_amd/io.py). */
+int wc_gauge_observables(const int fdims[4], const void *U, double *plaquette, double *link_trace, void *stream);
+
 /* message of the last failed call on this thread */
 const char *wc_last_error(void);
 
