@@ -413,9 +413,11 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
                           "true_residual": max(res), "rel_error_vs_fp64_solution": max(err),
                           "s_per_batch": tN, "solves_per_s": float(nrhs / tN)}
     log(f"bench: cg mixed {nrhs} {out['mixed_multi']}")
-    # (d) the same with 16-bit storage of the inner search direction and Schur intermediates
-    for key, op_, S_, nr in [("mixed_1rhs_half", w, srcs[0], 1), ("mixed_multi_half", wN, S, nrhs)]:
-        mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half")
+    # (d) the same with 16-bit storage (FP16, then BF16) of the inner search
+    # direction and Schur intermediates
+    for key, op_, S_, nr, st in [("mixed_1rhs_half", w, srcs[0], 1, "half"), ("mixed_multi_half", wN, S, nrhs, "half"),
+                                 ("mixed_1rhs_bf16", w, srcs[0], 1, "bf16"), ("mixed_multi_bf16", wN, S, nrhs, "bf16")]:
+        mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage=st)
         slvh = op_.propagator(inv.preconditioned(pc.eo2_ne(), mh))
         g(slvh * S_)
         lat = []
@@ -432,6 +434,8 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
         log(f"bench: cg {key} {out[key]}")
     out["speedup_mixed_multi_vs_fp64"] = out["mixed_multi"]["solves_per_s"] / out["fp64_cg"]["solves_per_s"]
     out["speedup_mixed_multi_half_vs_fp64"] = (out["mixed_multi_half"]["solves_per_s"]
+                                               / out["fp64_cg"]["solves_per_s"])
+    out["speedup_mixed_multi_bf16_vs_fp64"] = (out["mixed_multi_bf16"]["solves_per_s"]
                                                / out["fp64_cg"]["solves_per_s"])
     return out
 

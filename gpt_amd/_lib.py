@@ -12,7 +12,9 @@ LIB_PATH = os.environ.get("GPT_AMD_LIB", os.path.join(_HERE, "libgpt_amd_wc.so")
 
 WC_OK, WC_ERR_ARG, WC_ERR_HIP, WC_ERR_NUMERIC = 0, 1, 2, 3
 WC_DOUBLE, WC_SINGLE = 0, 1
-WC_STORAGE_NATIVE, WC_STORAGE_HALF = 0, 2
+WC_STORAGE_NATIVE, WC_STORAGE_HALF, WC_STORAGE_BF16 = 0, 2, 3
+# inner_storage names of the Python API -> the C codes
+STORAGE_CODES = {None: WC_STORAGE_NATIVE, "half": WC_STORAGE_HALF, "bf16": WC_STORAGE_BF16}
 
 
 class wc_params(ctypes.Structure):

@@ -19,8 +19,9 @@
   solution and the true residuals; used as
   ``preconditioned(eo2_ne(), cg_mixed(eps=1e-10))`` it stops on the full
   system's true relative residual |src - M dst| / |src| <= eps.
-  ``inner_storage="half"`` stores the inner search direction and the Schur
-  intermediates as 16-bit floats (FP32 arithmetic); ``inner_compress_links``
+  ``inner_storage="half"`` (IEEE FP16) or ``"bf16"`` (bfloat16) stores the
+  inner search direction and the Schur intermediates as 16-bit floats (FP32
+  arithmetic); ``inner_compress_links``
   (default True) gives the inner operator 12-real links.  The inner operator
   only drives the recursion between reliable updates -- every true residual
   and the stopping test use the exact FP64 operator -- so approximations in
@@ -146,9 +147,9 @@ class cg_mixed:
         self.maxiter = p["maxiter"]
         self.inner_precision = p["inner_precision"]
         self.delta = p["delta"]
-        if p["inner_storage"] not in (None, "half"):
-            raise ValueError("inner_storage must be None or 'half'")
-        if p["inner_storage"] == "half" and self.inner_precision is not _core.single:
+        if p["inner_storage"] not in _lib.STORAGE_CODES:
+            raise ValueError("inner_storage must be None, 'half' or 'bf16'")
+        if p["inner_storage"] is not None and self.inner_precision is not _core.single:
             raise ValueError("16-bit inner storage needs inner_precision=single")
         self.inner_storage = p["inner_storage"]
         self.inner_compress_links = bool(p["inner_compress_links"])
@@ -180,7 +181,7 @@ class cg_mixed:
         _lib.check(_lib.load().wc_solve_eo2_ne_mixed(
             op.handle, inner.handle, parity.tag, N, dst.ptr(), src.ptr(), float(self.eps),
             int(self.maxiter), float(self.delta),
-            _lib.WC_STORAGE_HALF if self.inner_storage == "half" else _lib.WC_STORAGE_NATIVE,
+            _lib.STORAGE_CODES[self.inner_storage],
             iters.ctypes.data, conv.ctypes.data, tres.ctypes.data, nupd.ctypes.data, core.stream_ptr()))
         self.iterations = [int(x) for x in iters]
         self.true_residual = [float(x) for x in tres]

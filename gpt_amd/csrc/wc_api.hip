@@ -407,11 +407,8 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   const long n = hi->g.Vh * 12 * N;
   double *sth = hi->st, *hsh = hi->host, *stl = lo->st, *hsl = lo->host;
   const bool lo_single = lo->prec == WC_SINGLE;
-  const int h16 = (storage16 && lo_single) ? 1 : 0;
-  // 16-bit storage resolves about 5e-4 of a field's scale: asking one inner
-  // cycle for more than a 1e-2 reduction makes the recursion stagnate or
-  // break down, so reliable updates come at least every 1e-2
-  if (h16) delta = std::max(delta, 1e-2);
+  const int h16 = lo_single ? storage16 : 0;  // kernels' 16-bit storage code
+  if (h16) delta = std::max(delta, s16_delta_floor(h16));
 
   // b_s = R src = src_D - Meooe A_C^-1 src_C ; |src|^2 per RHS
   site_op<double>(hi, N, pc, 1, half(src, pc), tC, nullptr, s);
@@ -513,8 +510,8 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
       WC_HIP(hipMemcpyAsync(rl, ml, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
       wck::scale_rhs<float>(n, N, sth + 11 * N, rl, s);
       if (!first) {
-        wck::scale_rhs_half(n, N, sth + 12 * N, pl, s);
-        wck::scale_rhs_half(n, N, sth + 12 * N, aDl, s);
+        wck::scale_rhs_half(n, N, sth + 12 * N, pl, h16, s);
+        wck::scale_rhs_half(n, N, sth + 12 * N, aDl, h16, s);
       }
     } else if (lo_single) {
       WC_HIP(hipMemcpyAsync(rl, ml, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
@@ -541,8 +538,8 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   if ((rc = reliable(0, true))) return rc;
   // zero sources converge at once (the true residual is 0 <= 0); p = r, A_D^-1 p
   if (h16) {
-    wck::convert_f2h(n, rl, pl, s);
-    site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s, 1);
+    wck::convert_f2h(n, rl, pl, h16, s);
+    site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s, h16);
   } else {
     WC_HIP(hipMemcpyAsync(pl, rl, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
     if (lo_single) site_op<float>(lo, N, pd, 1, pl, aDl, nullptr, s);
@@ -885,12 +882,12 @@ int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int n
   if (rc) return rc;
   if ((rc = maybe_tune(op, nrhs, (hipStream_t)stream))) return rc;
   if ((rc = maybe_tune(inner, nrhs, (hipStream_t)stream))) return rc;
-  if (inner_storage != WC_STORAGE_NATIVE && inner_storage != WC_STORAGE_HALF)
-    WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: inner_storage %d unknown", inner_storage);
-  if (inner_storage == WC_STORAGE_HALF && inner->prec != WC_SINGLE)
+  const int code = storage_code(inner_storage);
+  if (code < 0) WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: inner_storage %d unknown", inner_storage);
+  if (code && inner->prec != WC_SINGLE)
     WC_FAIL(WC_ERR_ARG, "wc_solve_eo2_ne_mixed: 16-bit storage needs an FP32 inner operator");
-  return mixed_impl(op, inner, nrhs, parity, dst, src, tol, maxiter, delta, inner_storage == WC_STORAGE_HALF,
-                    iterations, converged, true_residual, reliable_updates, (hipStream_t)stream);
+  return mixed_impl(op, inner, nrhs, parity, dst, src, tol, maxiter, delta, code, iterations, converged,
+                    true_residual, reliable_updates, (hipStream_t)stream);
 }
 
 int wc_solve_eo2_ne(wc_operator *op, int parity, void *dst, const void *src, double eps, long maxiter,

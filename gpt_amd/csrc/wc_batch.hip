@@ -18,6 +18,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "wc_internal.h"
@@ -27,10 +28,14 @@ namespace {
 // slot r of an N-RHS half field <-> a single-RHS half field: element e of
 // the single field is element e * N + r of the N-RHS one (same AoSoA order)
 template <typename CS, typename CD> __device__ __forceinline__ CD cvt(CS v) {
-  CD o;
-  o.x = (decltype(o.x))v.x;
-  o.y = (decltype(o.y))v.y;
-  return o;
+  if constexpr (std::is_same<CS, CD>::value) {
+    return v;
+  } else {
+    CD o;
+    o.x = (decltype(o.x))v.x;
+    o.y = (decltype(o.y))v.y;
+    return o;
+  }
 }
 template <typename CS, typename CD> __global__ void k_to_slot(long n, int N, int slot, const CS *src, CD *dst) {
   long e = (long)blockIdx.x * 256 + threadIdx.x;
@@ -62,7 +67,7 @@ inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
 struct wc_batch {
   wc_operator *hi = nullptr, *lo = nullptr;
   int N = 0, pd = 1;
-  int h16 = 0;  // 16-bit storage of the inner p and Schur intermediates
+  int h16 = 0;  // 16-bit storage code of the inner p and Schur intermediates (kS16*)
   double delta = 0.1;
   hipStream_t s = nullptr;
   // N-RHS half fields: FP64 y (solution of Mpc y = R src), R src, scratch
@@ -185,8 +190,8 @@ static int batch_reliable(wc_batch *b) {
     WC_HIP(hipMemcpyAsync(b->st + 11 * N, b->host + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
     WC_HIP(hipMemcpyAsync(b->rl, b->ml, hbytes(lo) * N, hipMemcpyDeviceToDevice, s));
     wck::scale_rhs<float>(n, N, b->st + 11 * N, b->rl, s);
-    wck::scale_rhs_half(n, N, b->st + 12 * N, b->pl, s);
-    wck::scale_rhs_half(n, N, b->st + 12 * N, b->aD, s);
+    wck::scale_rhs_half(n, N, b->st + 12 * N, b->pl, b->h16, s);
+    wck::scale_rhs_half(n, N, b->st + 12 * N, b->aD, b->h16, s);
   } else if (single) {
     WC_HIP(hipMemcpyAsync(b->rl, b->ml, hbytes(lo) * N, hipMemcpyDeviceToDevice, s));
   } else {
@@ -210,9 +215,9 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
     if (op->g.L[mu] != inner->g.L[mu]) WC_FAIL(WC_ERR_ARG, "wc_batch_create: operators of different lattices");
   if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_batch_create: parity must be 0 or 1");
   if (!(delta > 0.0 && delta < 1.0)) WC_FAIL(WC_ERR_ARG, "wc_batch_create: need 0 < delta < 1");
-  if (inner_storage != WC_STORAGE_NATIVE && inner_storage != WC_STORAGE_HALF)
-    WC_FAIL(WC_ERR_ARG, "wc_batch_create: inner_storage %d unknown", inner_storage);
-  if (inner_storage == WC_STORAGE_HALF && inner->prec != WC_SINGLE)
+  const int code = storage_code(inner_storage);
+  if (code < 0) WC_FAIL(WC_ERR_ARG, "wc_batch_create: inner_storage %d unknown", inner_storage);
+  if (code && inner->prec != WC_SINGLE)
     WC_FAIL(WC_ERR_ARG, "wc_batch_create: 16-bit storage needs an FP32 inner operator");
   int rc = check_nrhs(op, nslots, "wc_batch_create");
   if (rc) return rc;
@@ -221,10 +226,8 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
   b->lo = inner;
   b->N = nslots;
   b->pd = parity;
-  b->h16 = inner_storage == WC_STORAGE_HALF ? 1 : 0;
-  // 16-bit storage resolves about 5e-4 of a field's scale: reliable updates
-  // at least every 1e-2 of residual reduction
-  b->delta = b->h16 ? std::max(delta, 1e-2) : delta;
+  b->h16 = code;
+  b->delta = std::max(delta, s16_delta_floor(code));
   b->s = (hipStream_t)stream;
   const int N = nslots;
   const size_t hh = hbytes(op) * N, hl = hbytes(inner) * N;
@@ -345,11 +348,15 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
   if (b->h16) {
     k_to_slot_scaled<float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, sn, (const double2 *)rh1, (float2 *)b->rl);
     // p as 16-bit in a single-RHS field, its A_D^-1 from that (as the batch computes it)
-    k_to_slot_scaled<c2h><<<nblk(n1), 256, 0, s>>>(n1, 1, 0, sn, (const double2 *)rh1, (c2h *)pl1);
-    site_op<float>(lo, 1, pd, 1, pl1, aD1, nullptr, s, 1);
-    k_to_slot<c2h, c2h><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const c2h *)pl1, (c2h *)b->pl);
+    if (b->h16 == kS16Bf16)
+      k_to_slot_scaled<c2b><<<nblk(n1), 256, 0, s>>>(n1, 1, 0, sn, (const double2 *)rh1, (c2b *)pl1);
+    else
+      k_to_slot_scaled<c2h><<<nblk(n1), 256, 0, s>>>(n1, 1, 0, sn, (const double2 *)rh1, (c2h *)pl1);
+    site_op<float>(lo, 1, pd, 1, pl1, aD1, nullptr, s, b->h16);
+    // 16-bit entries are moved as 4-byte words
+    k_to_slot<uint32_t, uint32_t><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const uint32_t *)pl1, (uint32_t *)b->pl);
     k_zero_slot<float2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (float2 *)b->xl);
-    k_to_slot<c2h, c2h><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const c2h *)aD1, (c2h *)b->aD);
+    k_to_slot<uint32_t, uint32_t><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const uint32_t *)aD1, (uint32_t *)b->aD);
   } else if (lo->prec == WC_SINGLE) {
     wck::convert_d2f(n1, rh1, pl1, s);
     site_op<float>(lo, 1, pd, 1, pl1, aD1, nullptr, s);

@@ -25,22 +25,39 @@ template <> struct c2<float> { using t = float2; };
 
 // 16-bit storage of a complex FP32 value (4 bytes), used for the search
 // direction and the Schur intermediates of the mixed-precision inner solve;
-// arithmetic is always done after widening to FP32
+// arithmetic is always done after widening to FP32.  c2h holds IEEE half
+// precision (11-bit significand, range 6e-5 .. 65504), c2b bfloat16 (8-bit
+// significand, the FP32 range).  Narrowing rounds to nearest even.
 struct __align__(4) c2h {
   _Float16 x, y;
 };
+struct __align__(4) c2b {
+  __bf16 x, y;
+};
+// storage codes of the 16-bit fields (kernel arguments `half`, HopLaunch
+// half_in / half_out): 0 the working precision, 1 c2h, 2 c2b
+enum { kS16None = 0, kS16Half = 1, kS16Bf16 = 2 };
+template <typename S> constexpr bool is16() { return sizeof(S) == 4; }
 // complex type a stored value is computed in
 template <typename S> struct wide;
 template <> struct wide<double2> { using t = double2; };
 template <> struct wide<float2> { using t = float2; };
 template <> struct wide<c2h> { using t = float2; };
+template <> struct wide<c2b> { using t = float2; };
 template <typename C> __host__ __device__ __forceinline__ C widen(C v) { return v; }
 __device__ __forceinline__ float2 widen(c2h v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ float2 widen(c2b v) { return make_float2((float)v.x, (float)v.y); }
 template <typename S, typename C> __device__ __forceinline__ S narrow(C v) { return v; }
 template <> __device__ __forceinline__ c2h narrow<c2h, float2>(float2 v) {
   c2h o;
   o.x = (_Float16)v.x;
   o.y = (_Float16)v.y;
+  return o;
+}
+template <> __device__ __forceinline__ c2b narrow<c2b, float2>(float2 v) {
+  c2b o;
+  o.x = (__bf16)v.x;
+  o.y = (__bf16)v.y;
   return o;
 }
 
@@ -178,6 +195,11 @@ template <int AUX> struct bload_t<c2h, AUX> {
     return __builtin_bit_cast(c2h, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
   }
 };
+template <int AUX> struct bload_t<c2b, AUX> {
+  static __device__ __forceinline__ c2b go(wc_rsrc r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(c2b, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, AUX));
+  }
+};
 template <typename C, int AUX = 0>
 __device__ __forceinline__ C bload(wc_rsrc r, uint32_t voff, uint32_t soff) {
   return bload_t<C, AUX>::go(r, voff, soff);
@@ -198,6 +220,11 @@ template <int AUX> struct bstore_t<float2, AUX> {
 };
 template <int AUX> struct bstore_t<c2h, AUX> {
   static __device__ __forceinline__ void go(c2h v, wc_rsrc r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, AUX);
+  }
+};
+template <int AUX> struct bstore_t<c2b, AUX> {
+  static __device__ __forceinline__ void go(c2b v, wc_rsrc r, uint32_t voff, uint32_t soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)voff, (int)soff, AUX);
   }
 };

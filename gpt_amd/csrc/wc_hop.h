@@ -366,8 +366,8 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
 template <typename R, typename SI, typename SO> constexpr bool epi_instantiated(int e) {
   using C = typename c2<R>::t;
   if (std::is_same<SI, C>::value && std::is_same<SO, C>::value) return true;
-  if (std::is_same<SI, c2h>::value && std::is_same<SO, c2h>::value) return e == EPI_CLOV || e == EPI_AXPY;
-  if (std::is_same<SI, c2h>::value && std::is_same<SO, C>::value) return e == (EPI_CLOV | EPI_AXPY | EPI_DOT);
+  if (is16<SI>() && std::is_same<SO, SI>::value) return e == EPI_CLOV || e == EPI_AXPY;
+  if (is16<SI>() && std::is_same<SO, C>::value) return e == (EPI_CLOV | EPI_AXPY | EPI_DOT);
   return false;
 }
 
@@ -435,8 +435,9 @@ template <typename R> bool hop_launch(const HopLaunch &a, hipStream_t s) {
   if (sh.seg < 1 || (sh.seg > 1 && (a.bps <= 0 || sh.seg > a.bps))) return false;
   if (!a.half_in && !a.half_out) return hop_links<R, C, C>(a, s);
   if constexpr (sizeof(R) == 4) {
-    if (a.half_in && a.half_out) return hop_links<R, c2h, c2h>(a, s);
-    if (a.half_in) return hop_links<R, c2h, C>(a, s);
+    if (a.half_out && a.half_out != a.half_in) return false;
+    if (a.half_in == kS16Half) return a.half_out ? hop_links<R, c2h, c2h>(a, s) : hop_links<R, c2h, C>(a, s);
+    if (a.half_in == kS16Bf16) return a.half_out ? hop_links<R, c2b, c2b>(a, s) : hop_links<R, c2b, C>(a, s);
   }
   return false;
 }

@@ -119,6 +119,24 @@ struct wc_operator {
 
 inline size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
 
+// the kernels' 16-bit storage code of an inner_storage argument (-1: unknown)
+inline int storage_code(int inner_storage) {
+  switch (inner_storage) {
+    case WC_STORAGE_NATIVE: return kS16None;
+    case WC_STORAGE_HALF: return kS16Half;
+    case WC_STORAGE_BF16: return kS16Bf16;
+    default: return -1;
+  }
+}
+// Smallest reliable-update interval (residual reduction per inner cycle) of
+// a 16-bit storage: FP16 resolves about 5e-4 of a field's scale, BF16 about
+// 4e-3; asking one inner cycle for a reduction near that makes the recursion
+// stagnate or diverge (BF16 at 2e-2 diverges on an 8^3x16 clover solve;
+// 1e-1 needs the fewest iterations and updates on 24^3x48).
+inline double s16_delta_floor(int code) {
+  return code == kS16Bf16 ? 1e-1 : code == kS16Half ? 1e-2 : 0.0;
+}
+
 // n scratch half fields of nrhs right-hand sides each
 inline int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
   const size_t bytes = half_bytes(op) * (size_t)nrhs;

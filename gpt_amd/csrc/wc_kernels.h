@@ -59,8 +59,8 @@ struct HopLaunch {
   double ct;         // temporal hop coefficient -1/2
   double ph[4][2];   // boundary phases
   int open_bc;
-  int half_in;       // FP32 hops: in, y and dotv stored as 16-bit c2h
-  int half_out;      // FP32 hops: out stored as 16-bit c2h
+  int half_in;       // FP32 hops: in, y and dotv stored as 16-bit (kS16Half c2h / kS16Bf16 c2b)
+  int half_out;      // FP32 hops: out stored as the same 16-bit type (0: FP32)
   HopShape shape;    // launch shape (1 <= tpb <= kMaxTpb, 1 <= rb <= max_rb, 0 <= occ <= kMaxOcc)
   int bps;           // 64-site blocks per time slice (0: not a whole number; then seg must be 1)
   const int *halt;   // non-null and *halt != 0: the launch does nothing (chunked iterations)
@@ -85,7 +85,7 @@ template <typename R> bool hop(const HopLaunch &a, hipStream_t s);
 
 // out = A in (blocks != null) or diag * in on parity p, for nrhs right-hand
 // sides; zeroes the open boundary; optional axpy (out = y - A in).  half:
-// FP32 only, in and out stored as 16-bit c2h (y must be null).
+// FP32 only, in and out stored as 16-bit (kS16Half / kS16Bf16; y must be null).
 template <typename R>
 void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
                  const void *y, int open_bc, hipStream_t s, int half = 0);
@@ -161,7 +161,7 @@ void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, d
                  const int *halt = nullptr);
 // x += alpha_r p ; p = beta_r p + r ; t = A p with A the clover-inverse
 // blocks of parity par (or diag when null), zero on an open boundary.
-// half: FP32 only, p and t stored as 16-bit c2h.
+// half: FP32 only, p and t stored as 16-bit (kS16Half / kS16Bf16).
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
                   const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half = 0,
@@ -200,10 +200,10 @@ void iter_check(int nrhs, int mode, double delta2, double *st, double *dat, int 
 
 // per-right-hand-side scalings of the 16-bit-storage inner solve (n complex
 // entries, RHS innermost): dst = (float) (src * f[r]); x16 *= f[r];
-// hi += (double) lo * f[r]; dst16 = src
+// hi += (double) lo * f[r]; dst16 = src.  half: kS16Half or kS16Bf16.
 void convert_d2f_scaled(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s);
-void scale_rhs_half(long n, int nrhs, const double *f, void *x, hipStream_t s);
+void scale_rhs_half(long n, int nrhs, const double *f, void *x, int half, hipStream_t s);
 void accumulate_f2d_scaled(long n, int nrhs, const double *f, const void *lo, void *hi, hipStream_t s);
-void convert_f2h(long n, const void *src, void *dst, hipStream_t s);
+void convert_f2h(long n, const void *src, void *dst, int half, hipStream_t s);
 
 }  // namespace wck

@@ -54,9 +54,14 @@ void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag
   using C = typename c2<R>::t;
   const int nb = site_blocks(g.Vh * nrhs);
   if constexpr (sizeof(R) == 4) {
-    if (half) {
+    if (half == kS16Half) {
       k_site_clover<R, c2h><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const c2h *)in,
                                                         (c2h *)out, (const c2h *)y, open_bc);
+      return;
+    }
+    if (half == kS16Bf16) {
+      k_site_clover<R, c2b><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const c2b *)in,
+                                                        (c2b *)out, (const c2b *)y, open_bc);
       return;
     }
   }
@@ -847,9 +852,15 @@ void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void
   const int stage = (blocks && nrhs >= 4) ? 1 : 0;
   const size_t lds = stage ? sizeof(C) * 36 * (size_t)stage_sites(kSiteThreads, nrhs) : 0;
   if constexpr (sizeof(R) == 4) {
-    if (half) {
+    if (half == kS16Half) {
       k_cg_update_xp<R, c2h><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (c2h *)p, (C *)x,
                                                            (const C *)blocks, diag, (c2h *)t, open_bc, halt,
+                                                           stage);
+      return;
+    }
+    if (half == kS16Bf16) {
+      k_cg_update_xp<R, c2b><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (c2b *)p, (C *)x,
+                                                           (const C *)blocks, diag, (c2b *)t, open_bc, halt,
                                                            stage);
       return;
     }
@@ -916,15 +927,19 @@ __global__ void k_d2f_scaled(long n, int N, const double *f, const double2 *src,
 void convert_d2f_scaled(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s) {
   if (n > 0) k_d2f_scaled<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (const double2 *)src, (float2 *)dst);
 }
-__global__ void k_scale_half(long n, int N, const double *f, c2h *x) {
+template <typename S> __global__ void k_scale_half(long n, int N, const double *f, S *x) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float a = (float)f[i % N];
   const float2 v = widen(x[i]);
-  x[i] = narrow<c2h>(make_float2(a * v.x, a * v.y));
+  x[i] = narrow<S>(make_float2(a * v.x, a * v.y));
 }
-void scale_rhs_half(long n, int nrhs, const double *f, void *x, hipStream_t s) {
-  if (n > 0) k_scale_half<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (c2h *)x);
+void scale_rhs_half(long n, int nrhs, const double *f, void *x, int half, hipStream_t s) {
+  if (n <= 0) return;
+  if (half == kS16Bf16)
+    k_scale_half<c2b><<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (c2b *)x);
+  else
+    k_scale_half<c2h><<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (c2h *)x);
 }
 __global__ void k_acc_f2d_scaled(long n, int N, const double *f, const float2 *lo, double2 *hi) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
@@ -936,12 +951,16 @@ __global__ void k_acc_f2d_scaled(long n, int N, const double *f, const float2 *l
 void accumulate_f2d_scaled(long n, int nrhs, const double *f, const void *lo, void *hi, hipStream_t s) {
   if (n > 0) k_acc_f2d_scaled<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (const float2 *)lo, (double2 *)hi);
 }
-__global__ void k_f2h(long n, const float2 *src, c2h *dst) {
+template <typename S> __global__ void k_f2h(long n, const float2 *src, S *dst) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
-  if (i < n) dst[i] = narrow<c2h>(src[i]);
+  if (i < n) dst[i] = narrow<S>(src[i]);
 }
-void convert_f2h(long n, const void *src, void *dst, hipStream_t s) {
-  if (n > 0) k_f2h<<<elem_blocks(n), 256, 0, s>>>(n, (const float2 *)src, (c2h *)dst);
+void convert_f2h(long n, const void *src, void *dst, int half, hipStream_t s) {
+  if (n <= 0) return;
+  if (half == kS16Bf16)
+    k_f2h<c2b><<<elem_blocks(n), 256, 0, s>>>(n, (const float2 *)src, (c2b *)dst);
+  else
+    k_f2h<c2h><<<elem_blocks(n), 256, 0, s>>>(n, (const float2 *)src, (c2h *)dst);
 }
 
 }  // namespace wck

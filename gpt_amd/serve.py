@@ -39,15 +39,18 @@ from .core import lattice
 class batch:
     """One wc_batch: nslots slots of the mixed-precision eo2_ne solve on the
     operator pair (op FP64, inner in the inner precision).  inner_storage
-    "half" (the default for an FP32 inner operator) keeps the inner search
-    direction and Schur intermediates in 16-bit storage."""
+    "half" (the default for an FP32 inner operator; IEEE FP16) or "bf16"
+    keeps the inner search direction and Schur intermediates in 16-bit
+    storage."""
 
     def __init__(self, op, inner, nslots, parity=core.odd, delta=0.01, inner_storage="auto"):
         self.op, self.inner = op, inner
         self.nslots = int(nslots)
         if inner_storage == "auto":
             inner_storage = "half" if inner.precision is core.single else None
-        code = _lib.WC_STORAGE_HALF if inner_storage == "half" else _lib.WC_STORAGE_NATIVE
+        if inner_storage not in _lib.STORAGE_CODES:
+            raise ValueError("inner_storage must be 'auto', None, 'half' or 'bf16'")
+        code = _lib.STORAGE_CODES[inner_storage]
         h = ctypes.c_void_p()
         _lib.check(_lib.load().wc_batch_create(op.handle, inner.handle, parity.tag, self.nslots, float(delta), code,
                                                core.stream_ptr(), ctypes.byref(h)))

@@ -174,14 +174,16 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
  * satisfies |src - M dst| <= tol |src| (checked at reliable updates).
  * inner_storage WC_STORAGE_HALF (FP32 inner operator only) stores the
  * search direction and the Schur intermediates of the inner recursion as
- * 16-bit floats (FP32 arithmetic), each right-hand side rescaled to unit RMS
- * at every reliable update (delta is then at least 1e-2: 16-bit storage
- * resolves about 5e-4 of a field's scale); WC_STORAGE_NATIVE keeps the inner
- * precision.
+ * IEEE 16-bit floats (FP32 arithmetic), each right-hand side rescaled to
+ * unit RMS at every reliable update (delta is then at least 1e-2: FP16
+ * resolves about 5e-4 of a field's scale); WC_STORAGE_BF16 stores them as
+ * bfloat16 the same way (delta at least 1e-1: BF16 resolves about 4e-3);
+ * WC_STORAGE_NATIVE keeps the inner precision.
  * dst holds the initial guess on entry.  iterations / converged /
  * true_residual: nrhs entries (may be NULL); reliable_updates: one long. */
 #define WC_STORAGE_NATIVE 0
 #define WC_STORAGE_HALF 2
+#define WC_STORAGE_BF16 3
 int wc_solve_eo2_ne_mixed(wc_operator *op, wc_operator *inner, int parity, int nrhs, void *dst, const void *src,
                           double tol, long maxiter, double delta, int inner_storage, long *iterations,
                           int *converged, double *true_residual, long *reliable_updates, void *stream);

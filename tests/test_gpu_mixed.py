@@ -1,5 +1,5 @@
 """Mixed-precision reliable-update solve (FP32 inner CG, FP64 solution and
-true residuals), with the inner state in FP32 or in 16-bit storage: the true
+true residuals), with the inner state in FP32 or in 16-bit storage (FP16 or BF16): the true
 residual of the full system, recomputed here with the oracle's FP64
 operator, meets the requested tolerance; the solution agrees with the FP64
 CG one to 1e-8; a right-hand side gives the same result alone and inside a
@@ -41,7 +41,7 @@ def _dev(a, grid):
     return x
 
 
-STORAGE = [None, "half"]
+STORAGE = [None, "half", "bf16"]
 
 
 @pytest.mark.parametrize("storage", STORAGE)
@@ -85,7 +85,8 @@ def test_alone_equals_in_batch(setup, storage):
         assert np.linalg.norm(b - alone[i]) / np.linalg.norm(alone[i]) < 1e-8
 
 
-def test_half_storage_point_source_and_errors(setup):
+@pytest.mark.parametrize("storage", ["half", "bf16"])
+def test_half_storage_point_source_and_errors(setup, storage):
     """a point source (wide dynamic range across sites) converges with 16-bit
     inner storage; 16-bit storage needs the FP32 inner operator"""
     fd, U, srcs = setup
@@ -93,9 +94,11 @@ def test_half_storage_point_source_and_errors(setup):
     w = g.qcd.fermion.wilson_clover(U, P)
     pt = np.zeros_like(srcs[0])
     pt[0, 0, 0, 0, 0, 0] = 1.0
-    mcg = inv.cg_mixed(eps=1e-10, maxiter=5000, inner_storage="half")
+    mcg = inv.cg_mixed(eps=1e-10, maxiter=5000, inner_storage=storage)
     x = g(w.propagator(inv.preconditioned(pc.eo2_ne(), mcg)) * _dev(pt, w.F_grid)).to_numpy()
     assert mcg.converged
     assert np.linalg.norm(ref.M(x) - pt) / np.linalg.norm(pt) <= 1.05e-10
     with pytest.raises(ValueError):
-        inv.cg_mixed(eps=1e-10, inner_precision=g.double, inner_storage="half")
+        inv.cg_mixed(eps=1e-10, inner_precision=g.double, inner_storage=storage)
+    with pytest.raises(ValueError):
+        inv.cg_mixed(eps=1e-10, inner_storage="fp8")

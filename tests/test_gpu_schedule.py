@@ -75,7 +75,7 @@ def test_cg_bits_independent_of_schedule(setup):
             assert np.array_equal(a, b), chunk
 
 
-@pytest.mark.parametrize("storage", [None, "half"])
+@pytest.mark.parametrize("storage", [None, "half", "bf16"])
 def test_mixed_bits_independent_of_schedule(setup, storage):
     fd, U, srcs = setup
     out = {}

@@ -69,7 +69,7 @@ def solve_alone(Uh, mass, src_arr, storage="auto"):
     return x.to_numpy()
 
 
-@pytest.mark.parametrize("storage", [None, "half"])
+@pytest.mark.parametrize("storage", [None, "half", "bf16"])
 def test_batch_slots_and_mid_flight_admission(setup, storage):
     gr, U, srcs = setup
     mass = 0.1
