@@ -420,10 +420,16 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   // |src|^2 over both parities (a full field is two half fields back to back)
   wck::dot_partials_rhs<double>(2 * n, N, src, src, hi->partials, nullptr, s);
   wck::reduce_final_rhs(hi->partials, wck::kRedBlocks, N, sth + 8 * N, s);
-  // guess: y = L^-1 dst = A_D dst_D
+  // guess: y = L^-1 dst = A_D dst_D; with a zero guess (all right-hand
+  // sides) the first true residual is b_s itself
   site_op<double>(hi, N, pd, 0, half(dst, pd), xh, nullptr, s);
   if (lo_single) wck::zero<float>(n, xl, s);
   else wck::zero<double>(n, xl, s);
+  if ((rc = norm2_rhs<double>(hi, N, xh, sth + 14 * N, s))) return rc;
+  WC_HIP(hipMemcpyAsync(hsh + 14 * N, sth + 14 * N, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  bool guess_zero = true;
+  for (int i = 0; i < N; ++i) guess_zero = guess_zero && hsh[14 * N + i] == 0.0;
   WC_LAUNCHED();
 
   // ratio: |r_s|^2 / |r_h|^2 at the last reliable update, to predict from the
@@ -458,22 +464,30 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   // true = |r_s|^2, r_h = Mpc^dag r_s, r_inner = s r_h, c = s^2 |r_h|^2
   auto reliable = [&](long k, bool first) -> int {
     int rc2;
-    if (h16) wck::accumulate_f2d_scaled(n, N, sth + 13 * N, xl, xh, s);
-    else if (lo_single) wck::accumulate_f2d(n, xl, xh, s);
-    else wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
-    if (lo_single) wck::zero<float>(n, xl, s);
-    else wck::zero<double>(n, xl, s);
-    if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s))) return rc2;
-    wck::axpy<double>(n, -1.0, 0.0, mp, bs, rs, s);
-    if ((rc2 = norm2_rhs<double>(hi, N, rs, sth + 9 * N, s))) return rc2;
     if (lo_single) {
-      // Mpc^dag r_s only restarts the inner recursion, which runs in FP32:
-      // it is applied by the inner operator (into ml, free between iterations);
-      // r_s, its norm and the stopping test stay FP64
-      wck::convert_d2f(n, rs, rl, s);
+      // one pass folds x_inner into y and applies A_D^-1 to y (the first
+      // piece of Mpc y); r_s = b_s - Mpc y goes straight to FP32 (rl) with
+      // its FP64 norm.  Mpc^dag r_s only restarts the inner recursion, which
+      // runs in FP32: it is applied by the inner operator (into ml, free
+      // between iterations); the true residual and the stopping test stay FP64
+      if (first && guess_zero) {
+        wck::resid_d2f(n, N, bs, nullptr, rl, hi->partials, s);
+      } else {
+        void *aD = hi->tmp[0];
+        wck::fold_site_clover(hi->g, N, pd, hi->has_clover ? hi->Ai[pd] : nullptr, 1.0 / hi->diag,
+                              h16 ? sth + 13 * N : nullptr, xl, xh, aD, hi->open_bc ? 1 : 0, s);
+        if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s, aD))) return rc2;
+        wck::resid_d2f(n, N, bs, mp, rl, hi->partials, s);
+      }
+      wck::reduce_final_rhs(hi->partials, wck::kRedBlocks, N, sth + 9 * N, s);
       if ((rc2 = mpc_impl<float>(lo, N, pd, 1, ml, rl, nullptr, s))) return rc2;
       if ((rc2 = norm2_rhs<float>(lo, N, ml, sth + 10 * N, s))) return rc2;
     } else {
+      wck::axpy<double>(n, 1.0, 0.0, xl, xh, xh, s);
+      wck::zero<double>(n, xl, s);
+      if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s))) return rc2;
+      wck::axpy<double>(n, -1.0, 0.0, mp, bs, rs, s);
+      if ((rc2 = norm2_rhs<double>(hi, N, rs, sth + 9 * N, s))) return rc2;
       if ((rc2 = mpc_impl<double>(hi, N, pd, 1, rh, rs, nullptr, s))) return rc2;
       if ((rc2 = norm2_rhs<double>(hi, N, rh, sth + 10 * N, s))) return rc2;
     }
@@ -507,8 +521,7 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
     }
     if (h16) {
       WC_HIP(hipMemcpyAsync(sth + 11 * N, hsh + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
-      WC_HIP(hipMemcpyAsync(rl, ml, half_bytes(lo) * N, hipMemcpyDeviceToDevice, s));
-      wck::scale_rhs<float>(n, N, sth + 11 * N, rl, s);
+      wck::scale_copy_rhs(n, N, sth + 11 * N, ml, rl, s);
       if (!first) {
         wck::scale_rhs_half(n, N, sth + 12 * N, pl, h16, s);
         wck::scale_rhs_half(n, N, sth + 12 * N, aDl, h16, s);

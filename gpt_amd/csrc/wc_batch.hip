@@ -129,22 +129,27 @@ static int batch_reliable(wc_batch *b) {
   hipStream_t s = b->s;
   int rc;
   const bool single = lo->prec == WC_SINGLE;
-  if (b->h16) wck::accumulate_f2d_scaled(n, N, b->st + 13 * N, b->xl, b->xh, s);
-  else if (single) wck::accumulate_f2d(n, b->xl, b->xh, s);
-  else wck::axpy<double>(n, 1.0, 0.0, b->xl, b->xh, b->xh, s);
-  if (single) wck::zero<float>(n, b->xl, s);
-  else wck::zero<double>(n, b->xl, s);
-  if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, nullptr, b->partials))) return rc;
-  wck::axpy<double>(n, -1.0, 0.0, b->mp, b->bs, b->rs, s);
-  wck::dot_partials_rhs<double>(n, N, b->rs, b->rs, b->partials, nullptr, s);
-  wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
   if (single) {
+    // one pass folds x_inner into y and applies A_D^-1 (the first piece of
+    // Mpc y); r_s = b_s - Mpc y goes straight to FP32 with its FP64 norm.
     // Mpc^dag r_s only restarts the inner recursion, which runs in FP32: it is
     // applied by the inner operator (into ml, free between iterations)
-    wck::convert_d2f(n, b->rs, b->rl, s);
+    if ((rc = ensure_tmp(hi, 3, N))) return rc;
+    void *aD = hi->tmp[0];
+    wck::fold_site_clover(hi->g, N, b->pd, hi->has_clover ? hi->Ai[b->pd] : nullptr, 1.0 / hi->diag,
+                          b->h16 ? b->st + 13 * N : nullptr, b->xl, b->xh, aD, hi->open_bc ? 1 : 0, s);
+    if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, aD, b->partials))) return rc;
+    wck::resid_d2f(n, N, b->bs, b->mp, b->rl, b->partials, s);
+    wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
     if ((rc = mpc_impl<float>(lo, N, b->pd, 1, b->ml, b->rl, nullptr, s, nullptr, b->partials))) return rc;
     wck::dot_partials_rhs<float>(n, N, b->ml, b->ml, b->partials, nullptr, s);
   } else {
+    wck::axpy<double>(n, 1.0, 0.0, b->xl, b->xh, b->xh, s);
+    wck::zero<double>(n, b->xl, s);
+    if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, nullptr, b->partials))) return rc;
+    wck::axpy<double>(n, -1.0, 0.0, b->mp, b->bs, b->rs, s);
+    wck::dot_partials_rhs<double>(n, N, b->rs, b->rs, b->partials, nullptr, s);
+    wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
     if ((rc = mpc_impl<double>(hi, N, b->pd, 1, b->rh, b->rs, nullptr, s, nullptr, b->partials))) return rc;
     wck::dot_partials_rhs<double>(n, N, b->rh, b->rh, b->partials, nullptr, s);
   }
@@ -188,8 +193,7 @@ static int batch_reliable(wc_batch *b) {
   }
   if (b->h16) {
     WC_HIP(hipMemcpyAsync(b->st + 11 * N, b->host + 11 * N, sizeof(double) * 3 * N, hipMemcpyHostToDevice, s));
-    WC_HIP(hipMemcpyAsync(b->rl, b->ml, hbytes(lo) * N, hipMemcpyDeviceToDevice, s));
-    wck::scale_rhs<float>(n, N, b->st + 11 * N, b->rl, s);
+    wck::scale_copy_rhs(n, N, b->st + 11 * N, b->ml, b->rl, s);
     wck::scale_rhs_half(n, N, b->st + 12 * N, b->pl, b->h16, s);
     wck::scale_rhs_half(n, N, b->st + 12 * N, b->aD, b->h16, s);
   } else if (single) {

@@ -95,6 +95,16 @@ template <typename C> __device__ __forceinline__ C cconj(C a) { return cmk<C>(a.
 // i * a
 template <typename C> __device__ __forceinline__ C cimul(C a) { return cmk<C>(-a.y, a.x); }
 
+// A c2h value as a packed pair of halves: the 16-bit hop projects its FP16
+// neighbour spinors in FP16 (packed adds; the per-RHS frames keep the values
+// near 1, far from FP16's range limits) and widens the half spinors, so a
+// direction converts 6 complex values instead of 12.
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ h2v cadd(h2v a, h2v b) { return a + b; }
+__device__ __forceinline__ h2v csub(h2v a, h2v b) { return a - b; }
+__device__ __forceinline__ h2v cimul(h2v a) { return __builtin_shufflevector(a, -a, 3, 0); }
+__device__ __forceinline__ float2 widen(h2v v) { return make_float2((float)v.x, (float)v.y); }
+
 // Lattice geometry shared by all kernels.
 struct Geom {
   int L[4];   // Lx, Ly, Lz, Lt

@@ -90,11 +90,11 @@ __device__ __forceinline__ void prepare_link(const HopLaunch &a, int mu, int fb,
 }
 
 // direction (MU, FB): the link is prepared (unless PRE: staged prepared),
-// then every spinor is projected, multiplied and accumulated
-template <typename R, int NL, int DAG, int MU, int FB, int RB, bool PRE>
+// then every spinor is projected (in P: the working precision, or packed
+// FP16 for c2h storage), widened, multiplied and accumulated
+template <typename R, int NL, int DAG, int MU, int FB, int RB, bool PRE, typename P>
 __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], typename c2<R>::t u[9],
-                                            const typename c2<R>::t psi[RB][12],
-                                            typename c2<R>::t acc[RB][12]) {
+                                            const P psi[RB][12], typename c2<R>::t acc[RB][12]) {
   using C = typename c2<R>::t;
   constexpr int S = ((FB == 0) != (DAG != 0)) ? 1 : -1;
   if constexpr (!PRE) prepare_link<R, NL>(a, MU, FB, c, u);
@@ -102,9 +102,12 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], 
   for (int j = 0; j < RB; ++j) {
     C hs[2][3];
 #pragma unroll
-    for (int col = 0; col < 3; ++col)
-      project<MU, S>(psi[j][0 + col], psi[j][3 + col], psi[j][6 + col], psi[j][9 + col], hs[0][col],
-                     hs[1][col]);
+    for (int col = 0; col < 3; ++col) {
+      P h0, h1;
+      project<MU, S>(psi[j][0 + col], psi[j][3 + col], psi[j][6 + col], psi[j][9 + col], h0, h1);
+      hs[0][col] = widen(h0);
+      hs[1][col] = widen(h1);
+    }
     C ch[2][3];
 #pragma unroll
     for (int s = 0; s < 2; ++s)
@@ -143,6 +146,8 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
   // dynamically indexed (scratch) array
   const int c[4] = {cx, cy, cz, ct};
   using C = typename c2<R>::t;
+  // the projection's operand type: packed halves for FP16 storage
+  using P = typename std::conditional<std::is_same<SI, c2h>::value, h2v, C>::type;
   const Geom &g = a.g;
   const uint32_t N = (uint32_t)a.nrhs;
   const uint32_t hh = (uint32_t)h;
@@ -184,12 +189,18 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
         u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
     }
     const uint32_t vs = (uint32_t)aidx_site(nb, 12) * N;
-    C psi[RB][12];
+    P psi[RB][12];
 #pragma unroll
     for (int j = 0; j < RB; ++j) {
       const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(SI);
 #pragma unroll
-      for (int k = 0; k < 12; ++k) psi[j][k] = widen(bload<SI, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride));
+      for (int k = 0; k < 12; ++k) {
+        const SI v = bload<SI, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride);
+        if constexpr (std::is_same<SI, c2h>::value)
+          psi[j][k] = __builtin_bit_cast(h2v, v);
+        else
+          psi[j][k] = widen(v);
+      }
     }
     switch (dir) {
       case 0: compute_dir<R, NL, DAG, 0, 0, RB, LS>(a, c, u, psi, acc); break;

@@ -206,4 +206,14 @@ void scale_rhs_half(long n, int nrhs, const double *f, void *x, int half, hipStr
 void accumulate_f2d_scaled(long n, int nrhs, const double *f, const void *lo, void *hi, hipStream_t s);
 void convert_f2h(long n, const void *src, void *dst, int half, hipStream_t s);
 
+// fused passes of the mixed solve's reliable update (FP64 hi, FP32 lo):
+// rl = (float) (bs - mp) with per-RHS partials of |bs - mp|^2 for
+// reduce_final_rhs (mp null: bs); xh += f[r] xl (f null: 1), xl = 0,
+// out = A xh on parity p (blocks FP64 or null for diag; open boundary zero);
+// dst = (float) f[r] src.
+void resid_d2f(long n, int nrhs, const void *bs, const void *mp, void *rl, double *partials, hipStream_t s);
+void fold_site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const double *f, void *xl,
+                      void *xh, void *out, int open_bc, hipStream_t s);
+void scale_copy_rhs(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s);
+
 }  // namespace wck

@@ -951,6 +951,88 @@ __global__ void k_acc_f2d_scaled(long n, int N, const double *f, const float2 *l
 void accumulate_f2d_scaled(long n, int nrhs, const double *f, const void *lo, void *hi, hipStream_t s) {
   if (n > 0) k_acc_f2d_scaled<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (const float2 *)lo, (double2 *)hi);
 }
+// ------------------------------------------- fused reliable-update passes
+// rl = (float) (bs - mp) (mp null: bs) and per-RHS partials of |bs - mp|^2,
+// summed as k_dot_rhs sums |x|^2 (same grid, order and roundings)
+__global__ __launch_bounds__(256) void k_resid_d2f(long n, int N, const double2 *bs, const double2 *mp,
+                                                   float2 *rl, double *pre) {
+  __shared__ double lr[256];
+  const long stride = (long)gridDim.x * blockDim.x;
+  double sr = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double2 v = bs[i];
+    if (mp) {
+      const double2 m = mp[i];
+      v = make_double2(v.x - m.x, v.y - m.y);
+    }
+    rl[i] = make_float2((float)v.x, (float)v.y);
+    sr += v.x * v.x + v.y * v.y;
+  }
+  lr[threadIdx.x] = sr;
+  __syncthreads();
+  if ((int)threadIdx.x < N) pre[(long)threadIdx.x * gridDim.x + blockIdx.x] = seg_sum(lr, blockDim.x, N);
+}
+void resid_d2f(long n, int nrhs, const void *bs, const void *mp, void *rl, double *partials, hipStream_t s) {
+  k_resid_d2f<<<kRedBlocks, rhs_block_threads(nrhs), 0, s>>>(n, nrhs, (const double2 *)bs, (const double2 *)mp,
+                                                             (float2 *)rl, partials);
+}
+
+// xh += f[r] xl (f null: 1); xl = 0; out = A xh on parity p (A: FP64 blocks,
+// or diag when null; zero on an open boundary)
+__global__ __launch_bounds__(kSiteThreads) void k_fold_site_clover(Geom g, int N, int p, const double2 *A,
+                                                                  double diag, const double *f, float2 *xl,
+                                                                  double2 *xh, double2 *out, int open_bc) {
+  const long t = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
+  const long h = (N == 1) ? t : t / N;
+  const int r = (int)(t - h * N);
+  if (h >= g.Vh) return;
+  const double a = f ? f[r] : 1.0;
+  double2 v[12], o[12];
+  float2 l[12];
+  load12(xh, h, r, N, v);
+  load12(xl, h, r, N, l);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    v[k] = make_double2(fma((double)l[k].x, a, v[k].x), fma((double)l[k].y, a, v[k].y));
+    l[k] = make_float2(0.0f, 0.0f);
+  }
+  store12(xh, h, r, N, v);
+  store12(xl, h, r, N, l);
+  bool zb = false;
+  if (open_bc) {
+    int c[4];
+    half_coords(g, h, p, c);
+    zb = (c[3] == 0 || c[3] == g.L[3] - 1);
+  }
+  if (zb) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) o[k] = make_double2(0.0, 0.0);
+  } else if (A) {
+    clov_apply<double>(A, h, v, o);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) o[k] = cscale(v[k], diag);
+  }
+  store12(out, h, r, N, o);
+}
+void fold_site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const double *f, void *xl,
+                      void *xh, void *out, int open_bc, hipStream_t s) {
+  k_fold_site_clover<<<site_blocks(g.Vh * nrhs), kSiteThreads, 0, s>>>(
+      g, nrhs, p, (const double2 *)blocks, diag, f, (float2 *)xl, (double2 *)xh, (double2 *)out, open_bc);
+}
+
+// dst = (float) f[r] * src (FP32 fields)
+__global__ void k_scale_copy(long n, int N, const double *f, const float2 *src, float2 *dst) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float a = (float)f[i % N];
+  const float2 v = src[i];
+  dst[i] = make_float2(a * v.x, a * v.y);
+}
+void scale_copy_rhs(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s) {
+  if (n > 0) k_scale_copy<<<elem_blocks(n), 256, 0, s>>>(n, nrhs, f, (const float2 *)src, (float2 *)dst);
+}
+
 template <typename S> __global__ void k_f2h(long n, const float2 *src, S *dst) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i < n) dst[i] = narrow<S>(src[i]);
