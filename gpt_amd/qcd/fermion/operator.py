@@ -41,6 +41,13 @@ def _phases(bp):
     return out
 
 
+def _iteration_runs(handle):
+    """the C operator's cumulative (graph, direct) iteration-run counters"""
+    g_, d_ = ctypes.c_long(), ctypes.c_long()
+    _lib.check(_lib.load().wc_iteration_runs(handle, ctypes.addressof(g_), ctypes.addressof(d_)))
+    return int(g_.value), int(d_.value)
+
+
 class interface:
     """Owns one wc_operator handle (lib/gpt/qcd/fermion/operator/interface.py:29-103)."""
 
@@ -80,6 +87,8 @@ class interface:
             _lib.check(lib.wc_create(ctypes.byref(p), Ut.data_ptr(), stream, ctypes.byref(h)))
             self.obj = h.value
             operator_tag[self.obj] = tag
+        # a pooled handle keeps its run counters: count from here
+        self._runs0 = _iteration_runs(self.obj)
         core.torch.cuda.current_stream().synchronize()
 
     def setup(self, grid, params, U):
@@ -241,10 +250,10 @@ class fine_operator(matrix_operator):
     def iteration_runs(self):
         """(runs replayed from graphs, runs of one iteration) of this
         operator's CG solves, and of the mixed solves it is the inner operator
-        of"""
-        g_, d_ = ctypes.c_long(), ctypes.c_long()
-        _lib.check(_lib.load().wc_iteration_runs(self.handle, ctypes.addressof(g_), ctypes.addressof(d_)))
-        return int(g_.value), int(d_.value)
+        of, since it was set up"""
+        g_, d_ = _iteration_runs(self.handle)
+        g0, d0 = self.interface._runs0
+        return g_ - g0, d_ - d0
 
     def kappa(self):
         p = self.params
