@@ -264,8 +264,9 @@ def main():
     copy_gbs = copy_bandwidth(torch, dev) if not args.no_dhop else 0.0
     log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
 
-    # serving runs first after the Dhop section: a serving process starts from a clean device (after the
-    # 32^3x64 multi-RHS section its Poisson p99 was seen to rise several-fold)
+    # serving runs first after the Dhop section, as a serving process starts on a settled device: right
+    # after the multi-RHS section has freed ~200 GB, a hipMalloc of a new serving group can block the
+    # device for ~2 s (HIP API trace), which lands in the Poisson p99; 15 s later it does not
     serve_res = None
     if args.serve > 0:
         serve_res = serve_measure(torch, g, dev, args.serve)
@@ -646,7 +647,8 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
     pois["arrival_rate_per_s"] = lam
     log(f"bench: serving poisson {pois}")
     out = {"nslots": nslots, "streams": len(srv.streams), "load": load, "capacity": cap, "poisson": pois,
-           "peak_groups": srv.peak_groups}
+           "peak_groups": srv.peak_groups, "hbm_budget_gb": srv.hbm_budget / 2**30,
+           "scheduler_passes_waiting_for_hbm": srv.waited_for_memory}
     srv.close()
     return out
 
