@@ -202,6 +202,34 @@ def cpu_baseline(budget_s=12.0):
                       f"Grid's CPU path; {dt * 1e3:.2f} ms per call"}
 
 
+def slab_share(torch, g, dev, fd, args):
+    """The FP64 18-real hop on one GPU's share of a T-slab decomposition over 8
+    GPUs (Lx x Ly x Lz x Lt/8, periodic in t standing in for the halo), and the
+    bytes each hop would exchange: per t-face one parity of projected half
+    spinors, 6 complex per face site"""
+    sd = [fd[0], fd[1], fd[2], fd[3] // 8]
+    Vh = int(np.prod(sd)) // 2
+    Us = smooth_links_device(torch, sd, 0.3, 2001, dev)
+    w = g.qcd.fermion.wilson_clover(Us, mass=0.08, csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0,
+                                    boundary_phases=[1, 1, 1, -1])
+    pairs = []
+    for k in range(args.pairs):
+        src = gaussian_half(torch, g, w.F_grid_eo, g.odd, 2002 + k, dev)
+        d = g.lattice(w.F_grid_eo)
+        d.checkerboard(g.even)
+        pairs.append((src, d))
+    shape, _ = w.tune_hop(1, reps=3)
+    t_ev, _ = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
+    face_sites = fd[0] * fd[1] * fd[2] // 2
+    out = {"local_lattice": sd, "us_per_call": round(t_ev * 1e6, 2), "GBps": round(1536 * Vh / t_ev / 1e9, 2),
+           "halo_bytes_per_hop": 2 * face_sites * 6 * 16,
+           "halo_us_at_153GBps_per_link": round(face_sites * 6 * 16 / 153e9 * 1e6, 2),
+           "shape": list(shape)}
+    del w, pairs, Us
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -260,6 +288,9 @@ def main():
         log(f"bench: {name}: {res[name]}")
         del w, pairs
         torch.cuda.empty_cache()
+    if "fp64_18" in res and fd[3] % 8 == 0:
+        res["fp64_18_slab_share"] = slab_share(torch, g, dev, fd, args)
+        log(f"bench: fp64_18_slab_share: {res['fp64_18_slab_share']}")
 
     copy_gbs = copy_bandwidth(torch, dev) if not args.no_dhop else 0.0
     log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
