@@ -233,7 +233,7 @@ def slab_share(torch, g, dev, fd, args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; use --gpus 1"}))
         sys.exit(2)
     import torch
 

@@ -1,4 +1,4 @@
-"""Configuration IO on the GPU box.  NERSC: a save / load round trip is
+"""Configuration IO with the GPU computing the checks.  NERSC: a save / load round trip is
 exact, the GPU plaquette reproduces the reference's known answers
 (tests/random/simple.py:14-31) and the oracle's plaquette and link trace,
 a two-row IEEE32 file written to the format is read with the third row
