@@ -1,6 +1,6 @@
 // C ABI of the MI355X even-odd Wilson-clover operator (declared in
 // include/gpt_amd_wc.h).  Host-side orchestration: operator state, opcode
-// dispatch, the Schur-complement compositions and the CG driver.
+// dispatch, the Schur-complement compositions and the CG solve loops.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
 the ``cpu_baseline`` leg of ``bench.py`` may import this package, and only as
-the checker (or as the timed CPU stand-in for the reference's Grid CPU path).
+the reference the GPU results are compared with (or as the timed CPU stand-in
+for the reference's Grid CPU path).
 The product (``gpt_amd``) never imports it and has no CPU fallback.
 
 Contents, each restating a reference file (paths relative to the reference
