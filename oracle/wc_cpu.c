@@ -1,7 +1,8 @@
 /*
- * ORACLE / CPU BASELINE ONLY.  Not part of the product: tests use it as a
- * fast checker for solves, and bench.py times it on the host cores as the
- * stand-in for gpt's Grid CPU path (Grid cannot be built here).
+ * ORACLE / CPU BASELINE ONLY.  Not part of the product: tests compare the
+ * GPU solves with it where the Python restatement would be slow, and bench.py
+ * times it on the host cores as the stand-in for gpt's Grid CPU path (Grid
+ * is not available to build).
  *
  * C/OpenMP restatement of the even-odd Wilson-clover operator and CG:
  *   hop   : lib/gpt/qcd/fermion/reference/wilson_clover.py:182-200
