@@ -102,3 +102,28 @@ def test_half_storage_point_source_and_errors(setup, storage):
         inv.cg_mixed(eps=1e-10, inner_precision=g.double, inner_storage=storage)
     with pytest.raises(ValueError):
         inv.cg_mixed(eps=1e-10, inner_storage="fp8")
+
+
+def test_16bit_inner_operator_tracks_fp32(setup):
+    """the 16-bit inner state (FP16 or BF16 storage, FP16 spin projection)
+    changes the inner operator only at its rounding level: the solve needs
+    about the iterations of FP32 storage, and every reliable update finds the
+    true residual close to what the recursion predicted (a wrong 16-bit hop or
+    update would show as extra iterations or updates)"""
+    fd, U, srcs = setup
+    w = g.qcd.fermion.wilson_clover(U, P)
+    src = _dev(srcs[1], w.F_grid)
+    runs = {}
+    for storage in (None, "half", "bf16"):
+        m = inv.cg_mixed(eps=1e-10, maxiter=5000, inner_storage=storage)
+        g(w.propagator(inv.preconditioned(pc.eo2_ne(), m)) * src)
+        assert m.converged
+        runs[storage] = (m.iterations[0], m.reliable_updates)
+        print(f"storage {storage}: iterations {m.iterations[0]}, reliable updates {m.reliable_updates}", flush=True)
+    # 8^3x16, m 0.05: FP32 72 iterations / 6 updates, FP16 73 / 6, BF16 73 / 11
+    # (BF16 updates at least every 0.1 of residual reduction, the others every 0.01)
+    it32, up32 = runs[None]
+    for storage, max_up in (("half", up32 + 2), ("bf16", 2 * up32 + 2)):
+        it, up = runs[storage]
+        assert it <= it32 + 3, (storage, it, it32)
+        assert up <= max_up, (storage, up, up32)
