@@ -464,11 +464,36 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
                     "true_residual": max(res), "rel_error_vs_fp64_solution": max(err),
                     "s_per_batch": th, "solves_per_s": float(nr / th)}
         log(f"bench: cg {key} {out[key]}")
+    # (e) FP16 storage on 48 right-hand sides at once (the 12 above and 36 more from the same generator)
+    n48 = 48
+    extra = []
+    for _ in range(n48 - nrhs):
+        x = g.lattice(w.F_grid)
+        x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
+        extra.append(x)
+    w48 = g.qcd.fermion.wilson_clover(U, prm, n_rhs=n48)
+    S48 = pack(srcs + extra)
+    m48 = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half")
+    slv48 = w48.propagator(inv.preconditioned(pc.eo2_ne(), m48))
+    g(slv48 * S48)
+    lat = []
+    for _ in range(2):
+        X, t = timed(lambda: g(slv48 * S48))
+        lat.append(t)
+    res = true_residual(g, w48, X, S48)
+    t48 = float(np.median(lat))
+    out["mixed_multi48_half"] = {"nrhs": n48, "iterations_max": max(m48.iterations),
+                                 "reliable_updates": m48.reliable_updates, "max_true_residual": max(res),
+                                 "s_per_batch": t48, "solves_per_s": float(n48 / t48)}
+    log(f"bench: cg mixed_multi48_half {out['mixed_multi48_half']}")
+    del X, S48, slv48, w48, extra
     out["speedup_mixed_multi_vs_fp64"] = out["mixed_multi"]["solves_per_s"] / out["fp64_cg"]["solves_per_s"]
     out["speedup_mixed_multi_half_vs_fp64"] = (out["mixed_multi_half"]["solves_per_s"]
                                                / out["fp64_cg"]["solves_per_s"])
     out["speedup_mixed_multi_bf16_vs_fp64"] = (out["mixed_multi_bf16"]["solves_per_s"]
                                                / out["fp64_cg"]["solves_per_s"])
+    out["speedup_mixed_multi48_half_vs_fp64"] = (out["mixed_multi48_half"]["solves_per_s"]
+                                                 / out["fp64_cg"]["solves_per_s"])
     return out
 
 
