@@ -8,7 +8,9 @@ Checks, with the oracle's FP64 operator as the referee:
   (both are solves to tol = 1e-10; the reliable-update points depend on the
   batch, so the solutions agree to within a few condition numbers times tol);
 * the server serves requests for two resident fields and several masses,
-  with an HBM budget small enough that requests must wait for memory.
+  with an HBM budget small enough that requests must wait for memory;
+* a request served alone and among requests of other groups gives bitwise
+  the same solutions.
 """
 import numpy as np
 import pytest
@@ -162,3 +164,32 @@ def test_server_mixed_fields_masses_and_memory_wait(setup):
             assert rel(x, solve_alone(Uh[r.field], r.mass, srcs[i])) < AGREE
     st = serve.latency_stats(done)
     assert st["p50_s"] <= st["p99_s"]
+
+
+def test_request_alone_equals_request_among_others(setup):
+    """determinism, alone vs mixed: a request served on its own and the same
+    request served while requests for another field and other masses run in
+    their own groups on other streams give bitwise the same solutions and
+    iteration counts (groups share no arithmetic; launch shapes and iteration
+    schedules do not change bits)"""
+    gr, U, srcs = setup
+    Uh = {"a": U[0], "b": U[1]}
+    grid = g.grid([8, 8, 8, 16], g.double)
+
+    def serve_plan(plan):
+        srv = serve.server(P, nslots=4, streams=3, iters_per_step=7)
+        for k, u in Uh.items():
+            srv.add_field(k, u)
+        reqs = [serve.request(f, m, [field(srcs[i], grid) for i in idx], tol=TOL, maxiter=10000)
+                for f, m, idx in plan]
+        srv.run([(0.001 * n, r) for n, r in enumerate(reqs)])
+        srv.close()
+        return reqs
+
+    alone = serve_plan([("a", 0.1, [0, 1])])[0]
+    mixed = serve_plan([("b", 0.15, [3, 4, 5]), ("a", 0.1, [0, 1]), ("a", 0.2, [6]), ("b", 0.05, [7])])
+    among = mixed[1]
+    assert all(r.converged for r in mixed) and all(alone.converged)
+    assert alone.iterations == among.iterations
+    for x, y in zip(alone.results, among.results):
+        assert np.array_equal(x.to_numpy(), y.to_numpy())
