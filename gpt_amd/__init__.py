@@ -18,7 +18,8 @@ import types as _types
 from . import _lib  # noqa: F401
 from .core import (  # noqa: F401
     single, double, even, odd, none, full, redblack, grid, lattice, vspincolor, copy, axpy, scale,
-    norm2, inner_product, axpy_norm2, convert, pick_checkerboard, set_checkerboard, expression,
+    norm2, inner_product, rank_inner_product, inner_product_norm2, axpy_norm2, convert, pick_checkerboard,
+    set_checkerboard, expression,
     eval, matrix_operator,
 )
 from . import qcd  # noqa: F401

@@ -356,13 +356,33 @@ def norm2(x):
     return r.value
 
 
-def inner_product(a, b):
-    """<a, b> = sum conj(a) b, FP64 accumulation (lib/gpt/core/transform.py:101-106)"""
+def inner_product(a, b, n_block=1, use_accelerator=True):
+    """<a, b> = sum conj(a) b, FP64 accumulation (lib/gpt/core/transform.py:101-106);
+    with a list on either side, the matrix [<a_i, b_j>] as gpt returns it
+    (n_block and use_accelerator are accepted for gpt's signature)"""
+    if isinstance(a, (list, tuple)) or isinstance(b, (list, tuple)):
+        la = [eval(x) if isinstance(x, expression) else x for x in (a if isinstance(a, (list, tuple)) else [a])]
+        lb = [eval(x) if isinstance(x, expression) else x for x in (b if isinstance(b, (list, tuple)) else [b])]
+        return np.array([[inner_product(x, y) for y in lb] for x in la], dtype=np.complex128)
+    if isinstance(a, expression):
+        a = eval(a)
+    if isinstance(b, expression):
+        b = eval(b)
     _same(a, b)
     r = (_ct.c_double * 2)()
     _lib.check(_lib.load().wc_inner_product(a.grid.precision.code, a.nentries, a.ptr(), b.ptr(), r,
                                             stream_ptr()))
     return complex(r[0], r[1])
+
+
+def rank_inner_product(a, b, n_block=1, use_accelerator=True):
+    """gpt's rank_inner_product: on one device the same as inner_product"""
+    return inner_product(a, b, n_block, use_accelerator)
+
+
+def inner_product_norm2(a, b):
+    """(<a, b>, |a|^2) (lib/gpt/core/transform.py:117-130)"""
+    return inner_product(a, b), norm2(a)
 
 
 def axpy_norm2(z, a, x, y):

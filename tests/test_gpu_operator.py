@@ -189,6 +189,13 @@ def test_blas_kernels():
     assert np.abs(P.to_numpy() - (a + 2.0 * b)).max() < 1e-13
     D = g(A - B)
     assert np.abs(D.to_numpy() - (a - b)).max() < 1e-13
+    # lists: the matrix [<a_i, b_j>] (transform.py:101-106), inner_product_norm2
+    M = g.inner_product([A, B], [B, A, D])
+    ref = np.array([[np.vdot(x, y) for y in (b, a, a - b)] for x in (a, b)])
+    assert M.shape == (2, 3) and np.abs(M - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.array_equal(g.rank_inner_product([A], [B]), g.inner_product([A], [B]))
+    ip2, n2a = g.inner_product_norm2(A, B)
+    assert ip2 == ip and n2a == g.norm2(A)
 
 
 def test_errors_raise():
