@@ -20,8 +20,9 @@ from .core import (  # noqa: F401
     single, double, even, odd, none, full, redblack, grid, lattice, vspincolor, copy, axpy, scale,
     norm2, inner_product, rank_inner_product, inner_product_norm2, axpy_norm2, convert, pick_checkerboard,
     set_checkerboard, expression,
-    eval, matrix_operator,
+    eval, matrix_operator, mspincolor, adj, trace, slice_ as slice,
 )
+from . import create  # noqa: F401
 from . import qcd  # noqa: F401
 from . import algorithms  # noqa: F401
 from .io import format, load, save  # noqa: F401

@@ -301,6 +301,155 @@ import ctypes as _ct  # noqa: E402
 ctypes_int4 = _ct.c_int * 4
 
 
+class mspincolor_lattice:
+    """gpt's mspincolor: a 12 x 12 spin-colour matrix per site (a propagator),
+    held in HBM in natural layout ``data[site][s1][s2][c1][c2]`` (site
+    ((t Lz + z) Ly + y) Lx + x).  It is a container: operators act on it column
+    by column, column (s, c) being the vspincolor right-hand side c + 3 s
+    (lib/gpt/core/object_type/container.py:330-357)."""
+
+    def __init__(self, first):
+        g = first.grid if isinstance(first, (lattice, mspincolor_lattice)) else first
+        if g.cb is not full or g.nrhs != 1:
+            raise ValueError("an mspincolor lives on a full single-RHS grid")
+        self.grid = g
+        self.data = torch.zeros((g.gsites, 4, 4, 3, 3), dtype=_complex_dtype(g.precision), device=device())
+
+    def checkerboard(self, cb=None):
+        if cb is None or cb is none:
+            return none
+        raise ValueError("full-grid lattice has no checkerboard")
+
+    # columns as vspincolor fields
+    def column(self, s, c):
+        g = self.grid
+        col = self.data[:, :, s, :, c].reshape(g.gsites, 12)
+        nat = torch.view_as_real(col.contiguous()).to(g.precision.real_dtype).contiguous()
+        out = lattice(g)
+        fd = (ctypes_int4)(*g.fdimensions)
+        _lib.check(_lib.load().wc_field_import(fd, g.precision.code, nat.data_ptr(), out.ptr(), stream_ptr()))
+        torch.cuda.current_stream().synchronize()
+        return out
+
+    def set_column(self, s, c, x):
+        g = self.grid
+        if x.grid.cb is not full or x.grid.gsites != g.gsites or x.grid.nrhs != 1:
+            raise ValueError("set_column needs a full single-RHS vspincolor of the same lattice")
+        nat = torch.empty((g.gsites, 12, 2), dtype=x.data.dtype, device=x.data.device)
+        fd = (ctypes_int4)(*g.fdimensions)
+        _lib.check(_lib.load().wc_field_export(fd, g.precision.code, x.ptr(), nat.data_ptr(), stream_ptr()))
+        self.data[:, :, s, :, c] = torch.view_as_complex(nat.contiguous()).reshape(g.gsites, 4, 3)
+        return self
+
+    def __setitem__(self, key, value):
+        """src[:] = 0, and src[(x, y, z, t)] = identity (gpt's create.point)"""
+        if isinstance(key, slice) and key == slice(None):
+            if value != 0:
+                raise NotImplementedError("only field[:] = 0 is supported")
+            self.data.zero_()
+            return
+        x, y, z, t = key
+        L = self.grid.fdimensions
+        site = ((t * L[2] + z) * L[1] + y) * L[0] + x
+        self.data[site] = torch.as_tensor(np.asarray(value), dtype=self.data.dtype, device=self.data.device)
+
+    def identity(self):
+        """the 12 x 12 unit matrix in [s1][s2][c1][c2] order"""
+        return np.einsum("ab,cd->abcd", np.eye(4), np.eye(3))
+
+    # host transfer: natural numpy [t, z, y, x, 4, 4, 3, 3]
+    def to_numpy(self):
+        return self.data.cpu().numpy().reshape(tuple(reversed(self.grid.fdimensions)) + (4, 4, 3, 3))
+
+    def from_numpy(self, arr):
+        arr = np.asarray(arr).reshape(self.grid.gsites, 4, 4, 3, 3)
+        self.data.copy_(torch.from_numpy(np.ascontiguousarray(arr)).to(self.data.dtype))
+        return self
+
+    def __imatmul__(self, expr):
+        eval(self, expr)
+        return self
+
+    def __add__(self, o):
+        return expression.of(self) + o
+
+    def __sub__(self, o):
+        return expression.of(self) - o
+
+    def __mul__(self, o):
+        if isinstance(o, _adjoint):
+            return _site_product(self, o.of, True)
+        return expression.of(self) * o
+
+    def __rmul__(self, o):
+        return expression.of(self) * o
+
+    def __neg__(self):
+        return expression.of(self) * -1.0
+
+
+def _complex_dtype(prec):
+    return torch.complex128 if prec is double else torch.complex64
+
+
+def mspincolor(g):
+    """g.mspincolor(grid): a zero propagator field"""
+    return mspincolor_lattice(g)
+
+
+class _adjoint:
+    def __init__(self, of):
+        self.of = of
+
+
+def adj(x):
+    """g.adj of an mspincolor, for site-local contractions such as
+    g.trace(prop * g.adj(prop))"""
+    if isinstance(x, mspincolor_lattice):
+        return _adjoint(x)
+    raise NotImplementedError("g.adj is provided for mspincolor fields (operators have .adj())")
+
+
+class _site_product:
+    """the site-wise matrix product A B^dag of two mspincolor fields (lazy)"""
+
+    def __init__(self, a, b, dagger):
+        self.a, self.b, self.dagger = a, b, dagger
+
+
+class site_scalar:
+    """a complex number per site, natural order, in HBM (the result of g.trace)"""
+
+    def __init__(self, grid, data):
+        self.grid, self.data = grid, data
+
+    def to_numpy(self):
+        return self.data.cpu().numpy().reshape(tuple(reversed(self.grid.fdimensions)))
+
+
+def trace(x):
+    """g.trace of an mspincolor (the 12 x 12 trace per site) or of a site
+    product A * g.adj(B): tr(A B^dag) = sum_ij A_ij conj(B_ij), elementwise"""
+    if isinstance(x, _site_product):
+        a, b = x.a.data, x.b.data
+        return site_scalar(x.a.grid, (a * b.conj()).reshape(a.shape[0], -1).sum(dim=1))
+    if isinstance(x, mspincolor_lattice):
+        d = x.data
+        return site_scalar(x.grid, torch.diagonal(torch.diagonal(d, dim1=1, dim2=2), dim1=1, dim2=2).sum(dim=(1, 2)))
+    raise NotImplementedError("g.trace of this object")
+
+
+def slice_(x, dim):
+    """g.slice(field, dim): the sum over the sites of each coordinate value of
+    dimension dim (0 x, 1 y, 2 z, 3 t), a list of complex numbers"""
+    if not isinstance(x, site_scalar):
+        raise NotImplementedError("g.slice is provided for g.trace results")
+    L = x.grid.fdimensions
+    v = x.data.reshape(L[3], L[2], L[1], L[0])
+    other = tuple(d for d in range(4) if d != 3 - dim)
+    return [complex(c) for c in v.sum(dim=other).cpu().numpy()]
+
+
 def vspincolor(g):
     return lattice(g)
 
@@ -348,6 +497,8 @@ def norm2(x):
         return [norm2(y) for y in x]
     if isinstance(x, expression):
         x = eval(x)
+    if isinstance(x, mspincolor_lattice):
+        return float(torch.view_as_real(x.data).to(torch.float64).square().sum())
     if x.nrhs > 1:
         return [norm2(x.rhs(r)) for r in range(x.nrhs)]
     r = _ct.c_double()
@@ -445,7 +596,7 @@ class expression:
     def of(x):
         if isinstance(x, expression):
             return x
-        if isinstance(x, lattice):
+        if isinstance(x, (lattice, mspincolor_lattice)):
             return expression([(1.0, [], x)])
         raise TypeError(f"cannot form an expression from {type(x)}")
 
@@ -491,6 +642,13 @@ def eval(first, second=None):
         return [eval(x) for x in expr] if dst is None else [eval(d, x) for d, x in zip(dst, expr)]
     expr = expression.of(expr)
     fields = [(c, _apply_ops(ops, f)) for c, ops, f in expr.terms]
+    if isinstance(fields[0][1], mspincolor_lattice):
+        # a propagator container: linear combinations on its natural layout
+        acc = sum(c * f.data for c, f in fields)
+        if dst is None:
+            dst = mspincolor_lattice(fields[0][1])
+        dst.data.copy_(acc)
+        return dst
     if dst is None:
         dst = lattice(fields[0][1])
     c0, f0 = fields[0]
@@ -544,9 +702,22 @@ class matrix_operator:
         return d
 
     def _apply(self, src):
+        if isinstance(src, mspincolor_lattice):
+            return self._apply_columns(src)
         dst = self._new_dst(src)
         self.mat(dst, src)
         return dst
+
+    def _apply_columns(self, src):
+        """an mspincolor operand: its 12 columns as right-hand sides c + 3 s
+        (container.py:330-357), through the list path, then reassembled"""
+        cols = [src.column(s, c) for s in range(4) for c in range(3)]
+        out = self(cols)
+        res = mspincolor_lattice(out[0].grid)
+        for s in range(4):
+            for c in range(3):
+                res.set_column(s, c, out[c + 3 * s])
+        return res
 
     def __call__(self, first, second=None):
         if second is None:
@@ -557,6 +728,9 @@ class matrix_operator:
                     return dst
                 return [self(x) for x in first]
             return self._apply(eval(first) if isinstance(first, expression) else first)
+        if isinstance(second, mspincolor_lattice):
+            first.data.copy_(self._apply_columns(second).data)
+            return first
         if isinstance(first, list):
             if self.accept_list:
                 self.mat(first, second)

@@ -134,3 +134,45 @@ def test_mixed_precision_defect_correction(config1):
     assert dc.history[-1] < 1e-10
     a, b = x.to_numpy(), x64.to_numpy()
     assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-8
+
+
+def test_point_source_propagator_as_mspincolor(kats):
+    """The reference's propagator test written with its own API
+    (tests/qcd/fermion_operators.py:135-218): g.mspincolor, g.create.point,
+    dst @= slv * src over the 12 spin-colour columns, EO1 against EO2, the
+    true residual w * dst - src, and the correlator g.slice(g.trace(dst *
+    g.adj(dst)), 3) against its known answer; a column equals the
+    vspincolor solve of the same point source bitwise."""
+    k = kats["correlator"]
+    gr = R.grid(k["fdims"])
+    rng = R.random(k["seed"])
+    U = G.random_links(gr, rng)
+    p = k["params"]
+    w = g.qcd.fermion.wilson_clover(
+        U, kappa=p["kappa"], csw_r=0.0, csw_t=0.0, xi_0=1.0, nu=1.0, isAnisotropic=False,
+        boundary_phases=[cmath.exp(1j * a) for a in p["boundary_phase_angles"]], precision=g.single)
+    grid = w.F_grid
+    src = g.mspincolor(grid)
+    g.create.point(src, k["point_source_xyzt"])
+    cg = inv.cg({"eps": 1e-6, "maxiter": 1000})
+    slv_eo1 = w.propagator(inv.preconditioned(pc.eo1_ne(), cg))
+    slv_eo2 = w.propagator(inv.preconditioned(pc.eo2_ne(), cg))
+    dst_eo1 = g.mspincolor(grid)
+    dst_eo2 = g.mspincolor(grid)
+    dst_eo1 @= slv_eo1 * src
+    dst_eo2 @= slv_eo2 * src
+    eps2 = g.norm2(dst_eo1 - dst_eo2) / g.norm2(dst_eo1)
+    print(f"EO1 versus EO2: eps2 = {eps2}", flush=True)
+    assert eps2 < 1e-12
+    eps2 = g.norm2(w * dst_eo1 - src) / g.norm2(src)
+    assert eps2 < 1e-10
+    correlator = g.slice(g.trace(dst_eo2 * g.adj(dst_eo2)), 3)
+    eps = np.linalg.norm(np.array(correlator) - np.array(k["reference"]))
+    print(f"correlator eps = {eps}", flush=True)
+    assert eps < k["tol_norm"]
+    # column (s, c) = 2, 1 is the solve of the point source in spin 2, colour 1
+    x0, y0, z0, t0 = k["point_source_xyzt"]
+    one = np.zeros(gr.shape + (4, 3), np.complex64)
+    one[t0, z0, y0, x0, 2, 1] = 1.0
+    col = g(slv_eo2 * to_dev(one, grid)).to_numpy()
+    assert np.array_equal(dst_eo2.to_numpy()[..., :, 2, :, 1], col)
