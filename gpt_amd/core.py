@@ -430,11 +430,12 @@ class site_scalar:
 def trace(x):
     """g.trace of an mspincolor (the 12 x 12 trace per site) or of a site
     product A * g.adj(B): tr(A B^dag) = sum_ij A_ij conj(B_ij), elementwise"""
+    # sums in complex128 whatever the field's precision
     if isinstance(x, _site_product):
-        a, b = x.a.data, x.b.data
+        a, b = x.a.data.to(torch.complex128), x.b.data.to(torch.complex128)
         return site_scalar(x.a.grid, (a * b.conj()).reshape(a.shape[0], -1).sum(dim=1))
     if isinstance(x, mspincolor_lattice):
-        d = x.data
+        d = x.data.to(torch.complex128)
         return site_scalar(x.grid, torch.diagonal(torch.diagonal(d, dim1=1, dim2=2), dim1=1, dim2=2).sum(dim=(1, 2)))
     raise NotImplementedError("g.trace of this object")
 
