@@ -9,7 +9,8 @@ normal_equation.py:35-65) with D = ``parity`` sites:
 Mpc and Mpc_ne run as fused HIP kernels (wc_apply_mpc); the CG inverter
 recognises this operator and runs its native fused loop.
 
-The neighbouring preconditioners of even_odd_sites.py are compositions of
+g5m_ne (g5m.py) solves on the full lattice with (gamma5 M)^2; it and the
+neighbouring preconditioners of even_odd_sites.py are compositions of
 the operator's HIP pieces (Meooe, Mooee and their inverses and adjoints) and
 the HIP BLAS-1 kernels, solved by the generic CG recursion (single
 right-hand-side fields):
@@ -241,6 +242,57 @@ eo1_ne = _factory("eo1_ne", lambda op, p: _normal_equation(_schur_one(op, p)))
 eo2 = _factory("eo2", _schur_two)
 eo2_kappa_ne = _factory("eo2_kappa_ne",
                         lambda op, p: _normal_equation(_similarity_scalar(_schur_two(op, p), op.kappa())))
+
+
+def _gamma5(dst, src):
+    """dst = gamma5 src; gamma5 = diag(1, 1, -1, -1) in gpt's chiral basis
+    (lib/gpt/core/gamma.py:28-53): the lower spins (components 6..11) negated"""
+    core.copy(dst, src)
+    d = dst.data
+    d.view(-1, 12, d.shape[-3] * d.shape[-2] * 2)[:, 6:12].neg_()
+
+
+class _g5m_ne:
+    """g5m_ne (lib/gpt/qcd/fermion/preconditioner/g5m.py:28-60): the full
+    lattice with the Hermitian gamma5 M: Mpc = (G5M)^2, L = 1, R = G5M gamma5,
+    S = 0, so that L Mpc^-1 R = M^-1"""
+
+    def __init__(self, op):
+        self.op = op
+        self.F_grid = op.F_grid
+        G5M = op.G5M
+
+        def make(s):
+            return lattice(op.F_grid)
+
+        def mpc(dst, src):
+            G5M.mat(dst, G5M(src))
+
+        def ident(dst, src):
+            core.copy(dst, src)
+
+        def r(dst, src):
+            t = lattice(op.F_grid)
+            _gamma5(t, src)
+            G5M.mat(dst, t)
+
+        def zero(dst, src):
+            dst.data.zero_()
+
+        self.Mpc = matrix_operator(mpc, mpc, make_dst=make)
+        self.L = matrix_operator(ident, inv_mat=ident, make_dst=make)
+        self.R = matrix_operator(r, make_dst=make)
+        self.S = matrix_operator(zero, make_dst=make)
+
+
+def g5m_ne(params=None):
+    """pc.g5m_ne() (g5m.py:63-69)"""
+
+    def instantiate(op):
+        return _g5m_ne(op)
+
+    instantiate.kind = "g5m_ne"
+    return instantiate
 
 
 def eo2_ne(params=None, parity=core.odd):

@@ -87,3 +87,20 @@ def test_cg_through_preconditioner(setup, name):
     assert cg.converged and abs(len(cg.history) - it_ref) <= 1
     assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-9
     assert np.linalg.norm(ref.M(x) - src) / np.linalg.norm(src) < 1e-8
+
+
+def test_g5m_ne_against_eo1_ne(setup):
+    """g5m_ne (g5m.py: Mpc = (gamma5 M)^2 on the full lattice, R = G5M gamma5,
+    L = 1, S = 0) gives the solution of the even-odd solve, as the
+    reference's propagator test checks (fermion_operators.py:170-175:
+    eps2 < 1e-10), and meets the true residual via the oracle's M"""
+    gr, U, ref, src = setup
+    w = g.qcd.fermion.wilson_clover(U, P)
+    cg = inv.cg(eps=1e-12, maxiter=4000)
+    x_g5 = g(w.propagator(inv.preconditioned(pc.g5m_ne(), cg)) * _full(src)).to_numpy()
+    assert cg.converged
+    x_eo = g(w.propagator(inv.preconditioned(pc.eo1_ne(), inv.cg(eps=1e-12, maxiter=4000))) * _full(src)).to_numpy()
+    eps2 = np.linalg.norm(x_g5 - x_eo) ** 2 / np.linalg.norm(x_eo) ** 2
+    print(f"g5m_ne: {len(cg.history)} iterations, eps2 vs eo1_ne {eps2}", flush=True)
+    assert eps2 < 1e-10
+    assert np.linalg.norm(ref.M(x_g5) - src) / np.linalg.norm(src) < 1e-9
