@@ -104,3 +104,23 @@ def test_g5m_ne_against_eo1_ne(setup):
     print(f"g5m_ne: {len(cg.history)} iterations, eps2 vs eo1_ne {eps2}", flush=True)
     assert eps2 < 1e-10
     assert np.linalg.norm(ref.M(x_g5) - src) / np.linalg.norm(src) < 1e-9
+
+
+def test_inverse_of_the_daggered_operator(setup):
+    """the reference's check of inv(dag(w)) (fermion_operators.py:177-187):
+    <v2, M^-1 v1> = conj(<v1, (M^dag)^-1 v2>), with eo1_ne on w and on w.adj()"""
+    gr, U, ref, src = setup
+    w = g.qcd.fermion.wilson_clover(U, P)
+    rng = np.random.default_rng(17)
+    shape = tuple(reversed(FD)) + (4, 3)
+    v1 = _full(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    v2 = _full(rng.standard_normal(shape) + 1j * rng.standard_normal(shape))
+    cg = inv.cg(eps=1e-10, maxiter=4000)
+    w_v1 = g(inv.preconditioned(pc.eo1_ne(), cg)(w) * v1)
+    wadj_v2 = g(inv.preconditioned(pc.eo1_ne(), cg)(w.adj()) * v2)
+    eps = abs(g.inner_product(v2, w_v1) / g.inner_product(v1, wadj_v2).conjugate() - 1.0)
+    print(f"inv(dag(w)): {eps}", flush=True)
+    assert eps < 1e-8
+    # and (M^dag)^-1 v2 solves M^dag x = v2 (the oracle's adjoint)
+    x = wadj_v2.to_numpy()
+    assert np.linalg.norm(ref.Mdag(x) - v2.to_numpy()) / np.linalg.norm(v2.to_numpy()) < 1e-8
