@@ -236,3 +236,22 @@ def test_operator_pool_reuse_is_exact(small):
     ref = W.wilson_clover(U, **p2)
     got = w2(to_dev(src, w2.F_grid)).to_numpy()
     assert site_rel_err(got, ref.M(src)) < 1e-13
+
+
+@pytest.mark.parametrize("case", ["baseline_clover", "open_clover", "wilson_twisted"])
+def test_gauge_covariance(small, case):
+    """M[U^V] (V psi) = V (M[U] psi) for a random SU(3) gauge transformation V,
+    U^V_mu(x) = V(x) U_mu(x) V(x + mu)^dag (the reference's gauge
+    transformation check, fermion_operators.py:258-270, at the operator
+    level): pins the link directions, the double storage and the clover
+    leaves independently of the oracle"""
+    gr, U, src = small
+    V = G.random_links(gr, R.random("gauge_transform"), scale=1.0)[0]
+    Ut = [V @ U[mu] @ cv.dagger(cv.shift(V, mu, 1)) for mu in range(4)]
+    for compress in (False, True):
+        p = dict(CASES[case], compress_links=compress)
+        w = g.qcd.fermion.wilson_clover(U, p)
+        wt = g.qcd.fermion.wilson_clover(Ut, p)
+        a = g(w * to_dev(src, w.F_grid)).to_numpy()
+        b = g(wt * to_dev(cv.colour_mul(V, src), wt.F_grid)).to_numpy()
+        assert site_rel_err(b, cv.colour_mul(V, a)) < 1e-13, compress
