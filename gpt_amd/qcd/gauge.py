@@ -47,3 +47,19 @@ def unit(grid):
 def stacked(U):
     """[4][V][3][3] complex128 contiguous device tensor for the operator set-up."""
     return torch.stack([u.data for u in U]).contiguous()
+
+
+def plaquette(U):
+    """g.qcd.gauge.plaquette (lib/gpt/qcd/gauge/loops.py): the average of
+    Re tr of the plaquettes over the 6 planes, normalised to 1 for unit links,
+    computed on the GPU (wc_gauge_observables)"""
+    from .. import io
+
+    return io.gauge_observables(U)[0]
+
+
+def link_trace(U):
+    """the average Re tr U_mu(x) / 3 over sites and directions, on the GPU"""
+    from .. import io
+
+    return io.gauge_observables(U)[1]

@@ -41,8 +41,10 @@ def test_plaquette_known_answers():
         U = G.random_links(R.grid(case["fdims"], case["precision"]), rng, case["scale"])
         if case["precision"] != "double":
             continue
-        P, L = io.gauge_observables(gauge.from_numpy(g.grid(case["fdims"], g.double), U))
+        Ud = gauge.from_numpy(g.grid(case["fdims"], g.double), U)
+        P, L = io.gauge_observables(Ud)
         assert abs(P - case["value"]) ** 2 < case["tol_abs2"], (P, case["value"])
+        assert g.qcd.gauge.plaquette(Ud) == P and g.qcd.gauge.link_trace(Ud) == L
         L_ref = sum(np.trace(u, axis1=-2, axis2=-1).real.mean() for u in U) / 12.0
         assert abs(L - L_ref) < 1e-15
 
