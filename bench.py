@@ -225,7 +225,39 @@ def slab_share(torch, g, dev, fd, args):
            "halo_bytes_per_hop": 2 * face_sites * 6 * 16,
            "halo_us_at_153GBps_per_link": round(face_sites * 6 * 16 / 153e9 * 1e6, 2),
            "shape": list(shape)}
-    del w, pairs, Us
+    # the halo kernels of the exchange (wc_halo_pack / wc_halo_unpack); bytes
+    # per face site: pack reads 2 spinors and 1 link and writes 2 half
+    # spinors, unpack reads 2 half spinors and 1 link and updates 2 spinors
+    # a call takes a few microseconds, less than its host-side launch, so
+    # `reps` calls are captured in one graph and the replay is timed
+    src, dst = pairs[0]
+    lo, hi = w.halo_pack(src)
+    reps = 100
+    for name, fn, nbytes in [("halo_pack", lambda: w.halo_pack(src, out=(lo, hi)),
+                              (2 * 192 + 144 + 2 * 96) * face_sites),
+                             ("halo_unpack", lambda: w.halo_unpack(lo, hi, dst),
+                              (2 * 96 + 144 + 4 * 192) * face_sites)]:
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(reps):
+                fn()
+        graph.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        e0.record(stream)
+        for _ in range(5):
+            graph.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / (5 * reps) * 1e-3
+        out[name] = {"us_per_call": round(t * 1e6, 2), "bytes_per_call": nbytes, "GBps": round(nbytes / t / 1e9, 1)}
+        del graph
+    del w, pairs, Us, lo, hi
     torch.cuda.empty_cache()
     return out
 

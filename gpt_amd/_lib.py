@@ -75,6 +75,8 @@ SIGNATURES = {
     "wc_set_iteration_chunk": (_I, [_P, _I]),
     "wc_trim": (_I, [_P]),
     "wc_gauge_observables": (_I, [_P, _P, _P, _P, _P]),
+    "wc_halo_pack": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
+    "wc_halo_unpack": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
     "wc_block_cg_eo2_ne": (_I, [_P, _I, _I, _P, _P, _D, _D, _L, _L, _P, _P, _P, _P]),
     "wc_solve_eo2_ne_block_cg": (_I, [_P, _I, _I, _P, _P, _D, _D, _L, _L, _P, _P, _P, _P]),
     "wc_iteration_runs": (_I, [_P, _P, _P]),

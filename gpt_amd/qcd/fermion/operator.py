@@ -241,6 +241,54 @@ class fine_operator(matrix_operator):
         sh = (ctypes.c_int * 5)(*[int(x) for x in shape])
         _lib.check(_lib.load().wc_set_hop_shape(self.handle, n, ctypes.addressof(sh)))
 
+    # halo of a T-slab decomposition (wc_halo_pack / wc_halo_unpack): with
+    # this operator's lattice as one rank's slab, the spin-projected t-faces
+    # a hop would send to the ranks below and above, and the t-terms the
+    # received faces add to the two boundary slices
+    def halo_pack(self, src, dag=False, out=None):
+        """(lo, hi) face tensors of the half field src for the hop into parity
+        1 - src's: lo the projected t = 0 slice, hi U_t^dag times the projected
+        t = Lt - 1 slice; 6 * nf * n_rhs complex values each, entry
+        (3 * spin + colour, face site, rhs).  out: (lo, hi) to write into."""
+        self._check_halo_field(src)
+        fd = src.grid.fdimensions
+        nf = fd[0] * fd[1] * fd[2] // 2
+        n = src.grid.nrhs
+        dt = core._complex_dtype(src.grid.precision)
+        if out is None:
+            lo = core.torch.empty(6 * nf * n, dtype=dt, device=core.device())
+            hi = core.torch.empty_like(lo)
+        else:
+            lo, hi = out
+            self._check_faces(lo, hi, dt, 6 * nf * n)
+        pout = 1 - src.checkerboard().tag
+        _lib.check(_lib.load().wc_halo_pack(self.handle, pout, int(bool(dag)), n, src.ptr(), lo.data_ptr(),
+                                            hi.data_ptr(), core.stream_ptr()))
+        return lo, hi
+
+    def halo_unpack(self, lo, hi, dst, dag=False):
+        """dst (a half field) += the t-terms of its t = Lt - 1 slice from lo
+        (the t = 0 face of the rank above) and of its t = 0 slice from hi (the
+        t = Lt - 1 face of the rank below)"""
+        self._check_halo_field(dst)
+        fd = dst.grid.fdimensions
+        n = dst.grid.nrhs
+        self._check_faces(lo, hi, core._complex_dtype(dst.grid.precision), 6 * (fd[0] * fd[1] * fd[2] // 2) * n)
+        _lib.check(_lib.load().wc_halo_unpack(self.handle, dst.checkerboard().tag, int(bool(dag)), n,
+                                              lo.data_ptr(), hi.data_ptr(), dst.ptr(), core.stream_ptr()))
+
+    def _check_halo_field(self, x):
+        if x.grid.cb is not core.redblack:
+            raise ValueError("the halo calls take a half (checkerboarded) field")
+        if x.grid.precision is not self.precision or list(x.grid.fdimensions) != list(self.U_grid.fdimensions):
+            raise ValueError("field precision or lattice differs from the operator's")
+
+    @staticmethod
+    def _check_faces(lo, hi, dt, size):
+        for f in (lo, hi):
+            if f.dtype != dt or f.numel() != size or not f.is_contiguous() or f.device != core.device():
+                raise ValueError(f"a halo face must be a contiguous {dt} device tensor of {size} values")
+
     # CG iteration schedule (wc_set_iteration_chunk): iterations run in
     # captured graphs of up to `chunk` iterations with the stopping test on the
     # device; 0 automatic, 1 one at a time (host test after each iteration)

@@ -286,6 +286,33 @@ int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);
 int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double *true_residual, int *converged);
 int wc_batch_slots(const wc_batch *b, int *occupied, int *active);
 
+/* --- halo of a T-slab decomposition -------------------------------------
+ * Grid exchanges the faces of a cut lattice inside its stencil, with
+ * communication overlapped with the interior
+ * (lib/cgpt/lib/operators/wilson_clover.h:45, overlapCommsCompute); gpt's
+ * FFI has no call for it.  These two calls are the kernels one GPU runs for
+ * that exchange when the lattice is cut in t and this operator's lattice is
+ * one rank's slab (periodic in t, so on one GPU the rank above and below is
+ * the GPU itself).  For the hop (dag 0) or its adjoint (dag 1) of output
+ * parity parity_out on nrhs right-hand sides (field layout as for
+ * wc_apply_block), with nf = Lx Ly Lz / 2 face sites:
+ *   wc_halo_pack reads src (parity 1 - parity_out) and writes
+ *     face_lo: (1 -+ gamma_t) psi of the t = 0 slice (for the rank below),
+ *     face_hi: U_t^dag (1 +- gamma_t) psi of the t = Lt - 1 slice, hop
+ *              coefficient and boundary phase folded in (for the rank above);
+ *   wc_halo_unpack adds to dst (parity parity_out) the t-terms that
+ *     face_lo (received from above) gives its t = Lt - 1 slice and face_hi
+ *     (received from below) its t = 0 slice.
+ * A face buffer holds 6 * nf * nrhs complex values of the operator's
+ * precision: entry k = 3 * spin + colour of face site f (spatial half-site
+ * index) and right-hand side r at (k * nf + f) * nrhs + r.  A hop whose
+ * t-links on the last slice are zero, plus wc_halo_unpack of
+ * wc_halo_pack(src), is the hop of src.  Not for open boundaries. */
+int wc_halo_pack(wc_operator *op, int parity_out, int dag, int nrhs, const void *src, void *face_lo, void *face_hi,
+                 void *stream);
+int wc_halo_unpack(wc_operator *op, int parity_out, int dag, int nrhs, const void *face_lo, const void *face_hi,
+                   void *dst, void *stream);
+
 /* Gauge observables of natural-layout FP64 links U[4][V][3][3] (site
  * ((t*Lz + z)*Ly + y)*Lx + x, periodic): the average plaquette
  * (1/(18 V)) sum_x sum_{mu<nu} Re tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
