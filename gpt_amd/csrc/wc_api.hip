@@ -641,11 +641,12 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
   wck::zero<R>(n, op->tmp[1], s);
   std::vector<wck::HopShape> cand;
   // t-marching order (seg > 1) has never won on MI355X and is left to
-  // wc_set_hop_shape; LDS staging (ls) only pays when a site has several
-  // items (more right-hand sides than rb)
+  // wc_set_hop_shape; link staging (ls 1) only pays when a site has several
+  // items (more right-hand sides than rb), spinor-block staging (ls 2) is
+  // for one right-hand side
   const int occs[] = {0, 2, 4, 8, 16};
   for (int rb = 1; rb <= max_rb_prec(op->prec) && rb <= N; ++rb)
-    for (int ls = 0; ls <= (N > rb ? 1 : 0); ++ls)
+    for (int ls = 0; ls <= 2; ++ls)
       for (int tpb = 1; tpb <= wck::kMaxTpb; tpb *= 2)
         for (int oc : occs) {
           wck::HopShape sh;
@@ -653,7 +654,8 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
           sh.tpb = tpb;
           sh.occ = oc;
           sh.ls = ls;
-          if (shape_valid(op, sh)) cand.push_back(sh);
+          if ((ls == 1 && N <= rb) || (ls == 2 && N != 1)) continue;
+          if (shape_valid(op, sh) && (ls != 2 || stage_fits(op, N, sh))) cand.push_back(sh);
         }
   hipEvent_t e0, e1;
   WC_HIP(hipEventCreate(&e0));
@@ -750,7 +752,7 @@ int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape) {
   if (!shape_valid(op, sh) || !stage_fits(op, nrhs, sh))
     WC_FAIL(WC_ERR_ARG,
             "wc_set_hop_shape: rb=%d tpb=%d occ=%d seg=%d ls=%d invalid (rb 1..%d, tpb 1..%d, occ 0..%d, seg 1 or "
-            "2..%d, ls 0 or 1 with a staging area of at most %d bytes)",
+            "2..%d, ls 0, 1 or 2 with a staging area of at most %d bytes)",
             sh.rb, sh.tpb, sh.occ, sh.seg, sh.ls, max_rb_prec(op->prec), wck::kMaxTpb, wck::kMaxOcc, op->bps,
             wck::kMaxStageBytes);
   op->shape[nrhs] = sh;

@@ -137,11 +137,15 @@ __device__ __forceinline__ void compute_dir(const HopLaunch &a, const int c[4], 
 //
 // LS: the links are staged in LDS, prepared (prepare_link), entry e of
 // direction dir of the site at lw[(dir * 9 + e) * ns].
-template <typename R, int NL, int DAG, int RB, typename SI, bool LS>
+//
+// XS: xst (non-null) holds the input spinors of the half-sites xs0 ..
+// xs0 + xsn - 1, component k of site xs0 + i at xst[k * xsl + i]; a
+// neighbour in that range is read from there (one right-hand side).
+template <typename R, int NL, int DAG, int RB, typename SI, bool LS, bool XS>
 __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int rl[RB], const int cx,
                                          const int cy, const int cz, const int ct,
                                          typename c2<R>::t acc[RB][12], const typename c2<R>::t *lw,
-                                         int ns) {
+                                         int ns, const typename c2<R>::t *xst, long xs0, int xsn, int xsl) {
   // coordinates as scalars: a select among array elements would become a
   // dynamically indexed (scratch) array
   const int c[4] = {cx, cy, cz, ct};
@@ -188,18 +192,29 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
       for (int e = 0; e < NL; ++e)
         u[e] = bload<C, WC_LINK_AUX>(rw, vw, dsoff + (uint32_t)(e * kBlk * sizeof(C)));
     }
-    const uint32_t vs = (uint32_t)aidx_site(nb, 12) * N;
     P psi[RB][12];
+    bool staged = false;
+    if constexpr (XS) {
+      const uint32_t li = nb - (uint32_t)xs0;
+      if (xst != nullptr && li < (uint32_t)xsn) {
 #pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(SI);
+        for (int k = 0; k < 12; ++k) psi[0][k] = xst[k * xsl + (int)li];
+        staged = true;
+      }
+    }
+    if (!staged) {
+      const uint32_t vs = (uint32_t)aidx_site(nb, 12) * N;
 #pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        const SI v = bload<SI, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride);
-        if constexpr (std::is_same<SI, c2h>::value)
-          psi[j][k] = __builtin_bit_cast(h2v, v);
-        else
-          psi[j][k] = widen(v);
+      for (int j = 0; j < RB; ++j) {
+        const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(SI);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          const SI v = bload<SI, WC_SPINOR_AUX>(in, vo, (uint32_t)k * kstride);
+          if constexpr (std::is_same<SI, c2h>::value)
+            psi[j][k] = __builtin_bit_cast(h2v, v);
+          else
+            psi[j][k] = widen(v);
+        }
       }
     }
     switch (dir) {
@@ -230,6 +245,16 @@ __device__ __forceinline__ long block_order(long T, int seg, int bps, int lt) {
   const long off = T - (long)lt * lo;
   const long t = off / len;
   return t * bps + lo + (off - t * len);
+}
+
+// ls == 2 (spinor-block staging) applies to hops of one right-hand side in
+// natural block order whose input is in the working precision, without link
+// staging; xs_possible is the compile-time part, xs_on the launch's
+template <typename R, int RB, typename SI, bool LS> constexpr bool xs_possible() {
+  return !LS && RB == 1 && std::is_same<SI, typename c2<R>::t>::value;
+}
+__host__ __device__ inline bool xs_on(const HopLaunch &a) {
+  return a.shape.ls == 2 && a.nrhs == 1 && a.shape.seg <= 1;
 }
 
 // waves per SIMD the register allocation must allow: 4 for the FP32 hop
@@ -294,6 +319,28 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
       }
     __syncthreads();
   }
+  // spinor-block staging (xs_on): the workgroup's output sites are the
+  // half-sites q0 .. q0 + blockDim - 1 (one item per site); their input
+  // spinors (the other parity's sites of the same half-site index) are
+  // copied to LDS, component k of site q0 + i at stage[k * blockDim + i].
+  // The +-x neighbours of a site lie in its row and most +-y neighbours in
+  // the adjacent rows, inside that range, so those gathers read LDS and
+  // only the rest go through the texture path.
+  const C *xst = nullptr;
+  int xsn = 0;
+  if constexpr (xs_possible<R, RB, SI, LS>()) {
+    if (xs_on(a)) {
+      const long rem = g.Vh - q0;
+      xsn = rem < (long)blockDim.x ? (int)rem : (int)blockDim.x;
+      if ((int)threadIdx.x < xsn) {
+        const long hs = q0 + threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) stage[k * (int)blockDim.x + (int)threadIdx.x] = ((const C *)a.in)[aidx(k, hs, 12)];
+      }
+      __syncthreads();
+      xst = stage;
+    }
+  }
   if (v >= g.Vh) return;
   const int rb = (int)(q - v * Gt) * RB;
   const long h = site_of(v);
@@ -308,7 +355,8 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   for (int j = 0; j < RB; ++j)
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[j][k] = cmk<C>(0, 0);
-  hop_site<R, NL, DAG, RB, SI, LS>(a, h, rl, c[0], c[1], c[2], c[3], acc, stage + sv, ns);
+  hop_site<R, NL, DAG, RB, SI, LS, xs_possible<R, RB, SI, LS>()>(a, h, rl, c[0], c[1], c[2], c[3], acc,
+                                                                   stage + sv, ns, xst, q0, xsn, (int)blockDim.x);
   // epilogue
   const bool zb = a.open_bc && (c[3] == 0 || c[3] == g.L[3] - 1);
   if (zb) {
@@ -392,9 +440,13 @@ static bool hop_epi(const HopLaunch &a, hipStream_t s) {
   // LDS: the staging area (shape.ls), at least the share of a CU that caps
   // the workgroups per CU at shape.occ
   size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
-  if (a.shape.ls) {
+  if (a.shape.ls == 1) {
     const size_t st = (size_t)(72 + ((a.epi & EPI_CLOV) && a.clov ? 36 : 0)) *
                       (size_t)stage_sites(threads, G) * sizeof(C);
+    if (st > (size_t)kMaxStageBytes) return false;
+    lds = std::max(lds, st);
+  } else if (xs_possible<R, RB, SI, false>() && xs_on(a)) {
+    const size_t st = (size_t)12 * (size_t)threads * sizeof(C);
     if (st > (size_t)kMaxStageBytes) return false;
     lds = std::max(lds, st);
   }
@@ -402,7 +454,7 @@ static bool hop_epi(const HopLaunch &a, hipStream_t s) {
 #define WC_EPI(E)                                                                        \
   case E:                                                                                \
     if constexpr (epi_instantiated<R, SI, SO>(E)) {                                     \
-      if (a.shape.ls)                                                                    \
+      if (a.shape.ls == 1)                                                               \
         k_hop<R, NL, DAG, E, RB, SI, SO, true><<<grid, (unsigned)threads, lds, s>>>(a);  \
       else                                                                               \
         k_hop<R, NL, DAG, E, RB, SI, SO, false><<<grid, (unsigned)threads, lds, s>>>(a); \

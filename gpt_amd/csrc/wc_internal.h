@@ -209,6 +209,8 @@ inline int max_rb_prec(int prec) { return prec == WC_DOUBLE ? max_rb_of<double>(
 // limit
 inline bool stage_fits(const wc_operator *op, int nrhs, const wck::HopShape &sh) {
   if (!sh.ls) return true;
+  // ls 2: 12 spinor components per thread (the hop ignores it unless one RHS)
+  if (sh.ls == 2) return (size_t)12 * kBlk * sh.tpb * op->csz <= (size_t)wck::kMaxStageBytes;
   const long G = (nrhs + sh.rb - 1) / sh.rb;
   const size_t bytes = (size_t)(72 + 36) * (size_t)wck::stage_sites((long)kBlk * sh.tpb, G) * op->csz;
   return bytes <= (size_t)wck::kMaxStageBytes;
@@ -241,7 +243,7 @@ inline wck::HopShape default_shape(const wc_operator *op, int nrhs) {
 inline bool shape_valid(const wc_operator *op, const wck::HopShape &sh) {
   return sh.tpb >= 1 && sh.tpb <= wck::kMaxTpb && sh.rb >= 1 && sh.rb <= max_rb_prec(op->prec) && sh.occ >= 0 &&
          sh.occ <= wck::kMaxOcc && sh.seg >= 1 && (sh.seg == 1 || (op->bps > 0 && sh.seg <= op->bps)) &&
-         (sh.ls == 0 || sh.ls == 1);
+         (sh.ls >= 0 && sh.ls <= 2);
 }
 
 // process-wide cache of tuned shapes: (dims, precision, link format, class of

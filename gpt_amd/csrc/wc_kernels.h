@@ -26,7 +26,9 @@ struct HopShape {
   int tpb = 1;
   int occ = 0;
   int seg = 1;  // > 1: t-marching order, seg runs per time slice (block_order)
-  int ls = 0;   // 1: a workgroup stages its sites' links (and EPI_CLOV blocks) in LDS once
+  int ls = 0;   // 1: a workgroup stages its sites' links (and EPI_CLOV blocks) in LDS once;
+                // 2 (one right-hand side): it stages the input spinors of its own half-site
+                // range, where the +-x and most +-y neighbours are
 };
 constexpr int kLdsPerCU = 160 * 1024;
 constexpr int kMaxStageBytes = 80 * 1024;  // largest LDS staging area of one workgroup

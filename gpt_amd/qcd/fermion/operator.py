@@ -213,7 +213,8 @@ class fine_operator(matrix_operator):
     # hop launch shapes (rb, tpb, occ, seg, ls): rb right-hand sides per
     # thread, 64 * tpb threads per workgroup, at most occ workgroups per CU (0:
     # no cap), seg t-marching runs per time slice (1: natural order), ls 1: a
-    # workgroup stages its sites' links and clover blocks in LDS
+    # workgroup stages its sites' links and clover blocks in LDS, ls 2 (one
+    # right-hand side): it stages the input spinors of its own site range
     # (wc_tune_hop / wc_set_hop_shape / wc_get_hop_shape); every shape gives
     # bitwise the same result
     def tune_hop(self, n_rhs=None, reps=5):

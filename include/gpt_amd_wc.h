@@ -217,8 +217,11 @@ int wc_solve_eo2_ne_block_cg(wc_operator *op, int parity, int nrhs, void *dst, c
  * a whole number of blocks, seg <= that number), and ls = 1 makes a
  * workgroup copy the links (and clover blocks) of its sites to LDS once, so
  * that the several items of a site read them from there (at most 80 KB of
- * LDS per workgroup).  Every shape gives bitwise the same result; only the
- * speed differs.
+ * LDS per workgroup), and ls = 2, for one right-hand side in natural order
+ * (ignored otherwise), makes it copy the input spinors of its own
+ * half-site range to LDS, where the +-x and most +-y neighbours of its
+ * sites are, so that those gathers read LDS.  Every shape gives bitwise the
+ * same result; only the speed differs.
  *
  * wc_tune_hop times the hop of nrhs right-hand sides (reps calls per shape)
  * for every valid shape and keeps the fastest for this operator and, in a

@@ -151,7 +151,75 @@ def test_every_hop_shape_gives_identical_bits(setup):
         with pytest.raises(RuntimeError, match="invalid"):
             w.set_hop_shape((1, 1, 0, 5, 0))  # 8^3 x 16: 4 blocks per time slice
         with pytest.raises(RuntimeError, match="invalid"):
-            w.set_hop_shape((1, 1, 0, 1, 2))
+            w.set_hop_shape((1, 1, 0, 1, 3))
+
+
+@pytest.mark.parametrize("fd", [[8, 8, 8, 16], [12, 8, 8, 8]])
+def test_single_rhs_spinor_staging_gives_identical_bits(fd):
+    """ls = 2 (one right-hand side: the workgroup's own input spinors in LDS,
+    the +-x and most +-y gathers read from there) changes only where values
+    are read from: the hop, its adjoint, the fused normal operator and a CG
+    agree bit for bit with ls = 0 for every workgroup size.  Lx = 12 puts
+    rows of 6 half-sites across the 64-site blocks, so some x and y
+    neighbours of a workgroup's sites lie outside its range."""
+    rng = R.random("staging")
+    U = G.random_links(R.grid(fd), rng, scale=0.5)
+    src = rng.vspincolor(R.grid(fd))
+    for pr in (g.double, g.single):
+        for compress in (False, True):
+            w = g.qcd.fermion.wilson_clover(U, P, precision=pr, compress_links=compress)
+            s = g.lattice(w.F_grid)
+            s.from_numpy(src.astype(pr.complex_dtype))
+            H = g.pick_checkerboard(g.odd, s)
+            pre = pc.eo2_ne()(w)
+            ref = None
+            for tpb in (1, 2, 3, 4, 8):
+                for occ in (0, 4):
+                    for ls in (0, 2):
+                        try:
+                            w.set_hop_shape((1, tpb, occ, 1, ls))
+                        except RuntimeError as e:  # FP64 at 512 threads: 96 KB of staging
+                            assert ls == 2 and pr is g.double and tpb == 8 and "staging area" in str(e)
+                            continue
+                        out = [w.Meooe(H).data.clone(), w.Meooe.adj()(H).data.clone(),
+                               pre.Mpc(H).data.clone(), pre.Mpc.adj()(H).data.clone()]
+                        if ref is None:
+                            ref = out
+                        else:
+                            for a, b in zip(out, ref):
+                                assert torch.equal(a, b), (pr.__name__, compress, tpb, occ, ls)
+            res = []
+            for sh in [(1, 1, 0, 1, 0), (1, 2, 4, 1, 2)]:
+                w.set_hop_shape(sh)
+                cg1 = inv.cg(eps=1e-5 if pr is g.single else 1e-10, maxiter=500)
+                X = g(w.propagator(inv.preconditioned(pc.eo2_ne(), cg1)) * s)
+                res.append((X.data.clone(), cg1.iterations))
+            assert res[0][1] == res[1][1] and torch.equal(res[0][0], res[1][0])
+            w.set_hop_shape(None)
+
+
+def test_spinor_staging_shape_on_a_16bit_inner_operator():
+    """An ls = 2 shape on the FP32 inner operator of the mixed solve: its
+    16-bit hops do not stage (the shape's other fields still apply) and the
+    solve is the same bit for bit."""
+    fd = [8, 8, 8, 16]
+    U = G.smooth_links(fd, 0.3, 3001)
+    rng = np.random.default_rng(3002)
+    a = rng.standard_normal(tuple(reversed(fd)) + (4, 3)) + 1j * rng.standard_normal(tuple(reversed(fd)) + (4, 3))
+    Pm = dict(mass=0.05, csw_r=1.0, csw_t=1.0, boundary_phases=[1, 1, 1, -1])
+    w = g.qcd.fermion.wilson_clover(U, Pm)
+    s = g.lattice(w.F_grid)
+    s.from_numpy(a)
+    out = []
+    for ls in (0, 2):
+        for storage in ("half", None):
+            mcg = inv.cg_mixed(eps=1e-10, maxiter=5000, inner_storage=storage)
+            mcg.inner_operator(w).set_hop_shape((1, 2, 4, 1, ls), n_rhs=1)
+            X = g(w.propagator(inv.preconditioned(pc.eo2_ne(), mcg)) * s)
+            assert mcg.converged
+            out.append((X.data.clone(), mcg.iterations, mcg.reliable_updates))
+    for k in (0, 1):
+        assert torch.equal(out[k][0], out[k + 2][0]) and out[k][1:] == out[k + 2][1:]
 
 
 def test_tuner_picks_a_valid_shape(setup):
