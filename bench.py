@@ -26,14 +26,16 @@ HBM_PEAK_GBS = 8000.0
 
 def profiled_traffic(shape):
     """HBM bytes per launch of the headline hop from the committed rocprofv3
-    summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated), when the
-    launch shape in use is the profiled one; else None"""
+    summary (FETCH_SIZE and WRITE_SIZE passes, copy-calibrated; lines
+    ``traffic_bytes_per_launch[rb,tpb,occ,seg,ls]: bytes``), when the launch
+    shape in use is a profiled one; else None"""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hop_fp64_18_32x32x32x64.md")
+    key = "traffic_bytes_per_launch[" + ",".join(str(int(x)) for x in shape) + "]: "
     try:
-        kv = dict(line.split(": ", 1) for line in open(path).read().splitlines() if line.startswith("traffic_"))
-        if tuple(int(x) for x in kv["traffic_shape"].split(",")) == tuple(shape):
-            return int(kv["traffic_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
+        for line in open(path).read().splitlines():
+            if line.startswith(key):
+                return int(line[len(key):])
+    except (OSError, ValueError):
         pass
     return None
 
