@@ -599,8 +599,8 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     """Multi-RHS: smoothed links (seed 4001), mass 0.08, csw 1; nrhs
     complex-Gaussian sources (seeds 4100 + j) to a true relative residual
     tol.  FP64 eo2_ne CG one RHS at a time against the mixed-precision
-    multi-RHS solve with 16-bit inner storage, in batches of at most 12 RHS
-    whose half fields stay within the 4 GB offset window of the kernels."""
+    multi-RHS solve with 16-bit inner storage, in equal batches as large as
+    the 4 GB offset window of the kernels allows for an FP64 half field."""
     from gpt_amd.qcd.fermion.operator import pack
 
     U = smooth_links_device(torch, fd, 0.3, seed, dev)
@@ -609,7 +609,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     grid = g.grid(fd, g.double)
     Vh = int(np.prod(fd)) // 2
     per_rhs = Vh * 12 * 16  # FP64 half field of one RHS
-    nb = min(nrhs, 12 if per_rhs * nrhs >= (1 << 32) else nrhs)
+    nb = max(d for d in range(1, nrhs + 1) if nrhs % d == 0 and per_rhs * d < (1 << 32))
     srcs = []
     for j in range(nrhs):
         gen = torch.Generator(device=dev)
