@@ -122,14 +122,17 @@ __host__ __device__ __forceinline__ long aidx(int k, long h, int NK) {
 // same, split into the per-site part (for a VGPR offset) and the entry part
 __host__ __device__ __forceinline__ long aidx_site(long h, int NK) { return (h >> 6) * (long)NK * kBlk + (h & 63); }
 
-// coordinates of half-site h of parity p
+// coordinates of half-site h of parity p.  In 32-bit unsigned arithmetic:
+// h < Vh < 2^32 (a half field must fit the 4 GB window), and a 64-bit
+// division costs several times the instructions of a 32-bit one.
 __device__ __forceinline__ void half_coords(const Geom &g, long h, int p, int c[4]) {
-  long r = h / g.Lxh;
-  int xh = (int)(h - r * g.Lxh);
-  long r2 = r / g.L[1];
-  c[1] = (int)(r - r2 * g.L[1]);
-  long r3 = r2 / g.L[2];
-  c[2] = (int)(r2 - r3 * g.L[2]);
+  const uint32_t hu = (uint32_t)h, lxh = (uint32_t)g.Lxh, ly = (uint32_t)g.L[1], lz = (uint32_t)g.L[2];
+  const uint32_t r = hu / lxh;
+  const int xh = (int)(hu - r * lxh);
+  const uint32_t r2 = r / ly;
+  c[1] = (int)(r - r2 * ly);
+  const uint32_t r3 = r2 / lz;
+  c[2] = (int)(r2 - r3 * lz);
   c[3] = (int)r3;
   c[0] = 2 * xh + ((c[1] + c[2] + c[3] + p) & 1);
 }

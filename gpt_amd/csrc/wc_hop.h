@@ -279,7 +279,9 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   const int Gt = (N + RB - 1) / RB;
   const long q0 = xcd_tile(blockIdx.x, gridDim.x) * (long)blockDim.x;
   const long q = q0 + threadIdx.x;
-  const long v = q / Gt;  // site in visiting order
+  // items number Vh * Gt < 2^31 (a field of N right-hand sides fits 4 GB):
+  // 32-bit unsigned divisions
+  const long v = Gt == 1 ? q : (long)((uint32_t)q / (uint32_t)Gt);  // site in visiting order
   // a 64-site block has 64 * Gt items, a whole number of waves: a wave's
   // sites lie in one block, and blocks can be visited in any order
   auto site_of = [&](long vv) {
@@ -292,16 +294,16 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   // them through the texture path and preparing the links again
   extern __shared__ __align__(16) char wc_dyn_lds[];
   C *stage = (C *)wc_dyn_lds;
-  const long v0 = q0 / Gt;
+  const long v0 = Gt == 1 ? q0 : (long)((uint32_t)q0 / (uint32_t)Gt);
   int ns = 0;
   const bool clov_staged = LS && (EPI & EPI_CLOV) && a.clov;
   if constexpr (LS) {
-    long v1 = (q0 + (long)blockDim.x - 1) / Gt;
+    long v1 = (long)((uint32_t)(q0 + (long)blockDim.x - 1) / (uint32_t)Gt);
     if (v1 > g.Vh - 1) v1 = g.Vh - 1;
     ns = v1 >= v0 ? (int)(v1 - v0 + 1) : 0;
     const int nw = 8 * NL;
     for (int i = threadIdx.x; i < 8 * ns; i += blockDim.x) {
-      const int dir = i / ns, sidx = i - dir * ns;
+      const int dir = (int)((uint32_t)i / (uint32_t)ns), sidx = i - dir * ns;
       const long hv = site_of(v0 + sidx);
       int cs[4];
       half_coords(g, hv, a.pout, cs);
@@ -314,7 +316,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
     }
     if (clov_staged)
       for (int i = threadIdx.x; i < 36 * ns; i += blockDim.x) {
-        const int e = i / ns, sidx = i - e * ns;
+        const int e = (int)((uint32_t)i / (uint32_t)ns), sidx = i - e * ns;
         stage[72 * ns + i] = ((const C *)a.clov)[aidx(e, site_of(v0 + sidx), 36)];
       }
     __syncthreads();
@@ -496,6 +498,8 @@ template <typename R> bool hop_launch(const HopLaunch &a, hipStream_t s) {
       a.nrhs < 1 || a.nrhs > 64 || a.g.Vh % kBlk)
     return false;
   if (sh.seg < 1 || (sh.seg > 1 && (a.bps <= 0 || sh.seg > a.bps))) return false;
+  // k_hop indexes items in 32 bits
+  if ((double)a.g.Vh * (double)((a.nrhs + sh.rb - 1) / sh.rb) >= 2147483648.0) return false;
   if (!a.half_in && !a.half_out) return hop_links<R, C, C>(a, s);
   if constexpr (sizeof(R) == 4) {
     if (a.half_out && a.half_out != a.half_in) return false;
