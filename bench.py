@@ -177,6 +177,35 @@ def copy_bandwidth(torch, dev, nbytes=2 << 30, reps=20):
     return 2 * nbytes / t / 1e9
 
 
+def probe_bandwidth(torch, dev, nbytes=2 << 30, reps=20):
+    """The library's streaming probe (wc_bandwidth_probe): GB/s of a plain
+    16-byte-per-lane copy (bytes read + written) and of a read-only stream,
+    the practical HBM ceiling next to the 8 TB/s peak"""
+    from gpt_amd import _lib
+    from gpt_amd import core
+
+    lib = _lib.load()
+    a = torch.empty(nbytes, dtype=torch.uint8, device=dev).fill_(1)
+    b = torch.empty_like(a)
+    out = {}
+    for kind, name, moved in [(0, "copy_GBps", 2 * nbytes), (1, "read_GBps", nbytes)]:
+        for _ in range(3):
+            _lib.check(lib.wc_bandwidth_probe(kind, a.data_ptr(), b.data_ptr(), nbytes, core.stream_ptr()))
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        e0.record(stream)
+        for _ in range(reps):
+            _lib.check(lib.wc_bandwidth_probe(kind, a.data_ptr(), b.data_ptr(), nbytes, core.stream_ptr()))
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[name] = round(moved / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9, 1)
+    del a, b
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(budget_s=12.0):
     """The oracle's C/OpenMP EO hop on the host cores, 16^3 x 32, FP64."""
     from oracle import cpu as C
@@ -329,6 +358,8 @@ def main():
 
     copy_gbs = copy_bandwidth(torch, dev) if not args.no_dhop else 0.0
     log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
+    probe = probe_bandwidth(torch, dev) if not args.no_dhop else {}
+    log(f"bench: streaming probe {probe}")
 
     # serving runs first after the Dhop section, as a serving process starts on a settled device: right
     # after the multi-RHS section has freed ~200 GB, a hipMalloc of a new serving group can block the
@@ -384,6 +415,7 @@ def main():
         "cpu_baseline": cpu,
         "variants": res,
         "copy_kernel_GBps": round(copy_gbs, 1),
+        "hbm_probe": probe,
         "cg": cg_res,
         "c1_point_source_8x8x8x8": c1_res,
         "multi_rhs": c4_res,

@@ -325,6 +325,14 @@ int wc_halo_unpack(wc_operator *op, int parity_out, int dag, int nrhs, const voi
  * NERSC configuration reader and writer (gpt_amd/io.py). */
 int wc_gauge_observables(const int fdims[4], const void *U, double *plaquette, double *link_trace, void *stream);
 
+/* HBM bandwidth probe (no counterpart in the reference; a measurement
+ * utility): kind 0 copies `bytes` from src to dst, kind 1 only reads src (and
+ * writes a 16 KB XOR fold to dst, so bytes >= 16384); both stream 16 bytes
+ * per lane with non-temporal accesses over a fixed grid.  Timed by the
+ * caller, it gives the practical bandwidth ceiling of the device next to the
+ * 8 TB/s peak.  src and dst: 16-byte aligned device buffers of `bytes`. */
+int wc_bandwidth_probe(int kind, const void *src, void *dst, long bytes, void *stream);
+
 /* message of the last failed call on this thread */
 const char *wc_last_error(void);
 
