@@ -21,7 +21,7 @@ namespace {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kProbeThreads = 256;
-constexpr int kProbeBlocks = 4096;  // 16 per CU on 256 CUs
+constexpr int kProbeBlocks = 8192;  // 32 per CU on 256 CUs (the fastest of 1024..16384 on MI355X)
 
 __global__ __launch_bounds__(kProbeThreads) void k_probe_copy(const u32x4 *__restrict__ src, u32x4 *__restrict__ dst,
                                                               long n) {

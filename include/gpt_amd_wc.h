@@ -327,7 +327,7 @@ int wc_gauge_observables(const int fdims[4], const void *U, double *plaquette, d
 
 /* HBM bandwidth probe (no counterpart in the reference; a measurement
  * utility): kind 0 copies `bytes` from src to dst, kind 1 only reads src (and
- * writes a 16 KB XOR fold to dst, so bytes >= 16384); both stream 16 bytes
+ * writes a 32 KB XOR fold to dst, so bytes >= 32768); both stream 16 bytes
  * per lane with non-temporal accesses over a fixed grid.  Timed by the
  * caller, it gives the practical bandwidth ceiling of the device next to the
  * 8 TB/s peak.  src and dst: 16-byte aligned device buffers of `bytes`. */

@@ -39,7 +39,7 @@ def test_read_folds_the_source():
     b = torch.zeros(nbytes, dtype=torch.uint8, device="cuda")
     probe(1, a, b, nbytes)
     assert torch.equal(a, keep)
-    words = b[:4 * 4096].cpu().numpy().view(np.uint32)
+    words = b[:4 * 8192].cpu().numpy().view(np.uint32)
     total = functools.reduce(np.bitwise_xor, a.cpu().numpy().view(np.uint32))
     assert functools.reduce(np.bitwise_xor, words) == total
 
