@@ -332,8 +332,8 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   int xsn = 0;
   if constexpr (xs_possible<R, RB, SI, LS>()) {
     if (xs_on(a)) {
-      const long rem = g.Vh - q0;
-      xsn = rem < (long)blockDim.x ? (int)rem : (int)blockDim.x;
+      const long rem = g.Vh - q0;  // > 0 for every workgroup of the grid; clamped all the same
+      xsn = rem <= 0 ? 0 : (rem < (long)blockDim.x ? (int)rem : (int)blockDim.x);
       if ((int)threadIdx.x < xsn) {
         const long hs = q0 + threadIdx.x;
 #pragma unroll
