@@ -627,6 +627,12 @@ def c1_measure(torch, g, dev, reps=3):
     return out
 
 
+def c4_batch(Vh, nrhs):
+    """the largest batch that divides nrhs and keeps an FP64 half field of
+    that many right-hand sides (Vh * 12 * 16 bytes each) under 4 GB"""
+    return max(d for d in range(1, nrhs + 1) if nrhs % d == 0 and Vh * 12 * 16 * d < (1 << 32))
+
+
 def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     """Multi-RHS: smoothed links (seed 4001), mass 0.08, csw 1; nrhs
     complex-Gaussian sources (seeds 4100 + j) to a true relative residual
@@ -640,8 +646,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
     grid = g.grid(fd, g.double)
     Vh = int(np.prod(fd)) // 2
-    per_rhs = Vh * 12 * 16  # FP64 half field of one RHS
-    nb = max(d for d in range(1, nrhs + 1) if nrhs % d == 0 and per_rhs * d < (1 << 32))
+    nb = c4_batch(Vh, nrhs)
     srcs = []
     for j in range(nrhs):
         gen = torch.Generator(device=dev)
