@@ -267,7 +267,8 @@ def slab_share(torch, g, dev, fd, args):
     for name, fn, nbytes in [("halo_pack", lambda: w.halo_pack(src, out=(lo, hi)),
                               (2 * 192 + 144 + 2 * 96) * face_sites),
                              ("halo_unpack", lambda: w.halo_unpack(lo, hi, dst),
-                              (2 * 96 + 144 + 4 * 192) * face_sites)]:
+                              (2 * 96 + 144 + 4 * 192) * face_sites),
+                             ("interior_hop", lambda: w.halo_interior(src, dst=dst), 1536 * Vh)]:
         for _ in range(3):
             fn()
         torch.cuda.synchronize()
@@ -288,6 +289,9 @@ def slab_share(torch, g, dev, fd, args):
         t = e0.elapsed_time(e1) / (5 * reps) * 1e-3
         out[name] = {"us_per_call": round(t * 1e6, 2), "bytes_per_call": nbytes, "GBps": round(nbytes / t / 1e9, 1)}
         del graph
+    # the rank's whole hop without the transfer: interior, then pack and unpack
+    out["interior_pack_unpack_us"] = round(sum(out[k]["us_per_call"] for k in
+                                               ("interior_hop", "halo_pack", "halo_unpack")), 2)
     del w, pairs, Us, lo, hi
     torch.cuda.empty_cache()
     return out

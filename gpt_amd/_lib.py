@@ -78,6 +78,7 @@ SIGNATURES = {
     "wc_halo_pack": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
     "wc_halo_unpack": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
     "wc_bandwidth_probe": (_I, [_I, _P, _P, _L, _P]),
+    "wc_halo_interior": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "wc_block_cg_eo2_ne": (_I, [_P, _I, _I, _P, _P, _D, _D, _L, _L, _P, _P, _P, _P]),
     "wc_solve_eo2_ne_block_cg": (_I, [_P, _I, _I, _P, _P, _D, _D, _L, _L, _P, _P, _P, _P]),
     "wc_iteration_runs": (_I, [_P, _P, _P]),

@@ -17,7 +17,8 @@
 //       above (the sender owns that link; the hop coefficient -1/2 and the
 //       boundary phase of a wrapping link are folded in as the hop folds them),
 // and unpack adds the rebuilt t-terms of the received faces to the boundary
-// slices of an output field.  On one GPU with periodic t the rank above and
+// slices of an output field.  wc_halo_interior is the rest of the rank's hop:
+// every term except those two.  On one GPU with periodic t the rank above and
 // below is the GPU itself: unpack(lo, hi) of its own pack completes a hop
 // whose wrapping t-links were left out.  Grid exchanges these faces inside
 // its stencil (lib/cgpt/lib/operators/wilson_clover.h:45 turns on its
@@ -196,9 +197,33 @@ int halo_call(wc_operator *op, int unpack, int pout, int dag, int nrhs, const vo
                                : halo_impl<float>(op, unpack, pout, dag, nrhs, fld, lo, hi, s);
 }
 
+// dst = the hop of src without the wrapping t-terms (k_hop with
+// skip_t_faces), the part of a rank's hop that needs no halo
+template <typename R>
+int interior_impl(const wc_operator *op, int pout, int dag, int nrhs, void *dst, const void *src, hipStream_t s) {
+  HopLaunch a = hop_args<R>(op, nrhs, pout, dag, 0, src, dst);
+  a.skip_t_faces = 1;
+  WC_HOP(R, a, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int wc_halo_interior(wc_operator *op, int parity_out, int dag, int nrhs, void *dst, const void *src, void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_halo_interior: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_halo_interior: dst must not alias src");
+  if (parity_out != 0 && parity_out != 1) WC_FAIL(WC_ERR_ARG, "wc_halo_interior: parity must be 0 or 1");
+  if (dag != 0 && dag != 1) WC_FAIL(WC_ERR_ARG, "wc_halo_interior: dag must be 0 or 1");
+  if (op->open_bc) WC_FAIL(WC_ERR_ARG, "wc_halo_interior: open boundaries in t leave no halo in t");
+  int rc = check_nrhs(op, nrhs, "wc_halo_interior");
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  return op->prec == WC_DOUBLE ? interior_impl<double>(op, parity_out, dag, nrhs, dst, src, s)
+                               : interior_impl<float>(op, parity_out, dag, nrhs, dst, src, s);
+}
 
 int wc_halo_pack(wc_operator *op, int parity_out, int dag, int nrhs, const void *src, void *face_lo, void *face_hi,
                  void *stream) {

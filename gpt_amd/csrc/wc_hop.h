@@ -168,6 +168,7 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
 #pragma unroll 1
   for (int dir = 0; dir < 8; ++dir) {
     const int mu = dir >> 1, fb = dir & 1;
+    if (mu == 3 && a.skip_t_faces && (fb == 0 ? ct == g.L[3] - 1 : ct == 0)) continue;
     uint32_t nb;
     if (mu == 0) {
       const int xn = fb == 0 ? (cx == g.L[0] - 1 ? 0 : cx + 1) : (cx == 0 ? g.L[0] - 1 : cx - 1);

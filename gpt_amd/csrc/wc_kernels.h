@@ -66,6 +66,8 @@ struct HopLaunch {
   HopShape shape;    // launch shape (1 <= tpb <= kMaxTpb, 1 <= rb <= max_rb, 0 <= occ <= kMaxOcc)
   int bps;           // 64-site blocks per time slice (0: not a whole number; then seg must be 1)
   const int *halt;   // non-null and *halt != 0: the launch does nothing (chunked iterations)
+  int skip_t_faces;  // 1: leave out the t-terms through the links that wrap in t (the interior
+                     // part of a T-slab rank's hop; the halo kernels add them)
 };
 
 // number of blocks a site launch uses for Vh sites

@@ -278,6 +278,22 @@ class fine_operator(matrix_operator):
         _lib.check(_lib.load().wc_halo_unpack(self.handle, dst.checkerboard().tag, int(bool(dag)), n,
                                               lo.data_ptr(), hi.data_ptr(), dst.ptr(), core.stream_ptr()))
 
+    def halo_interior(self, src, dag=False, dst=None):
+        """the hop of the half field src into parity 1 - src's without the
+        t-terms that the halo brings (wc_halo_interior); halo_unpack of
+        halo_pack(src) into the result completes the hop"""
+        self._check_halo_field(src)
+        if dst is None:
+            dst = lattice(src)
+        else:
+            self._check_halo_field(dst)
+            if dst.grid.nrhs != src.grid.nrhs or dst is src:
+                raise ValueError("dst must be a different field of the same right-hand-side count")
+        dst.checkerboard(src.checkerboard().inv())
+        _lib.check(_lib.load().wc_halo_interior(self.handle, 1 - src.checkerboard().tag, int(bool(dag)),
+                                                src.grid.nrhs, dst.ptr(), src.ptr(), core.stream_ptr()))
+        return dst
+
     def _check_halo_field(self, x):
         if x.grid.cb is not core.redblack:
             raise ValueError("the halo calls take a half (checkerboarded) field")

@@ -315,6 +315,12 @@ int wc_halo_pack(wc_operator *op, int parity_out, int dag, int nrhs, const void 
                  void *stream);
 int wc_halo_unpack(wc_operator *op, int parity_out, int dag, int nrhs, const void *face_lo, const void *face_hi,
                    void *dst, void *stream);
+/* the rest of the rank's hop: dst (parity parity_out) = the hop of src
+ * without the t-terms of the t = Lt - 1 slice's forward and the t = 0
+ * slice's backward neighbour (those wc_halo_unpack adds), so that
+ * wc_halo_interior then wc_halo_unpack of wc_halo_pack(src) is the hop of
+ * src (Meooe / MeooeDag, opcodes 2003 / 2004).  Not for open boundaries. */
+int wc_halo_interior(wc_operator *op, int parity_out, int dag, int nrhs, void *dst, const void *src, void *stream);
 
 /* Gauge observables of natural-layout FP64 links U[4][V][3][3] (site
  * ((t*Lz + z)*Ly + y)*Lx + x, periodic): the average plaquette
