@@ -268,7 +268,8 @@ def slab_share(torch, g, dev, fd, args):
                               (2 * 192 + 144 + 2 * 96) * face_sites),
                              ("halo_unpack", lambda: w.halo_unpack(lo, hi, dst),
                               (2 * 96 + 144 + 4 * 192) * face_sites),
-                             ("interior_hop", lambda: w.halo_interior(src, dst=dst), 1536 * Vh)]:
+                             ("interior_hop", lambda: w.halo_interior(src, dst=dst), 1536 * Vh),
+                             ("periodic_hop_graph", lambda: w.Meooe.mat(dst, src), 1536 * Vh)]:
         for _ in range(3):
             fn()
         torch.cuda.synchronize()
