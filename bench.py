@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--no-c1", action="store_true", help="skip the 8^4 point-source propagator")
     ap.add_argument("--c4", default="16x16x16x32:12,16x16x16x32:48,32x32x32x64:12,32x32x32x64:48",
                     help="multi-RHS cases LXxLYxLZxLT:NRHS, comma-separated ('' to skip)")
+    ap.add_argument("--slab-overlap", action="store_true",
+                    help="also time the slab share's hop with its face exchange on a second stream "
+                         "(a device copy standing in for the transfer), graph-captured")
     ap.add_argument("--shape", default=None,
                     help="hop launch shape rb,tpb,occ,seg,ls for the Dhop variants instead of tuning")
     return ap.parse_args()
@@ -293,6 +296,37 @@ def slab_share(torch, g, dev, fd, args):
     # the rank's whole hop without the transfer: interior, then pack and unpack
     out["interior_pack_unpack_us"] = round(sum(out[k]["us_per_call"] for k in
                                                ("interior_hop", "halo_pack", "halo_unpack")), 2)
+    if args.slab_overlap:
+        # pack, the faces copied on a second stream (standing in for the
+        # transfer) while the interior runs, unpack: one rank's schedule
+        comm = torch.cuda.Stream()
+        lo_in, hi_in = torch.empty_like(lo), torch.empty_like(hi)
+
+        def exchange(a, b):
+            lo_in.copy_(a)
+            hi_in.copy_(b)
+            return lo_in, hi_in
+
+        step = lambda: w.halo_hop(src, dst, exchange, comm_stream=comm)  # noqa: E731
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            for _ in range(reps):
+                step()
+        graph.replay()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        stream = torch.cuda.current_stream()
+        e0.record(stream)
+        for _ in range(5):
+            graph.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out["overlapped_halo_hop_us"] = round(e0.elapsed_time(e1) / (5 * reps) * 1e3, 2)
+        del graph
     del w, pairs, Us, lo, hi
     torch.cuda.empty_cache()
     return out

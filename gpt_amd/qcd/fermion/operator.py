@@ -294,6 +294,29 @@ class fine_operator(matrix_operator):
                                                 src.grid.nrhs, dst.ptr(), src.ptr(), core.stream_ptr()))
         return dst
 
+    def halo_hop(self, src, dst, exchange, comm_stream=None, dag=False):
+        """one T-slab rank's hop with its face exchange overlapped: pack the
+        t-faces, run exchange(lo, hi) -> (lo_in, hi_in) on comm_stream (a
+        second stream at the default priority) while the interior hop runs
+        on the current stream, then unpack the received faces into dst.
+        exchange stands for the transfer to and from the neighbouring ranks
+        (on one GPU, with periodic t, a copy of the rank's own faces)."""
+        torch = core.torch
+        main = torch.cuda.current_stream()
+        lo, hi = self.halo_pack(src, dag=dag)
+        comm = comm_stream if comm_stream is not None else torch.cuda.Stream()
+        comm.wait_stream(main)
+        with torch.cuda.stream(comm):
+            lo_in, hi_in = exchange(lo, hi)
+        lo.record_stream(comm)
+        hi.record_stream(comm)
+        self.halo_interior(src, dag=dag, dst=dst)
+        main.wait_stream(comm)
+        lo_in.record_stream(main)
+        hi_in.record_stream(main)
+        self.halo_unpack(lo_in, hi_in, dst, dag=dag)
+        return dst
+
     def _check_halo_field(self, x):
         if x.grid.cb is not core.redblack:
             raise ValueError("the halo calls take a half (checkerboarded) field")
