@@ -143,23 +143,38 @@ def gaussian_half(torch, g, grid_eo, cb, seed, dev):
     return x
 
 
+def call_stats(times):
+    """median, mean, min and max (seconds) of per-call times, and whether
+    the mean is more than 5 % away from the median (a stall inside the timed
+    calls: the median is the call's time, the mean is not)"""
+    t = np.asarray(times, dtype=np.float64)
+    med = float(np.median(t))
+    return {"median_s": med, "mean_s": float(t.mean()), "min_s": float(t.min()), "max_s": float(t.max()),
+            "mean_off_median": bool(abs(t.mean() - med) > 0.05 * med)}
+
+
 def time_meooe(torch, g, w, pairs, steps, warmup):
+    """Meooe calls on rotating (src, dst) pairs: `warmup` untimed, then
+    `steps` calls each between its own pair of HIP events on the launch
+    stream (consecutive calls share an event, so the events add nothing
+    between calls).  Returns call_stats of the per-call times and the wall
+    time per call of the whole timed loop."""
     stream = torch.cuda.current_stream()
     for i in range(warmup):
         s, d = pairs[i % len(pairs)]
         w.Meooe.mat(d, s)
     torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
     t0 = time.perf_counter()
+    ev[0].record(stream)
     for i in range(steps):
         s, d = pairs[i % len(pairs)]
         w.Meooe.mat(d, s)
-    e1.record(stream)
+        ev[i + 1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    return e0.elapsed_time(e1) / steps * 1e-3, (t1 - t0) / steps
+    per_call = [ev[i].elapsed_time(ev[i + 1]) * 1e-3 for i in range(steps)]
+    return call_stats(per_call), (t1 - t0) / steps
 
 
 def copy_bandwidth(torch, dev, nbytes=2 << 30, reps=20):
@@ -209,16 +224,29 @@ def probe_bandwidth(torch, dev, nbytes=2 << 30, reps=20):
     return out
 
 
-def cpu_baseline(budget_s=12.0):
-    """The oracle's C/OpenMP EO hop on the host cores, 16^3 x 32, FP64."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(U_dev, budget_s=15.0):
+    """The oracle's C/OpenMP EO hop on the host cores, on the headline's own
+    lattice and links (copied from the GPU), FP64, 18-real links: the same
+    workload as the headline, timed for about budget_s seconds."""
+    from gpt_amd.qcd import gauge as GA
     from oracle import cpu as C
-    from oracle import gauge as G
     from oracle import wilson_clover as W
 
-    fd = [16, 16, 16, 32]
-    U = G.smooth_links(fd, 0.3, 2001)
+    U = GA.to_numpy(U_dev)
+    fd = list(reversed(U[0].shape[:4]))
     w = W.wilson_clover(U, mass=0.08, csw_r=0.0, csw_t=0.0, boundary_phases=[1, 1, 1, -1])
     wc = C.wilson_clover_cpu(w)
+    del w, U
     rng = np.random.default_rng(2002)
     src = (rng.standard_normal((wc.Vh, 4, 3)) + 1j * rng.standard_normal((wc.Vh, 4, 3))) / np.sqrt(2)
     wc.meooe(src, 1)
@@ -228,12 +256,16 @@ def cpu_baseline(budget_s=12.0):
         wc.meooe(src, 1)
         n += 1
     dt = (time.perf_counter() - t0) / n
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(1536.0 * wc.Vh / dt / 1e9, 3), "unit": "GB/s", "cores": threads,
-            "kind": "port",
-            "sample": f"EO Dhop (Meooe odd->even), 16^3x32 FP64 18-real links, {n} calls in "
-                      f"{budget_s:.0f} s, C/OpenMP restatement (oracle/wc_cpu.c) standing in for "
-                      f"Grid's CPU path; {dt * 1e3:.2f} ms per call"}
+    threads = C.threads()
+    out = {"value": round(1536.0 * wc.Vh / dt / 1e9, 3), "unit": "GB/s", "cores": threads,
+           "kind": "port", "cpu_model": cpu_model(), "omp_threads": threads,
+           "host_cpus": os.cpu_count(),
+           "sample": f"EO Dhop (Meooe odd->even), {'x'.join(map(str, fd))} FP64 18-real links (the headline's "
+                     f"lattice and links), {n} calls in {budget_s:.0f} s on {threads} OpenMP threads, "
+                     f"C/OpenMP restatement (oracle/wc_cpu.c) standing in for Grid's CPU path; "
+                     f"{dt * 1e3:.2f} ms per call"}
+    del wc, src
+    return out
 
 
 def slab_share(torch, g, dev, fd, args):
@@ -253,7 +285,8 @@ def slab_share(torch, g, dev, fd, args):
         d.checkerboard(g.even)
         pairs.append((src, d))
     shape, _ = w.tune_hop(1, reps=3)
-    t_ev, _ = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
+    st, _ = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
+    t_ev = st["median_s"]
     face_sites = fd[0] * fd[1] * fd[2] // 2
     out = {"local_lattice": sd, "us_per_call": round(t_ev * 1e6, 2), "GBps": round(1536 * Vh / t_ev / 1e9, 2),
            "halo_bytes_per_hop": 2 * face_sites * 6 * 16,
@@ -377,12 +410,17 @@ def main():
         else:
             # 10 calls per shape: the best shapes of the single-RHS hops differ by ~1-4 %
             shape, _ = w.tune_hop(nrhs, reps=10)
-        t_ev, t_wall = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
+        st, t_wall = time_meooe(torch, g, w, pairs, args.steps, args.warmup)
+        t_ev = st["median_s"]
         real_b = prec.nbytes
         nl = 12 if comp else 18
         # compulsory bytes per output site: links once, nrhs spinors in and out
         bytes_site = (8 * nl + nrhs * 2 * 24) * real_b
         res[name] = {"us_per_call": round(t_ev * 1e6, 2), "wall_us_per_call": round(t_wall * 1e6, 2),
+                     "us_per_call_mean": round(st["mean_s"] * 1e6, 2),
+                     "us_per_call_min": round(st["min_s"] * 1e6, 2),
+                     "us_per_call_max": round(st["max_s"] * 1e6, 2),
+                     "mean_off_median": st["mean_off_median"],
                      "GBps": round(bytes_site * Vh / t_ev / 1e9, 2), "nrhs": nrhs,
                      "GFLOPps": round(1320.0 * nrhs * Vh / t_ev / 1e9, 2), "bytes_per_site": bytes_site,
                      "shape": {"rhs_per_thread": shape[0], "threads_per_block": 64 * shape[1],
@@ -423,7 +461,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(U)
         log(f"bench: cpu baseline {cpu}")
 
     head = res.get("fp64_18") or next(iter(res.values()), {"GBps": 0.0, "us_per_call": 0.0})
@@ -451,6 +489,11 @@ def main():
                    "lattice": fd, "half_sites": Vh},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        # the headline's per-call spread at the top level: value and ms_per_step are the median call
+        "headline": {k: head.get(k) for k in ("us_per_call", "us_per_call_mean", "us_per_call_min",
+                                               "us_per_call_max", "mean_off_median", "wall_us_per_call",
+                                               "shape", "bytes_per_site")},
+        "timing": "HIP events around every call on the launch stream; value and ms_per_step from the median call",
         "cpu_baseline": cpu,
         "variants": res,
         "copy_kernel_GBps": round(copy_gbs, 1),

@@ -300,7 +300,8 @@ class fine_operator(matrix_operator):
         second stream at the default priority) while the interior hop runs
         on the current stream, then unpack the received faces into dst.
         exchange stands for the transfer to and from the neighbouring ranks
-        (on one GPU, with periodic t, a copy of the rank's own faces)."""
+        (on one GPU, with periodic t, a copy of the rank's own faces).
+        dst=None returns a new field."""
         torch = core.torch
         main = torch.cuda.current_stream()
         lo, hi = self.halo_pack(src, dag=dag)
@@ -310,7 +311,7 @@ class fine_operator(matrix_operator):
             lo_in, hi_in = exchange(lo, hi)
         lo.record_stream(comm)
         hi.record_stream(comm)
-        self.halo_interior(src, dag=dag, dst=dst)
+        dst = self.halo_interior(src, dag=dag, dst=dst)
         main.wait_stream(comm)
         lo_in.record_stream(main)
         hi_in.record_stream(main)

@@ -33,8 +33,15 @@ def _lib():
         lib.wcc_mpc_ne.argtypes = [P, P, P, ctypes.c_int, P]
         lib.wcc_cg_ne.restype = ctypes.c_long
         lib.wcc_cg_ne.argtypes = [P, P, P, ctypes.c_int, ctypes.c_double, ctypes.c_long, P]
+        lib.wcc_threads.restype = ctypes.c_int
+        lib.wcc_threads.argtypes = []
         _LIB = lib
     return _LIB
+
+
+def threads():
+    """size of the OpenMP team the C restatement's loops run with"""
+    return int(_lib().wcc_threads())
 
 
 def half_indices(shape, p):

@@ -235,7 +235,6 @@ typedef struct {
   long ldim[8];
   long nblocks, block_sites;
   gr_stream *streams; /* one per 2^nd block, block index x fastest */
-  gr_stream serial;   /* seeded by the seed string alone */
 } gr_engine;
 
 /* dist: 0 = uniform [lo,hi) real, 1 = normal(mu=lo, sigma=hi) real,
@@ -285,10 +284,6 @@ gr_engine *gr_create(const char *seed, const long *fdims, const long *gdims, int
     sw[nseed - 1] = (uint64_t)hash;
     stream_init(&e->streams[b], sw, nseed, lux);
   }
-  uint64_t *ss = (uint64_t *)malloc(sizeof(uint64_t) * (ls ? ls : 1));
-  for (size_t i = 0; i < ls; ++i) ss[i] = (uint64_t)(unsigned char)seed[i];
-  stream_init(&e->serial, ss, ls, lux);
-  free(ss);
   free(sw);
   return e;
 }
@@ -319,11 +314,6 @@ void gr_fill(gr_engine *e, int dist, double lo, double hi, int n_per_site, doubl
       for (int k = 0; k < n_per_site; ++k) draw(s, dist, lo, hi, out + 2 * (site * n_per_site + k));
     }
   }
-}
-
-/* one value from the serial stream (the non-lattice draws of gpt.random) */
-void gr_serial(gr_engine *e, int dist, double lo, double hi, double out[2]) {
-  draw(&e->serial, dist, lo, hi, out);
 }
 
 /* raw lane outputs for known-answer checks of the generator itself */

@@ -35,8 +35,6 @@ def _lib():
         lib.gr_destroy.argtypes = [ctypes.c_void_p]
         lib.gr_fill.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                 ctypes.c_int, ctypes.c_void_p]
-        lib.gr_serial.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
-                                  ctypes.c_double, ctypes.c_void_p]
         lib.gr_raw.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_long, ctypes.c_void_p]
         lib.gr_sha256_words.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_void_p]
         _LIB = lib
@@ -60,7 +58,6 @@ class random:
         self.seed = seed
         self.lux = ENGINES[engine]
         self._streams = {}
-        self._serial = None
 
     def __del__(self):
         lib = _LIB
@@ -68,8 +65,6 @@ class random:
             return
         for h in self._streams.values():
             lib.gr_destroy(h[1])
-        if self._serial is not None:
-            lib.gr_destroy(self._serial)
 
     def _handle(self, g):
         key = id(g)
@@ -120,13 +115,6 @@ class random:
         if g.precision == "single":
             u = u.astype(np.complex64)
         return u
-
-    def serial_normal(self):
-        if self._serial is None:
-            self._serial = _lib().gr_create(self.seed.encode(), None, None, 0, self.lux)
-        out = np.zeros(2)
-        _lib().gr_serial(self._serial, 1, 0.0, 1.0, out.ctypes.data)
-        return out[0]
 
 
 def sha256_words(data: bytes):

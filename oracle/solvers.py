@@ -22,6 +22,12 @@ Pinned by: the 16-value correlator of tests/qcd/fermion_operators.py:193-218
 brute force on a 2^4 / 4^4 lattice against numpy.linalg.solve of the dense
 operator; and by the Schur determinant identity
 det M = det DD det CC det Mpc (tests/test_oracle_solvers.py).
+``defect_correcting`` is pinned by closed forms on a dense 2^3 x 4 operator:
+one outer step with an exact inner inverse, exactly 2^-i true residual after
+i steps with an inner inverse that returns half the correction, the zero
+source, and a 1e-45 source solved with a single-precision inner inverse
+(the normalisation), which copies of the loop with the normalisation
+removed fail.  ``block_cg``: see test_block_cg_pinned.
 """
 import numpy as np
 

@@ -22,11 +22,12 @@
  * complex double; links are double-stored per output parity: W[p][h][8][3][3]
  * with dir 2mu = U_mu(x), 2mu+1 = U_mu(x-mu)^dag, boundary phases applied.
  *
- * Pinned against the numpy oracle per site (tests/test_oracle_cpu.py), which
+ * Pinned against the numpy oracle per site (tests/test_oracle_solvers.py), which
  * in turn reproduces the reference's Grid fingerprints.
  */
 #include <complex.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -149,6 +150,17 @@ void wcc_destroy(wcc *w) {
 }
 
 long wcc_half_volume(const wcc *w) { return w->Vh; }
+
+/* threads of the OpenMP team a parallel loop of this library runs with */
+int wcc_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
+}
 
 static inline void zero_open(const wcc *w, int p, long h, cd *o) {
   if (w->open_bc) {

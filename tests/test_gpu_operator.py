@@ -105,16 +105,21 @@ def test_every_opcode_matches_oracle_fp64(small, case, compress):
             assert site_rel_err(out.to_numpy(), fn(masked)) < tol, (case, name, pin)
 
 
-def test_fp32_operator_matches_oracle(small):
+@pytest.mark.parametrize("compress", [False, True])
+def test_fp32_operator_matches_oracle(small, compress):
+    """FP32 with 18-real links and with 12-real links (the inner operator of
+    every mixed solve: third row rebuilt in FP32)"""
     gr, U, src = small
     p = CASES["baseline_clover"]
     ref = W.wilson_clover(U, **p)
-    w = g.qcd.fermion.wilson_clover(U, p, precision=g.single)
+    w = g.qcd.fermion.wilson_clover(U, p, precision=g.single, compress_links=compress)
     s = to_dev(src.astype(np.complex64), w.F_grid)
     assert site_rel_err(w(s).to_numpy().astype(np.complex128), ref.M(src)) < 2e-6
     sh = half_dev(src.astype(np.complex64), w.F_grid_eo, g.odd)
     got = w.Meooe(sh).to_numpy().astype(np.complex128)
     assert site_rel_err(got, ref.Meooe(ref._mask(src, 1), 1)) < 2e-6
+    got = w.Meooe.adj()(sh).to_numpy().astype(np.complex128)
+    assert site_rel_err(got, ref.MeooeDag(ref._mask(src, 1), 1)) < 2e-6
 
 
 def test_grid_fingerprints_on_gpu(kats):

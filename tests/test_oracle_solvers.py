@@ -179,3 +179,77 @@ def test_eo1_ne_and_eo2_kappa_ne_pinned(tiny_op):
         assert np.linalg.norm(x - x_exact) / np.linalg.norm(x_exact) < 1e-11, name
         its[name] = it
     assert abs(its["eo2_kappa_ne"] - its["eo2_ne"]) <= 1
+
+
+def _dense_small():
+    g = R.grid([2, 2, 2, 4])
+    U = G.random_links(g, R.random("dc"), scale=0.7)
+    w = W.wilson_clover(U, mass=0.2, csw_r=1.0, csw_t=1.2, boundary_phases=[1, 1, 1, -1])
+    M = _dense(w.M, g.shape)
+    b = R.random("dc_rhs").vspincolor(g)
+    return g, M, b
+
+
+def _mat(M, shape):
+    return lambda x: (M @ x.reshape(-1)).reshape(x.shape)
+
+
+def test_defect_correcting_pinned():
+    """defect_correcting (lib/gpt/algorithms/inverter/defect_correcting.py:74-142)
+    pinned by closed forms: an exact inner inverse converges in one outer
+    step; an inner inverse that returns half the exact correction leaves a
+    true residual of exactly 2^-i after i steps, so the loop stops at the first
+    i with 2^-i < eps; a zero source returns the zero guess at once."""
+    g, M, b = _dense_small()
+    A = _mat(M, g.shape)
+    x_exact = np.linalg.solve(M, b.reshape(-1)).reshape(b.shape)
+    exact = lambda s: np.linalg.solve(M, s.reshape(-1)).reshape(s.shape)  # noqa: E731
+    x, it, conv = S.defect_correcting(A, exact, b, eps=1e-12)
+    assert conv and it == 1
+    assert np.linalg.norm(x - x_exact) / np.linalg.norm(x_exact) < 1e-13
+    x, it, conv = S.defect_correcting(A, lambda s: 0.5 * exact(s), b, eps=1e-6)
+    assert conv and it == 20  # 2^-19 = 1.9e-6 >= 1e-6 > 2^-20
+    res = np.linalg.norm(b - A(x)) / np.linalg.norm(b)
+    assert abs(res / 2.0**-20 - 1.0) < 1e-6
+    x, it, conv = S.defect_correcting(A, exact, np.zeros_like(b), eps=1e-12)
+    assert conv and it == 0 and np.abs(x).max() == 0.0
+    x, it, conv = S.defect_correcting(A, lambda s: 0.5 * exact(s), b, eps=1e-12, maxiter=5)
+    assert not conv and it == 5
+
+
+def _single_precision_inner(M):
+    M32 = M.astype(np.complex64)
+    return lambda s: np.linalg.solve(M32, s.reshape(-1).astype(np.complex64)).astype(np.complex128).reshape(s.shape)
+
+
+def test_defect_correcting_normalises_the_inner_source():
+    """the defect is divided by |src| before the inner solve and the
+    correction multiplied back (defect_correcting.py:124-134): with a source
+    of magnitude 1e-45, below the single-precision range, a single-precision
+    inner solve still converges to 1e-12 in a few outer steps"""
+    g, M, b = _dense_small()
+    A = _mat(M, g.shape)
+    tiny = b * 1e-45
+    x, it, conv = S.defect_correcting(A, _single_precision_inner(M), tiny, eps=1e-12)
+    assert conv and 2 <= it <= 6
+    assert np.linalg.norm(tiny - A(x)) / np.linalg.norm(tiny) < 1e-12
+
+
+def test_defect_correcting_planted_normalisation_mistakes_fail():
+    """the normalisation test has teeth: the same loop with the division by
+    |src| dropped, or the multiplication back dropped, does not converge"""
+    import inspect
+    import textwrap
+
+    src_code = textwrap.dedent(inspect.getsource(S.defect_correcting))
+    g, M, b = _dense_small()
+    A = _mat(M, g.shape)
+    tiny = b * 1e-45
+    for old, new in [("s = s / n2src**0.5", "s = s"),
+                     ("inner_inv(s) * n2src**0.5", "inner_inv(s)")]:
+        assert old in src_code
+        ns = dict(S.__dict__)
+        exec(src_code.replace(old, new), ns)
+        with np.errstate(all="ignore"):  # the broken loops overflow single precision
+            x, it, conv = ns["defect_correcting"](A, _single_precision_inner(M), tiny, eps=1e-12, maxiter=20)
+        assert not conv, old
