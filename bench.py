@@ -55,7 +55,8 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the CG solve measurement")
     ap.add_argument("--only", default=None,
                     help="comma-separated variants to run: fp64_18, fp64_12, fp32_18, fp32_12, "
-                         "fp64_18_rhs12, fp32_18_rhs12")
+                         "fp64_18_rhs12, fp32_18_rhs12, fp16_rhs (the FP16-storage multi-RHS hop on the "
+                         "matrix cores and on the vector ALUs)")
     ap.add_argument("--cg-lattice", default="24,24,24,48")
     ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
     ap.add_argument("--nrhs", type=int, default=12)
@@ -175,6 +176,49 @@ def time_meooe(torch, g, w, pairs, steps, warmup):
     t1 = time.perf_counter()
     per_call = [ev[i].elapsed_time(ev[i + 1]) * 1e-3 for i in range(steps)]
     return call_stats(per_call), (t1 - t0) / steps
+
+
+def hop16_measure(torch, g, dev, U, fd, nrhs=12, steps=20, warmup=5):
+    """The FP16-storage Meooe of nrhs right-hand sides, as the 16-bit inner
+    solve runs it (FP32 operator with 12-real links, FP16 spinors in and
+    out), on the matrix cores (k_hop_mma) and on the vector ALUs (k_hop):
+    per-call HIP events, median.  Compulsory bytes per output site: 8 links
+    of 12 FP32 reals and nrhs spinors of 24 FP16 reals in and out."""
+    Vh = int(np.prod(fd)) // 2
+    w = g.qcd.fermion.wilson_clover(U, mass=0.05, csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0,
+                                    boundary_phases=[1, 1, 1, -1], precision=g.single, compress_links=True,
+                                    n_rhs=nrhs)
+    w.tune_hop(nrhs, reps=5)  # the FP32 shape the vector form of the 16-bit hop uses
+    pairs = []
+    for k in range(2):
+        s = gaussian_half(torch, g, w.F_grid_eo, g.odd, 2102 + k, dev)
+        pairs.append((s.data.to(torch.float16), torch.empty(s.data.shape, dtype=torch.float16, device=dev)))
+    bytes_site = 8 * 12 * 4 + nrhs * 2 * 24 * 2
+    out = {"nrhs": nrhs, "bytes_per_site": bytes_site, "lattice": fd}
+    stream = torch.cuda.current_stream()
+    for form, mma in (("mma", True), ("vector", False)):
+        w.set_hop_mma(mma)
+        for i in range(warmup):
+            s16, d16 = pairs[i % 2]
+            w.hop_s16(s16, 1, dst=d16)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(stream)
+        for i in range(steps):
+            s16, d16 = pairs[i % 2]
+            w.hop_s16(s16, 1, dst=d16)
+            ev[i + 1].record(stream)
+        torch.cuda.synchronize()
+        st = call_stats([ev[i].elapsed_time(ev[i + 1]) * 1e-3 for i in range(steps)])
+        t = st["median_s"]
+        out[form] = {"us_per_call": round(t * 1e6, 2), "us_per_call_min": round(st["min_s"] * 1e6, 2),
+                     "GBps": round(bytes_site * Vh / t / 1e9, 2),
+                     "frac_hbm_peak": round(bytes_site * Vh / t / 1e9 / HBM_PEAK_GBS, 4),
+                     "GFLOPps": round(1320.0 * nrhs * Vh / t / 1e9, 2)}
+    w.set_hop_mma("auto")
+    out["mma_speedup"] = round(out["vector"]["us_per_call"] / out["mma"]["us_per_call"], 3)
+    w.release()
+    return out
 
 
 def copy_bandwidth(torch, dev, nbytes=2 << 30, reps=20):
@@ -432,6 +476,14 @@ def main():
     if "fp64_18" in res and fd[3] % 8 == 0:
         res["fp64_18_slab_share"] = slab_share(torch, g, dev, fd, args)
         log(f"bench: fp64_18_slab_share: {res['fp64_18_slab_share']}")
+    hop16 = []
+    if not args.no_dhop and (not args.only or "fp16_rhs" in args.only.split(",")):
+        for dims in ([int(x) for x in args.cg_lattice.split(",")], fd):
+            Ud = U if dims == fd else smooth_links_device(torch, dims, 0.3, 3001, dev)
+            hop16.append(hop16_measure(torch, g, dev, Ud, dims, nrhs=args.nrhs))
+            log(f"bench: fp16 hop {dims} N={args.nrhs}: {hop16[-1]}")
+            del Ud
+            torch.cuda.empty_cache()
 
     copy_gbs = copy_bandwidth(torch, dev) if not args.no_dhop else 0.0
     log(f"bench: copy kernel {copy_gbs:.0f} GB/s")
@@ -496,6 +548,7 @@ def main():
         "timing": "HIP events around every call on the launch stream; value and ms_per_step from the median call",
         "cpu_baseline": cpu,
         "variants": res,
+        "fp16_hop_multi_rhs": hop16,
         "copy_kernel_GBps": round(copy_gbs, 1),
         "hbm_probe": probe,
         "cg": cg_res,
@@ -611,7 +664,22 @@ def cg_measure(torch, g, dev, fd=(24, 24, 24, 48), mass=0.05, nrhs=12, tol=1e-10
         th = float(np.median(lat))
         out[key] = {"iterations": mh.iterations, "reliable_updates": mh.reliable_updates,
                     "true_residual": max(res), "rel_error_vs_fp64_solution": max(err),
-                    "s_per_batch": th, "solves_per_s": float(nr / th)}
+                    "s_per_batch": th, "solves_per_s": float(nr / th),
+                    "inner_hop_on_matrix_cores": mh.inner_operator(op_).hop_mma(nr)}
+        log(f"bench: cg {key} {out[key]}")
+    # (d') the 12-RHS FP16 solve with the inner hops forced onto each form
+    for key, form in (("mixed_multi_half_matrix_cores", True), ("mixed_multi_half_vector_alu", False)):
+        mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half", mma=form)
+        slvh = wN.propagator(inv.preconditioned(pc.eo2_ne(), mh))
+        g(slvh * S)
+        lat = []
+        for _ in range(2):
+            X, t = timed(lambda: g(slvh * S))
+            lat.append(t)
+        th = float(np.median(lat))
+        out[key] = {"iterations": mh.iterations, "reliable_updates": mh.reliable_updates,
+                    "max_true_residual": max(true_residual(g, wN, X, S)), "s_per_batch": th,
+                    "solves_per_s": float(nrhs / th)}
         log(f"bench: cg {key} {out[key]}")
     # (e) FP16 storage on 48 right-hand sides at once (the 12 above and 36 more from the same generator)
     n48 = 48
@@ -759,6 +827,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     slvh = wN.propagator(inv.preconditioned(pc.eo2_ne(), mh))
     batches = [pack(srcs[i:i + nb]) for i in range(0, nrhs, nb)]
     g(slvh * batches[0])  # warm-up (tuning, graphs)
+    out["inner_hop_on_matrix_cores"] = mh.inner_operator(wN).hop_mma(nb)
 
     def run():
         res, itr = [], []

@@ -139,7 +139,8 @@ class cg_mixed:
         from .. import core as _core
 
         p = {"eps": 1e-10, "maxiter": 100000, "inner_precision": _core.single, "delta": 0.01,
-             "inner_storage": None, "inner_compress_links": True, "fail_if_not_converged": False}
+             "inner_storage": None, "inner_compress_links": True, "fail_if_not_converged": False,
+             "mma": "auto"}
         p.update(params or {})
         p.update(kw)
         self.params = p
@@ -153,6 +154,9 @@ class cg_mixed:
             raise ValueError("16-bit inner storage needs inner_precision=single")
         self.inner_storage = p["inner_storage"]
         self.inner_compress_links = bool(p["inner_compress_links"])
+        # FP16-storage inner hops of 4k right-hand sides on the matrix cores:
+        # True, False or "auto" (the faster, timed at first use)
+        self.mma = p["mma"] if p["mma"] == "auto" else bool(p["mma"])
         self.fail_if_not_converged = p["fail_if_not_converged"]
         self.iterations = []
         self.true_residual = []
@@ -168,6 +172,7 @@ class cg_mixed:
                 inner = type(op)(op.name, op.U, params, op.daggered, self.inner_precision)
             else:
                 inner = op.converted(self.inner_precision)
+            inner.set_hop_mma(self.mma)
             self._inner[key] = (op, inner)
         return self._inner[key][1]
 

@@ -3,7 +3,11 @@
 // dispatch, the Schur-complement compositions and the CG solve loops.
 #include <hip/hip_runtime.h>
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -20,6 +24,7 @@
 thread_local std::string g_err;
 std::mutex g_shape_mu;
 std::map<ShapeKey, wck::HopShape> g_shape_cache;
+std::map<ShapeKey, int> g_mma_cache;
 
 // gamma matrices of gpt's chiral basis (lib/gpt/core/gamma.py:28-42)
 typedef std::complex<double> cdbl;
@@ -686,6 +691,83 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
   return WC_OK;
 }
 
+// ------------------------------------------------- persisted tuner results
+// gpt keeps tuned parameters across processes as one JSON file per tuned
+// object, .gpt_auto_tune/<sha256(tag)>.json in the working directory, written
+// after tuning and read when the object is set up
+// (lib/gpt/core/auto_tune.py:20-37,99-136).  The hop tuner does the same:
+// one file per (device, lattice, precision, link format, class of
+// right-hand-side counts) in the directory WC_TUNE_CACHE (default
+// .gpt_amd_auto_tune; empty or 0: nothing read or written), holding
+// {"tag": ..., "params": [rb, tpb, occ, seg, ls], "ms": ...}.  A shape read
+// back is used only if it is valid for the operator; every shape gives the
+// same bits, so a stale file can cost speed, never correctness.
+static std::string tune_cache_dir() {
+  const char *e = getenv("WC_TUNE_CACHE");
+  if (!e) return ".gpt_amd_auto_tune";
+  if (!e[0] || (e[0] == '0' && !e[1])) return "";
+  return e;
+}
+
+// tag and file name: device architecture and CU count, lattice, precision,
+// link format, right-hand-side class
+static std::string tune_tag(const wc_operator *op, int nrhs) {
+  int dev = 0;
+  hipDeviceProp_t pr{};
+  std::string arch = "unknown";
+  int cus = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&pr, dev) == hipSuccess) {
+    arch = pr.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    cus = pr.multiProcessorCount;
+  }
+  char b[256];
+  snprintf(b, sizeof b, "hop_%s_cu%d_%dx%dx%dx%d_%s_links%d_rhs%d", arch.c_str(), cus, op->g.L[0], op->g.L[1],
+           op->g.L[2], op->g.L[3], op->prec == WC_DOUBLE ? "f64" : "f32", 2 * op->nl, nrhs_class(nrhs));
+  std::string t = b;
+  for (char &c : t)
+    if (!isalnum((unsigned char)c) && c != '_' && c != '-') c = '_';
+  return t;
+}
+
+static bool load_tuned(const wc_operator *op, int nrhs, wck::HopShape &sh) {
+  const std::string dir = tune_cache_dir();
+  if (dir.empty()) return false;
+  const std::string tag = tune_tag(op, nrhs);
+  FILE *f = fopen((dir + "/" + tag + ".json").c_str(), "rt");
+  if (!f) return false;
+  char buf[1024];
+  const size_t n = fread(buf, 1, sizeof buf - 1, f);
+  fclose(f);
+  buf[n] = 0;
+  const std::string txt = buf;
+  if (txt.find("\"tag\": \"" + tag + "\"") == std::string::npos) return false;
+  const size_t at = txt.find("\"params\": [");
+  if (at == std::string::npos) return false;
+  wck::HopShape x;
+  if (sscanf(buf + at, "\"params\": [%d, %d, %d, %d, %d]", &x.rb, &x.tpb, &x.occ, &x.seg, &x.ls) != 5) return false;
+  if (!shape_valid(op, x) || x.rb > nrhs || !stage_fits(op, nrhs, x)) return false;
+  sh = x;
+  return true;
+}
+
+static void save_tuned(const wc_operator *op, int nrhs, const wck::HopShape &sh, float ms) {
+  const std::string dir = tune_cache_dir();
+  if (dir.empty()) return;
+  mkdir(dir.c_str(), 0755);  // may exist already
+  const std::string tag = tune_tag(op, nrhs);
+  const std::string fn = dir + "/" + tag + ".json";
+  char tmp_name[64];
+  snprintf(tmp_name, sizeof tmp_name, ".tmp%ld", (long)getpid());
+  const std::string tmp = fn + tmp_name;
+  FILE *f = fopen(tmp.c_str(), "wt");
+  if (!f) return;  // an unwritable directory only costs the next process a re-tune
+  fprintf(f, "{\"tag\": \"%s\", \"params\": [%d, %d, %d, %d, %d], \"ms\": %.6g}\n", tag.c_str(), sh.rb, sh.tpb,
+          sh.occ, sh.seg, sh.ls, (double)ms);
+  const bool ok = fclose(f) == 0;
+  if (!ok || rename(tmp.c_str(), fn.c_str()) != 0) remove(tmp.c_str());
+}
+
 static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *ms_out) {
   wck::HopShape best;
   float ms = 0.0f;
@@ -698,23 +780,75 @@ static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *m
     std::lock_guard<std::mutex> lk(g_shape_mu);
     g_shape_cache[shape_key(op, nrhs)] = best;
   }
+  save_tuned(op, nrhs, best, ms);
   if (ms_out) *ms_out = ms;
+  return WC_OK;
+}
+
+// Automatic hop form (op->mma == 2) of an FP32 operator whose 16-bit inner
+// hops can run on the matrix cores (a multiple of 4 right-hand sides): the
+// normal operator's first 16-bit hop (clover epilogue, FP16 in and out) is
+// timed in both forms and the faster is kept process-wide for the lattice,
+// link format and class of right-hand-side counts.  Both forms give results
+// within the FP16 storage's rounding of each other, so the choice changes
+// speed, not the solve's tolerance.  Scratch: tmp[0..1].
+static int pick_mma(wc_operator *op, int N, hipStream_t s) {
+  if (op->mma != 2 || op->prec != WC_SINGLE || N < 4 || N % 4) return WC_OK;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    if (g_mma_cache.count(shape_key(op, N))) return WC_OK;
+  }
+  int rc = ensure_tmp(op, 2, N);
+  if (rc) return rc;
+  wck::zero<float>(op->g.Vh * 12 * N, op->tmp[0], s);
+  wck::zero<float>(op->g.Vh * 12 * N, op->tmp[1], s);
+  hipEvent_t e0, e1;
+  WC_HIP(hipEventCreate(&e0));
+  WC_HIP(hipEventCreate(&e1));
+  float ms[2] = {0.0f, 0.0f};
+  for (int form = 0; form < 2; ++form) {
+    HopLaunch a = hop_args<float>(op, N, 0, 0, op->has_clover ? wck::EPI_CLOV : 0, op->tmp[1], op->tmp[0]);
+    a.clov = op->has_clover ? op->Ai[0] : nullptr;
+    a.half_in = a.half_out = kS16Half;
+    a.mma = form;
+    WC_HOP(float, a, s);  // warm-up (first launch of the form)
+    WC_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < 3; ++k) WC_HOP(float, a, s);
+    WC_HIP(hipEventRecord(e1, s));
+    WC_HIP(hipEventSynchronize(e1));
+    WC_HIP(hipEventElapsedTime(&ms[form], e0, e1));
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  g_mma_cache[shape_key(op, N)] = ms[1] < ms[0] ? 1 : 0;
   return WC_OK;
 }
 
 // Solvers tune the hop on first use for lattices where launch shapes matter
 // (>= 2^17 site-RHS pairs; WC_AUTOTUNE=0 turns it off), once per lattice,
-// precision, link format and class of right-hand-side counts.
+// precision, link format and class of right-hand-side counts, unless an
+// earlier process left its result in the tuner's directory.
 int maybe_tune(wc_operator *op, int nrhs, hipStream_t s) {
-  if (op->shape_set[nrhs]) return WC_OK;
-  {
-    std::lock_guard<std::mutex> lk(g_shape_mu);
-    if (g_shape_cache.count(shape_key(op, nrhs))) return WC_OK;
-  }
   const char *env = getenv("WC_AUTOTUNE");
   if (env && env[0] == '0') return WC_OK;
   if (op->g.Vh * (long)nrhs < (1L << 17)) return WC_OK;
-  return tune_hop(op, nrhs, 3, s, nullptr);
+  bool have = op->shape_set[nrhs];
+  if (!have) {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    have = g_shape_cache.count(shape_key(op, nrhs)) > 0;
+  }
+  if (have) return pick_mma(op, nrhs, s);
+  wck::HopShape sh;
+  int rc = WC_OK;
+  if (load_tuned(op, nrhs, sh)) {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shape_cache[shape_key(op, nrhs)] = sh;
+  } else {
+    rc = tune_hop(op, nrhs, 3, s, nullptr);
+  }
+  if (rc == WC_OK) rc = pick_mma(op, nrhs, s);
+  return rc;
 }
 
 extern "C" {
@@ -831,6 +965,68 @@ int wc_apply_mpc_block(wc_operator *op, int parity, int which, int nrhs, void *d
 
 int wc_apply_mpc(wc_operator *op, int parity, int which, void *dst, const void *src, void *stream) {
   return wc_apply_mpc_block(op, parity, which, 1, dst, src, stream);
+}
+
+int wc_set_hop_mma(wc_operator *op, int mode) {
+  if (!op) WC_FAIL(WC_ERR_ARG, "wc_set_hop_mma: null operator");
+  if (mode < 0 || mode > 2) WC_FAIL(WC_ERR_ARG, "wc_set_hop_mma: mode %d is not 0, 1 or 2", mode);
+  op->mma = mode;
+  return WC_OK;
+}
+
+int wc_get_hop_mma(const wc_operator *op, int nrhs, int *form) {
+  if (!op || !form) WC_FAIL(WC_ERR_ARG, "wc_get_hop_mma: null argument");
+  if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "wc_get_hop_mma: nrhs = %d must be in [1, 64]", nrhs);
+  HopLaunch a{};
+  a.mma = mma_for(op, nrhs);
+  a.half_in = a.half_out = kS16Half;
+  a.nrhs = nrhs;
+  a.g = op->g;
+  a.nl = op->nl;
+  *form = op->prec == WC_SINGLE && wck::hop_mma_applies(a) ? 1 : 0;
+  return WC_OK;
+}
+
+// the kernels' 16-bit code of a storage argument of the *_s16 entry points
+// (FP32 operator only), in *code
+static int s16_code(const wc_operator *op, int storage, const char *fn, int *code) {
+  if (op->prec != WC_SINGLE) WC_FAIL(WC_ERR_ARG, "%s: 16-bit storage needs an FP32 operator", fn);
+  *code = storage_code(storage);
+  if (*code != kS16Half && *code != kS16Bf16)
+    WC_FAIL(WC_ERR_ARG, "%s: storage %d is not WC_STORAGE_HALF or WC_STORAGE_BF16", fn, storage);
+  return WC_OK;
+}
+
+int wc_apply_block_s16(wc_operator *op, int opcode, int nrhs, int storage, void *dst, const void *src, int src_cb,
+                       void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply_block_s16: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply_block_s16: dst must not alias src");
+  if (opcode != 2003 && opcode != 2004) WC_FAIL(WC_ERR_ARG, "wc_apply_block_s16: opcode %d is not 2003 or 2004", opcode);
+  if (src_cb != 0 && src_cb != 1) WC_FAIL(WC_ERR_ARG, "wc_apply_block_s16: src_cb must be 0 or 1");
+  int rc = check_nrhs(op, nrhs, "wc_apply_block_s16");
+  if (rc) return rc;
+  int code = 0;
+  if ((rc = s16_code(op, storage, "wc_apply_block_s16", &code))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  HopLaunch a = hop_args<float>(op, nrhs, 1 - src_cb, opcode == 2004, 0, src, dst);
+  a.half_in = a.half_out = code;
+  WC_HOP(float, a, s);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+int wc_apply_mpc_block_s16(wc_operator *op, int parity, int nrhs, int storage, void *dst, const void *src,
+                           void *stream) {
+  if (!op || !dst || !src) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc_block_s16: null argument");
+  if (dst == src) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc_block_s16: dst must not alias src");
+  if (parity != 0 && parity != 1) WC_FAIL(WC_ERR_ARG, "wc_apply_mpc_block_s16: parity must be 0 or 1");
+  int rc = check_nrhs(op, nrhs, "wc_apply_mpc_block_s16");
+  if (rc) return rc;
+  int code = 0;
+  if ((rc = s16_code(op, storage, "wc_apply_mpc_block_s16", &code))) return rc;
+  // as the inner CG applies it: the last hop also forms Re<src, dst> per
+  // 64-site block (unused here)
+  return mpc_impl<float>(op, nrhs, parity, 2, dst, src, src, (hipStream_t)stream, nullptr, nullptr, code);
 }
 
 int wc_cg_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *x, const void *b, double eps, long maxiter,

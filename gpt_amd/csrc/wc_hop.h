@@ -424,11 +424,12 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
 
 // epilogues instantiated per storage: all of them in the working precision;
 // with 16-bit in/y/dotv those the normal operator of the 16-bit inner solve
-// uses (CLOV, AXPY to 16-bit, CLOV|AXPY|DOT to FP32)
+// uses (CLOV, AXPY to 16-bit, CLOV|AXPY|DOT to FP32) and the bare hop
+// (16-bit to 16-bit, wc_apply_block_s16)
 template <typename R, typename SI, typename SO> constexpr bool epi_instantiated(int e) {
   using C = typename c2<R>::t;
   if (std::is_same<SI, C>::value && std::is_same<SO, C>::value) return true;
-  if (is16<SI>() && std::is_same<SO, SI>::value) return e == EPI_CLOV || e == EPI_AXPY;
+  if (is16<SI>() && std::is_same<SO, SI>::value) return e == 0 || e == EPI_CLOV || e == EPI_AXPY;
   if (is16<SI>() && std::is_same<SO, C>::value) return e == (EPI_CLOV | EPI_AXPY | EPI_DOT);
   return false;
 }

@@ -113,6 +113,10 @@ struct wc_operator {
   wck::HopShape shape[65];      // hop launch shape per right-hand-side count
   bool shape_set[65] = {};      // shape chosen (by the tuner or the caller)
   int chunk = 0;                // iterations per run: 0 automatic, 1 one at a time, else at most this many
+  // FP16-storage hops of 4k right-hand sides: 0 vector ALUs, 1 matrix cores,
+  // 2 automatic (the faster of the two, timed per lattice and class of
+  // right-hand-side counts at first use; the vector form until then)
+  int mma = 2;
   const int *halt_cur = nullptr;  // halt word of the run being enqueued (null outside runs)
   IterRun run;                  // chunked iterations of this operator's CG and mixed solves
 };
@@ -278,6 +282,16 @@ inline wck::HopShape shape_for(const wc_operator *op, int nrhs) {
   return default_shape(op, nrhs);
 }
 
+// the hop form timed faster for the operator's lattice, link format and
+// class of right-hand-side counts (process-wide, like the shapes), or -1
+extern std::map<ShapeKey, int> g_mma_cache;
+inline int mma_for(const wc_operator *op, int nrhs) {
+  if (op->mma != 2) return op->mma;
+  std::lock_guard<std::mutex> lk(g_shape_mu);
+  auto it = g_mma_cache.find(shape_key(op, nrhs));
+  return it != g_mma_cache.end() ? it->second : 0;
+}
+
 // ------------------------------------------------------------ compositions
 template <typename R>
 inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, int epi, const void *in, void *out) {
@@ -303,6 +317,7 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
   a.shape = shape_for(op, nrhs);
   a.bps = op->bps;
   a.halt = op->halt_cur;
+  a.mma = mma_for(op, nrhs);
   return a;
 }
 
@@ -328,18 +343,16 @@ inline void site_op(const wc_operator *op, int nrhs, int p, int inv, const void 
 // per-RHS partials of Re<dotv, out>.  tD_in (optional, which 0 or 2): a
 // field already holding A_D^-1 in.  half (FP32, which 2): in, dotv, tD_in
 // and the intermediates are stored as 16-bit c2h, out in FP32.
-// Scratch: tmp[0..2].
+// Scratch: tmp[0..2] (mpc_impl), or the fields given to mpc_run: tD (which
+// 0, 2), tC (always), tY (which 2; may be tD, which is dead by the time tY
+// is written); dots and partials only with dotv.
 template <typename R>
-inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
-                    hipStream_t s, const void *tD_in = nullptr, double *partials = nullptr, int half = 0) {
-  if (!partials) partials = op->partials;
+inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
+                   hipStream_t s, void *tD, void *tC, void *tY, const void *tD_in, double *dots, double *partials,
+                   int half) {
   if (half && (sizeof(R) != 4 || which != 2))
     WC_FAIL(WC_ERR_ARG, "mpc: 16-bit storage is for the FP32 normal operator only");
   const int pc = 1 - pd;
-  int rc = ensure_tmp(op, 3, nrhs);
-  if (rc) return rc;
-  if ((rc = ensure_red(op, nrhs))) return rc;
-  void *tD = op->tmp[0], *tC = op->tmp[1], *tY = op->tmp[2];
   const void *clovC = op->has_clover ? op->Ai[pc] : nullptr;
   const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
   const void *src = in;
@@ -358,10 +371,10 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
                               dst_mpc);
     b.y = src;
     b.dotv = dotv;
-    b.dots = op->dots;
+    b.dots = dots;
     b.half_in = b.half_out = half;
     WC_HOP(R, b, s);
-    if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s, op->halt_cur);
+    if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, dots, partials, s, op->halt_cur);
     src = dst_mpc;
   }
   if (which == 1 || which == 2) {
@@ -374,13 +387,25 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
     b.clov = clovD;
     b.y = src;
     b.dotv = dotv;
-    b.dots = op->dots;
+    b.dots = dots;
     b.half_in = half;  // the product itself stays in the working precision
     WC_HOP(R, b, s);
-    if (dotv) wck::dot_blocks(op->g, nrhs, op->dots, partials, s, op->halt_cur);
+    if (dotv) wck::dot_blocks(op->g, nrhs, dots, partials, s, op->halt_cur);
   }
   WC_LAUNCHED();
   return WC_OK;
+}
+
+template <typename R>
+inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
+                    hipStream_t s, const void *tD_in = nullptr, double *partials = nullptr, int half = 0) {
+  int rc = ensure_tmp(op, 3, nrhs);
+  if (rc) return rc;
+  if ((rc = ensure_red(op, nrhs))) return rc;
+  // the operator's partials exist only from here on
+  if (!partials) partials = op->partials;
+  return mpc_run<R>(op, nrhs, pd, which, out, in, dotv, s, op->tmp[0], op->tmp[1], op->tmp[2], tD_in, op->dots,
+                    partials, half);
 }
 
 // per-RHS |x|^2 into dev_out[0..nrhs)

@@ -68,6 +68,7 @@ struct HopLaunch {
   const int *halt;   // non-null and *halt != 0: the launch does nothing (chunked iterations)
   int skip_t_faces;  // 1: leave out the t-terms through the links that wrap in t (the interior
                      // part of a T-slab rank's hop; the halo kernels add them)
+  int mma;           // 1: FP16-storage hops of 4k right-hand sides run on the matrix cores (k_hop_mma)
 };
 
 // number of blocks a site launch uses for Vh sites
@@ -86,6 +87,11 @@ constexpr int kRedBlocks = 1024;  // fixed grid of the reduction kernels
 
 // returns false (nothing launched) for a shape outside the limits above
 template <typename R> bool hop(const HopLaunch &a, hipStream_t s);
+// the matrix-core form of the FP16-storage hop: hop_mma_applies says whether
+// the launch has that form (a.mma set, c2h input, N a multiple of 4, ...);
+// hop_mma launches it (false: not applicable or epilogue not instantiated)
+bool hop_mma_applies(const HopLaunch &a);
+bool hop_mma(const HopLaunch &a, hipStream_t s);
 
 // out = A in (blocks != null) or diag * in on parity p, for nrhs right-hand
 // sides; zeroes the open boundary; optional axpy (out = y - A in).  half:

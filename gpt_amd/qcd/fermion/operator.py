@@ -14,6 +14,8 @@ The arithmetic happens in the HIP library behind include/gpt_amd_wc.h.
 import copy as _copy
 import ctypes
 
+import torch
+
 from ... import _lib
 from ... import core
 from ...core import even, odd, none, full, redblack, lattice, matrix_operator
@@ -67,6 +69,7 @@ class interface:
             _lib.check(lib.wc_set_mass(self.obj, float(params["mass"]), stream))
             _lib.check(lib.wc_update(self.obj, Ut.data_ptr(), stream))
             _lib.check(lib.wc_set_iteration_chunk(self.obj, 0))  # a new operator's default schedule
+            _lib.check(lib.wc_set_hop_mma(self.obj, 2))  # and its default (automatic) hop form
         else:
             p = _lib.wc_params()
             for i in range(4):
@@ -241,6 +244,41 @@ class fine_operator(matrix_operator):
             return
         sh = (ctypes.c_int * 5)(*[int(x) for x in shape])
         _lib.check(_lib.load().wc_set_hop_shape(self.handle, n, ctypes.addressof(sh)))
+
+    # FP16-storage hops of 4k right-hand sides on the matrix cores
+    # (wc_set_hop_mma): True, False, or "auto" (the default: the faster form,
+    # timed at first use on lattices where the solvers tune)
+    def set_hop_mma(self, mode):
+        code = 2 if mode == "auto" else int(bool(mode))
+        _lib.check(_lib.load().wc_set_hop_mma(self.handle, code))
+
+    def hop_mma(self, n_rhs=None):
+        """True if FP16-storage hops of n_rhs right-hand sides run on the matrix cores now"""
+        n = self.n_rhs if n_rhs is None else int(n_rhs)
+        f = ctypes.c_int()
+        _lib.check(_lib.load().wc_get_hop_mma(self.handle, n, ctypes.byref(f)))
+        return bool(f.value)
+
+    # the inner solve's pieces with 16-bit storage (wc_apply_block_s16,
+    # wc_apply_mpc_block_s16), on torch tensors holding (re, im) pairs of
+    # float16 (storage "half") or bfloat16 ("bf16") in a half field's layout
+    def hop_s16(self, src, src_cb, dag=False, storage="half", n_rhs=None, dst=None):
+        """Meooe (MeooeDag) of the 16-bit half field src of parity src_cb"""
+        n = self.n_rhs if n_rhs is None else int(n_rhs)
+        dst = torch.empty_like(src) if dst is None else dst
+        _lib.check(_lib.load().wc_apply_block_s16(self.handle, 2004 if dag else 2003, n,
+                                                  _lib.STORAGE_CODES[storage], dst.data_ptr(), src.data_ptr(),
+                                                  int(src_cb), core.stream_ptr()))
+        return dst
+
+    def mpc_ne_s16(self, src, parity, storage="half", n_rhs=None):
+        """Mpc^dag Mpc (D = parity) of the 16-bit src with 16-bit Schur
+        intermediates, as an FP32 tensor of the same layout"""
+        n = self.n_rhs if n_rhs is None else int(n_rhs)
+        dst = torch.empty(src.shape, dtype=torch.float32, device=src.device)
+        _lib.check(_lib.load().wc_apply_mpc_block_s16(self.handle, int(parity), n, _lib.STORAGE_CODES[storage],
+                                                      dst.data_ptr(), src.data_ptr(), core.stream_ptr()))
+        return dst
 
     # halo of a T-slab decomposition (wc_halo_pack / wc_halo_unpack): with
     # this operator's lattice as one rank's slab, the spin-projected t-faces

@@ -236,6 +236,35 @@ int wc_tune_hop(wc_operator *op, int nrhs, int reps, int *shape, float *ms_per_c
 int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape);
 int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape);
 
+/* Matrix cores for the FP16-storage hops (no cgpt counterpart: Grid's 5-d
+ * multi-RHS operator, lib/cgpt/lib/operators/wilson_clover.h:58-66, runs its
+ * stencil on the vector units).  The hops of a mixed solve or batch with
+ * WC_STORAGE_HALF on a multiple of 4 right-hand sides can run as
+ * v_mfma_f32_16x16x32_f16 products of the FP16 spinors with the FP32 links
+ * split into FP16 hi + lo parts (FP32 accumulation).  mode 0: vector ALUs,
+ * 1: matrix cores, 2 (the default): the faster of the two, timed when the
+ * solvers tune (first use of a lattice and class of right-hand-side counts,
+ * Vh * nrhs >= 2^17), the vector form until then.  The two forms differ by
+ * the order of their FP32 sums, not bitwise. */
+int wc_set_hop_mma(wc_operator *op, int mode);
+/* the form an FP16-storage hop of nrhs right-hand sides takes now: 1 matrix
+ * cores, 0 vector ALUs */
+int wc_get_hop_mma(const wc_operator *op, int nrhs, int *form);
+
+/* The inner operator's pieces with 16-bit storage, as the mixed solve's inner
+ * CG runs them (no cgpt counterpart: gpt's inner operator of
+ * lib/gpt/algorithms/inverter/mixed_precision.py:23-32 is a plain FP32 one).
+ * FP32 operator only; storage WC_STORAGE_HALF or WC_STORAGE_BF16, each
+ * complex value two 16-bit floats (re, im) in the layout of a half field.
+ *   wc_apply_block_s16     opcode 2003 / 2004 (Meooe / MeooeDag) of the
+ *                          16-bit src (parity src_cb) into the 16-bit dst;
+ *   wc_apply_mpc_block_s16 dst (FP32) = Mpc^dag Mpc src (16-bit src, Schur
+ *                          intermediates stored in 16 bits), D = parity. */
+int wc_apply_block_s16(wc_operator *op, int opcode, int nrhs, int storage, void *dst, const void *src, int src_cb,
+                       void *stream);
+int wc_apply_mpc_block_s16(wc_operator *op, int parity, int nrhs, int storage, void *dst, const void *src,
+                           void *stream);
+
 /* CG iteration schedule (no cgpt counterpart: gpt's cg.py reads the
  * residual on the host after every iteration).  The CG, solve, mixed and
  * batch entry points run their iterations in runs of up to `chunk`: a run is
