@@ -92,6 +92,7 @@ SIGNATURES = {
     "wc_get_hop_mma": (_I, [_P, _I, _P]),
     "wc_apply_block_s16": (_I, [_P, _I, _I, _I, _P, _P, _I, _P]),
     "wc_apply_mpc_block_s16": (_I, [_P, _I, _I, _I, _P, _P, _P]),
+    "wc_apply_mpc_block_noalloc": (_I, [_P, _I, _I, _I, _P, _P, _P]),
     "wc_last_error": (ctypes.c_char_p, []),
 }
 

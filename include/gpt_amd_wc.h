@@ -161,6 +161,18 @@ int wc_solve_eo2_ne_block(wc_operator *op, int parity, int nrhs, void *dst, cons
                           long maxiter, double *history, long *iterations, int *converged, double *r2,
                           double *b2, void *stream);
 
+/* wc_apply_mpc_block without allocating.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  The same kernels
+ * and bitwise the same results as wc_apply_mpc_block, with the scratch half
+ * fields in a fixed 64 MiB array per GPU (held from the library's first use on
+ * that GPU until the process ends): which 0 and 2 need two fields, which 1
+ * one, of 12 * Vh * nrhs complex values; a larger need is refused
+ * (WC_ERR_ARG) before anything runs.  Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (other streams or threads) race. */
+int wc_apply_mpc_block_noalloc(wc_operator *op, int parity, int which, int nrhs, void *dst, const void *src,
+                               void *stream);
+
 /* Mixed-precision eo2_ne solve with reliable updates (no cgpt counterpart;
  * the precision pattern of lib/gpt/algorithms/inverter/defect_correcting.py
  * and mixed_precision.py, done inside one CG).  `op` is the FP64 operator,
