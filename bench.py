@@ -60,7 +60,9 @@ def parse():
     ap.add_argument("--cg-lattice", default="24,24,24,48")
     ap.add_argument("--no-dhop", action="store_true", help="skip the Dhop measurement")
     ap.add_argument("--nrhs", type=int, default=12)
-    ap.add_argument("--serve", type=int, default=12, help="serving stream: number of requests (0: skip)")
+    ap.add_argument("--serve", type=int, default=12,
+                    help="serving stream: requests of the capacity burst (0: skip serving)")
+    ap.add_argument("--serve-poisson", type=int, default=200, help="serving stream: Poisson-arrival requests")
     ap.add_argument("--no-c1", action="store_true", help="skip the 8^4 point-source propagator")
     ap.add_argument("--c4", default="16x16x16x32:12,16x16x16x32:48,32x32x32x64:12,32x32x32x64:48",
                     help="multi-RHS cases LXxLYxLZxLT:NRHS, comma-separated ('' to skip)")
@@ -495,7 +497,7 @@ def main():
     # device for ~2 s (HIP API trace), which lands in the Poisson p99; 15 s later it does not
     serve_res = None
     if args.serve > 0:
-        serve_res = serve_measure(torch, g, dev, args.serve)
+        serve_res = serve_measure(torch, g, dev, args.serve, npoisson=args.serve_poisson)
 
     cg_res = None
     if not args.no_cg:
@@ -777,10 +779,15 @@ def c1_measure(torch, g, dev, reps=3):
     return out
 
 
-def c4_batch(Vh, nrhs):
-    """the largest batch that divides nrhs and keeps an FP64 half field of
-    that many right-hand sides (Vh * 12 * 16 bytes each) under 4 GB"""
-    return max(d for d in range(1, nrhs + 1) if nrhs % d == 0 and Vh * 12 * 16 * d < (1 << 32))
+def c4_batch(Vh, nrhs, lt, free_bytes):
+    """the largest batch that divides nrhs such that a t-slice of an FP64 half
+    field of that many right-hand sides fits one buffer descriptor (4 GB; the
+    hop addresses larger fields per t-slice) and about 20 FP64 half fields of
+    it (the mixed solve's 13, the sources, solutions and block CG's) fit in
+    85 % of the free device memory"""
+    field = Vh * 12 * 16
+    return max(d for d in range(1, nrhs + 1)
+               if nrhs % d == 0 and field // lt * d < (1 << 32) and 20 * field * d <= 0.85 * free_bytes)
 
 
 def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
@@ -788,7 +795,8 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     complex-Gaussian sources (seeds 4100 + j) to a true relative residual
     tol.  FP64 eo2_ne CG one RHS at a time against the mixed-precision
     multi-RHS solve with 16-bit inner storage, in equal batches as large as
-    the 4 GB offset window of the kernels allows for an FP64 half field."""
+    the device memory allows (c4_batch; a batch that fails to allocate is
+    run again as two)."""
     from gpt_amd.qcd.fermion.operator import pack
 
     U = smooth_links_device(torch, fd, 0.3, seed, dev)
@@ -796,7 +804,8 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
     grid = g.grid(fd, g.double)
     Vh = int(np.prod(fd)) // 2
-    nb = c4_batch(Vh, nrhs)
+    torch.cuda.empty_cache()
+    nb = c4_batch(Vh, nrhs, fd[3], torch.cuda.mem_get_info()[0])
     srcs = []
     for j in range(nrhs):
         gen = torch.Generator(device=dev)
@@ -822,56 +831,88 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     t64 = float(np.median(lat))
     out["fp64_sequential"] = {"eps_normal_equation": eps, "iterations": its, "s_per_solve": t64,
                               "solves_per_s": 1.0 / t64}
-    wN = g.qcd.fermion.wilson_clover(U, prm, n_rhs=nb)
-    mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half")
-    slvh = wN.propagator(inv.preconditioned(pc.eo2_ne(), mh))
-    batches = [pack(srcs[i:i + nb]) for i in range(0, nrhs, nb)]
-    g(slvh * batches[0])  # warm-up (tuning, graphs)
-    out["inner_hop_on_matrix_cores"] = mh.inner_operator(wN).hop_mma(nb)
+    def batched(nb):
+        """the 16-bit mixed multi-RHS solve and FP64 block CG in batches of nb"""
+        res_out = {}
+        wN = g.qcd.fermion.wilson_clover(U, prm, n_rhs=nb)
+        mh = inv.cg_mixed(eps=tol, maxiter=20000, inner_storage="half")
+        try:
+            slvh = wN.propagator(inv.preconditioned(pc.eo2_ne(), mh))
+            batches = [pack(srcs[i:i + nb]) for i in range(0, nrhs, nb)]
+            g(slvh * batches[0])  # warm-up (tuning, graphs)
+            res_out["inner_hop_on_matrix_cores"] = mh.inner_operator(wN).hop_mma(nb)
 
-    def run():
-        res, itr = [], []
-        for S in batches:
-            X = g(slvh * S)
-            res += true_residual(g, wN, X, S)
-            itr += list(mh.iterations)
-        return res, itr
+            def run():
+                res, itr = [], []
+                for S in batches:
+                    X = g(slvh * S)
+                    res += true_residual(g, wN, X, S)
+                    itr += list(mh.iterations)
+                return res, itr
 
-    (res, itr), t = _timed(torch, run)
-    out["mixed_multi_half"] = {"iterations_max": max(itr), "reliable_updates": mh.reliable_updates,
-                               "max_true_residual": max(res), "s_total": t, "solves_per_s": nrhs / t}
-    out["speedup_vs_fp64_sequential"] = out["mixed_multi_half"]["solves_per_s"] * t64
-    # FP64 block CG (block_cg.py) on the same batches, eps as the FP64 CG's
-    wB = wN
-    bcg = inv.block_cg(eps=eps, maxiter=20000)
-    slvb = wB.propagator(inv.preconditioned(pc.eo2_ne(), bcg))
-    g(slvb * batches[0])
+            (res, itr), t = _timed(torch, run)
+            res_out["mixed_multi_half"] = {"iterations_max": max(itr), "reliable_updates": mh.reliable_updates,
+                                           "max_true_residual": max(res), "s_total": t,
+                                           "solves_per_s": nrhs / t}
+            res_out["speedup_vs_fp64_sequential"] = res_out["mixed_multi_half"]["solves_per_s"] * t64
+            # the mixed solve's operators and scratch are not needed by block CG
+            mh.inner_operator(wN).release()
+            wN.release()
+            torch.cuda.empty_cache()
+            # FP64 block CG (block_cg.py) on the same batches, eps as the FP64 CG's
+            wB = g.qcd.fermion.wilson_clover(U, prm, n_rhs=nb)
+            bcg = inv.block_cg(eps=eps, maxiter=20000)
+            slvb = wB.propagator(inv.preconditioned(pc.eo2_ne(), bcg))
+            g(slvb * batches[0])
 
-    def run_block():
-        res, itr = [], []
-        for S in batches:
-            X = g(slvb * S)
-            res += true_residual(g, wB, X, S)
-            itr.append(bcg.iterations)
-        return res, itr
+            def run_block():
+                res, itr = [], []
+                for S in batches:
+                    X = g(slvb * S)
+                    res += true_residual(g, wB, X, S)
+                    itr.append(bcg.iterations)
+                return res, itr
 
-    (res, itr), t = _timed(torch, run_block)
-    out["block_cg_fp64"] = {"iterations": itr, "max_true_residual": max(res), "s_total": t,
-                            "solves_per_s": nrhs / t}
+            (res, itr), t = _timed(torch, run_block)
+            res_out["block_cg_fp64"] = {"iterations": itr, "max_true_residual": max(res), "s_total": t,
+                                        "solves_per_s": nrhs / t}
+            wB.release()
+        finally:
+            for op in (wN, mh.inner_operator(wN)):
+                op.release()
+            torch.cuda.empty_cache()
+        return res_out
+
+    while True:
+        try:
+            out.update(batched(nb))
+            break
+        except RuntimeError as e:
+            if nb == 1 or "hipMalloc" not in str(e) and "out of memory" not in str(e):
+                raise
+            log(f"bench: multi-rhs {fd} N={nrhs}: batch {nb} does not fit ({e}); halving")
+            nb = max(d for d in range(1, nb // 2 + 1) if nrhs % d == 0)
+            out["batch"] = nb
+            out["batch_fallback"] = True
     log(f"bench: multi-rhs {fd} N={nrhs}: {out}")
-    for op in (w, wN, mh.inner_operator(wN)):
-        op.release()
-    del w, wN, batches, srcs, U
+    w.release()
+    del w, srcs, U
     return out
 
 
-def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10, iteration_chunk=None):
+def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10, iteration_chunk=None,
+                  npoisson=200):
     """Serving stream: requests over volumes {8^4, 16^3x32, 24^3x48, 32^3x64},
     3 resident fields per volume (seeds 5101..5103), mass U[0.02, 0.2], 1..12
-    Gaussian right-hand sides, tolerance tol on the true residual.  First all
-    nreq requests at once (capacity, right-hand sides per second), then nreq
+    Gaussian right-hand sides, tolerance tol on the true residual.  First nreq
+    requests at once (capacity, right-hand sides per second), then npoisson
     new ones with Poisson arrivals at load x that capacity: p50/p99 latency
-    and solves/s."""
+    and solves/s.  One group per field serves every mass (per-slot clover
+    inverses); the time to set a group up (operators and batch) is reported
+    apart from the latencies, which include it.  Sources come from a pool of
+    12 Gaussian fields per volume (a request takes distinct ones), and
+    results are dropped once checked, so that hundreds of requests fit in
+    memory."""
     from gpt_amd import serve
 
     vols = [[8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64]]
@@ -883,19 +924,23 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
+    pool = []
+    for fd in vols:
+        grid = g.grid(fd, g.double)
+        fs = []
+        for _ in range(12):
+            x = g.lattice(grid)
+            x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
+            fs.append(x)
+        pool.append(fs)
 
     def make(n):
         reqs = []
         for _ in range(n):
             vi, k = int(rng.integers(4)), int(rng.integers(3))
             m, nr = float(rng.uniform(0.02, 0.2)), int(rng.integers(1, 13))
-            grid = g.grid(vols[vi], g.double)
-            srcs = []
-            for _ in range(nr):
-                x = g.lattice(grid)
-                x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
-                srcs.append(x)
-            reqs.append(serve.request((vi, k), m, srcs, tol=tol, maxiter=50000))
+            pick = rng.permutation(12)[:nr]
+            reqs.append(serve.request((vi, k), m, [pool[vi][int(i)] for i in pick], tol=tol, maxiter=50000))
         return reqs
 
     def summary(done, T):
@@ -907,27 +952,40 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
                 "all_converged": all(all(r.converged) for r in done)}
 
     def progress(r):
-        log(f"bench: serving request {r.id} done: {len(r.sources)} rhs, field {r.field}, mass {r.mass:.3f}, "
-            f"latency {r.latency:.3f} s, iterations {max(r.iterations)}")
+        if r.id % 20 == 0:
+            log(f"bench: serving request {r.id} done: {len(r.sources)} rhs, field {r.field}, mass {r.mass:.3f}, "
+                f"latency {r.latency:.3f} s, iterations {max(r.iterations)}")
+        r.results = [None] * len(r.results)  # checked (true residual, converged): free the fields
+
+    def setups(ts):
+        ts = np.asarray(ts) if len(ts) else np.zeros(1)
+        return {"groups": int(len(ts)), "median_s": float(np.median(ts)), "max_s": float(ts.max()),
+                "total_s": float(ts.sum())}
 
     reqs = make(nreq)
     t0 = time.perf_counter()
     done = srv.run([(0.0, r) for r in reqs], on_done=progress)
     cap = summary(done, time.perf_counter() - t0)
+    cap["group_setup"] = setups(srv.setup_times)
     log(f"bench: serving capacity {cap}")
     del reqs, done
-    reqs = make(nreq)
+    srv.setup_times = []
+    peak_cap = srv.peak_groups
+    srv.peak_groups = 0
+    reqs = make(npoisson)
     mean_rhs = float(np.mean([len(r.sources) for r in reqs]))
     lam = load * cap["solves_per_s"] / mean_rhs
-    arrivals = np.cumsum(rng.exponential(1.0 / lam, nreq))
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, npoisson))
     t0 = time.perf_counter()
     done = srv.run(list(zip(arrivals.tolist(), reqs)), on_done=progress)
     pois = summary(done, time.perf_counter() - t0)
     pois["arrival_rate_per_s"] = lam
+    pois["group_setup"] = setups(srv.setup_times)
     log(f"bench: serving poisson {pois}")
     out = {"nslots": nslots, "streams": len(srv.streams), "load": load, "capacity": cap, "poisson": pois,
-           "peak_groups": srv.peak_groups, "hbm_budget_gb": srv.hbm_budget / 2**30,
-           "scheduler_passes_waiting_for_hbm": srv.waited_for_memory}
+           "peak_groups_capacity": peak_cap, "peak_groups_poisson": srv.peak_groups,
+           "resident_fields": len(srv.fields), "grouping": "one group per resident field, any mass",
+           "hbm_budget_gb": srv.hbm_budget / 2**30, "scheduler_passes_waiting_for_hbm": srv.waited_for_memory}
     srv.close()
     return out
 

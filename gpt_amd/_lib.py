@@ -85,6 +85,7 @@ SIGNATURES = {
     "wc_batch_create": (_I, [_P, _P, _I, _I, _D, _I, _P, ctypes.POINTER(_P)]),
     "wc_batch_destroy": (_I, [_P]),
     "wc_batch_admit": (_I, [_P, _I, _P, _P, _D, _L]),
+    "wc_batch_admit_mass": (_I, [_P, _I, _D, _P, _P, _D, _L]),
     "wc_batch_step": (_I, [_P, _I, _P, _P]),
     "wc_batch_extract": (_I, [_P, _I, _P, _P, _P, _P]),
     "wc_batch_slots": (_I, [_P, _P, _P]),

@@ -69,6 +69,28 @@ static wck::CloverParams clover_params(const wc_operator *op) {
   return cp;
 }
 
+// The clover-inverse blocks of both parities for another mass than the
+// operator's (a slot of a multi-mass batch): the operator's clover term with
+// the diagonal of `mass`, built and inverted as build_all builds the
+// operator's own, into ai0 / ai1 in the operator's precision (and, if a0 / a1
+// are given, the blocks themselves).  Synchronises (checks the pivots).
+int build_mass_blocks(const wc_operator *op, double mass, void *ai0, void *ai1, void *a0, void *a1, int *err,
+                      hipStream_t s) {
+  wck::CloverParams cp = clover_params(op);
+  cp.diag = mass + 1.0 + 3.0 * op->p.nu / op->p.xi_0;  // 1/(2 kappa), as set_mass_values
+  WC_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+  if (op->prec == WC_DOUBLE)
+    wck::build_clover<double>(op->g, op->Unat, cp, a0, a1, ai0, ai1, err, s);
+  else
+    wck::build_clover<float>(op->g, op->Unat, cp, a0, a1, ai0, ai1, err, s);
+  WC_LAUNCHED();
+  int herr = 0;
+  WC_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  WC_HIP(hipStreamSynchronize(s));
+  if (herr) WC_FAIL(WC_ERR_NUMERIC, "clover block is singular or near-singular at some site (mass %g)", mass);
+  return WC_OK;
+}
+
 static int build_all(wc_operator *op, hipStream_t s) {
   const Geom &g = op->g;
   if (op->prec == WC_DOUBLE)
@@ -479,8 +501,8 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
         wck::resid_d2f(n, N, bs, nullptr, rl, hi->partials, s);
       } else {
         void *aD = hi->tmp[0];
-        wck::fold_site_clover(hi->g, N, pd, hi->has_clover ? hi->Ai[pd] : nullptr, 1.0 / hi->diag,
-                              h16 ? sth + 13 * N : nullptr, xl, xh, aD, hi->open_bc ? 1 : 0, s);
+        wck::fold_site_clover(hi->g, N, pd, clov_inv(hi, pd), 1.0 / hi->diag,
+                              h16 ? sth + 13 * N : nullptr, xl, xh, aD, hi->open_bc ? 1 : 0, s, clov_rstride(hi));
         if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s, aD))) return rc2;
         wck::resid_d2f(n, N, bs, mp, rl, hi->partials, s);
       }
@@ -625,7 +647,7 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
   site_op<double>(hi, N, pc, 1, half(src, pc), tC, nullptr, s);
   {
     HopLaunch a = hop_args<double>(hi, N, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
-    a.clov = hi->has_clover ? hi->Ai[pc] : nullptr;
+    a.clov = clov_inv(hi, pc);
     a.y = tC;
     WC_HOP(double, a, s);
   }

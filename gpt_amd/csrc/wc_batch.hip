@@ -92,6 +92,42 @@ struct wc_batch {
   bool dirty = true;
   std::vector<double> cref, clast, rate;
   std::vector<long> since;
+  // several masses (wc_batch_admit_mass): each slot has its own clover-inverse
+  // blocks of both parities, FP64 for the outer operator (ai_hi) and in the
+  // inner precision (ai_lo), slot r of parity p at ai_*[p] + r * 36 Vh
+  // complex values; while the batch works, the operators use them through
+  // their `cov` (every clover-inverse application indexes them by slot)
+  bool mm = false;
+  void *ai_hi[2] = {nullptr, nullptr}, *ai_lo[2] = {nullptr, nullptr};
+  int *err = nullptr;
+  std::vector<double> mass;
+  ClovSets cov_hi, cov_lo;
+};
+
+// Sets the operators' per-slot clover inverses for the duration of a batch
+// call (all slots: slot < 0; one slot's blocks as a single set: slot >= 0)
+// and clears them again.
+struct SlotClover {
+  wc_batch *b;
+  ClovSets one_hi, one_lo;
+  SlotClover(wc_batch *b_, int slot) : b(b_) {
+    if (!b->mm) return;
+    if (slot < 0) {
+      b->hi->cov = &b->cov_hi;
+      b->lo->cov = &b->cov_lo;
+      return;
+    }
+    const size_t sh = (size_t)36 * b->hi->g.Vh * b->hi->csz, sl = (size_t)36 * b->lo->g.Vh * b->lo->csz;
+    for (int p = 0; p < 2; ++p) {
+      one_hi.Ai[p] = (const char *)b->ai_hi[p] + (size_t)slot * sh;
+      one_lo.Ai[p] = (const char *)b->ai_lo[p] + (size_t)slot * sl;
+    }
+    b->hi->cov = &one_hi;
+    b->lo->cov = &one_lo;
+  }
+  ~SlotClover() {
+    if (b->mm) b->hi->cov = b->lo->cov = nullptr;
+  }
 };
 
 static size_t hbytes(const wc_operator *op) { return half_bytes(op); }
@@ -102,9 +138,61 @@ static void free_batch(wc_batch *b) {
   for (void *q : b->src) (void)hipFree(q);
   (void)hipFree(b->st);
   (void)hipFree(b->partials);
+  for (int p = 0; p < 2; ++p) {
+    (void)hipFree(b->ai_hi[p]);
+    (void)hipFree(b->ai_lo[p]);
+  }
+  (void)hipFree(b->err);
   if (b->host) (void)hipHostFree(b->host);
   free_run(b->run);
   delete b;
+}
+
+// slot's clover inverses for `mass` (both operators; the operator's own
+// blocks are copied when the mass is the operator's and it has a clover term)
+static int set_slot_mass(wc_batch *b, int slot, double mass) {
+  wc_operator *hi = b->hi, *lo = b->lo;
+  const size_t sh = (size_t)36 * hi->g.Vh * hi->csz, sl = (size_t)36 * lo->g.Vh * lo->csz;
+  char *h0 = (char *)b->ai_hi[0] + (size_t)slot * sh, *h1 = (char *)b->ai_hi[1] + (size_t)slot * sh;
+  char *l0 = (char *)b->ai_lo[0] + (size_t)slot * sl, *l1 = (char *)b->ai_lo[1] + (size_t)slot * sl;
+  int rc;
+  if (mass == hi->mass && hi->has_clover && lo->has_clover && mass == lo->mass) {
+    WC_HIP(hipMemcpyAsync(h0, hi->Ai[0], sh, hipMemcpyDeviceToDevice, b->s));
+    WC_HIP(hipMemcpyAsync(h1, hi->Ai[1], sh, hipMemcpyDeviceToDevice, b->s));
+    WC_HIP(hipMemcpyAsync(l0, lo->Ai[0], sl, hipMemcpyDeviceToDevice, b->s));
+    WC_HIP(hipMemcpyAsync(l1, lo->Ai[1], sl, hipMemcpyDeviceToDevice, b->s));
+  } else {
+    if ((rc = build_mass_blocks(hi, mass, h0, h1, nullptr, nullptr, b->err, b->s))) return rc;
+    if ((rc = build_mass_blocks(lo, mass, l0, l1, nullptr, nullptr, b->err, b->s))) return rc;
+  }
+  b->mass[slot] = mass;
+  return WC_OK;
+}
+
+// switch a batch to one clover-inverse set per slot, every slot at the
+// operator's mass to begin with
+static int enable_masses(wc_batch *b) {
+  if (b->mm) return WC_OK;
+  wc_operator *hi = b->hi, *lo = b->lo;
+  const int N = b->N;
+  const size_t sh = (size_t)36 * hi->g.Vh * hi->csz, sl = (size_t)36 * lo->g.Vh * lo->csz;
+  for (int p = 0; p < 2; ++p) {
+    WC_HIP(hipMalloc(&b->ai_hi[p], sh * N));
+    WC_HIP(hipMalloc(&b->ai_lo[p], sl * N));
+  }
+  WC_HIP(hipMalloc((void **)&b->err, sizeof(int)));
+  b->mass.assign(N, hi->mass);
+  int rc;
+  for (int i = 0; i < N; ++i)
+    if ((rc = set_slot_mass(b, i, hi->mass))) return rc;
+  for (int p = 0; p < 2; ++p) {
+    b->cov_hi.Ai[p] = b->ai_hi[p];
+    b->cov_lo.Ai[p] = b->ai_lo[p];
+  }
+  b->cov_hi.rstride = 36 * hi->g.Vh;
+  b->cov_lo.rstride = 36 * lo->g.Vh;
+  b->mm = true;
+  return WC_OK;
 }
 
 // single-RHS FP64 temporaries of the outer operator (tmp[3..8]) and of the
@@ -136,8 +224,9 @@ static int batch_reliable(wc_batch *b) {
     // applied by the inner operator (into ml, free between iterations)
     if ((rc = ensure_tmp(hi, 3, N))) return rc;
     void *aD = hi->tmp[0];
-    wck::fold_site_clover(hi->g, N, b->pd, hi->has_clover ? hi->Ai[b->pd] : nullptr, 1.0 / hi->diag,
-                          b->h16 ? b->st + 13 * N : nullptr, b->xl, b->xh, aD, hi->open_bc ? 1 : 0, s);
+    wck::fold_site_clover(hi->g, N, b->pd, clov_inv(hi, b->pd), 1.0 / hi->diag,
+                          b->h16 ? b->st + 13 * N : nullptr, b->xl, b->xh, aD, hi->open_bc ? 1 : 0, s,
+                          clov_rstride(hi));
     if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, aD, b->partials))) return rc;
     wck::resid_d2f(n, N, b->bs, b->mp, b->rl, b->partials, s);
     wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
@@ -300,16 +389,25 @@ int wc_batch_destroy(wc_batch *b) {
   return WC_OK;
 }
 
-int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter) {
+}  // extern "C"
+
+// admission of one right-hand side; mass (optional) is the slot's own mass
+static int admit_impl(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter,
+                      const double *mass) {
   if (!b || !src) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: null argument");
   if (slot < 0 || slot >= b->N) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: slot %d out of range [0, %d)", slot, b->N);
   if (b->occupied[slot]) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: slot %d is occupied", slot);
   if (!(tol > 0.0) || maxiter < 0) WC_FAIL(WC_ERR_ARG, "wc_batch_admit: need tol > 0 and maxiter >= 0");
+  if (mass && !std::isfinite(*mass)) WC_FAIL(WC_ERR_ARG, "wc_batch_admit_mass: mass %g", *mass);
   wc_operator *hi = b->hi, *lo = b->lo;
   const int N = b->N, pd = b->pd, pc = 1 - pd;
   hipStream_t s = b->s;
   int rc;
   if ((rc = ensure_single(b))) return rc;
+  // a mass other than the operators' makes the batch one of several masses
+  if (mass && *mass != hi->mass && (rc = enable_masses(b))) return rc;
+  if (b->mm && (rc = set_slot_mass(b, slot, mass ? *mass : hi->mass))) return rc;
+  SlotClover use(b, slot);
   const size_t hb = hbytes(hi);
   const long n1 = hi->g.Vh * 12;
   void *t1 = hi->tmp[3], *bs1 = hi->tmp[4], *y1 = hi->tmp[5], *mp1 = hi->tmp[6], *rs1 = hi->tmp[7],
@@ -324,9 +422,24 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
     a.y = half(src, pd);
     WC_HOP(double, a, s);
   }
-  // y = L^-1 guess = A_D guess_D, or 0
-  if (guess) site_op<double>(hi, 1, pd, 0, half(guess, pd), y1, nullptr, s);
-  else wck::zero<double>(n1, y1, s);
+  // y = L^-1 guess = A_D guess_D, or 0 (A_D of the slot's mass)
+  if (guess && b->mm && b->mass[slot] != hi->mass) {
+    void *aD = nullptr;
+    const size_t sh = (size_t)36 * hi->g.Vh * hi->csz;
+    WC_HIP(hipMalloc(&aD, sh));
+    char *h0 = (char *)b->ai_hi[0] + (size_t)slot * sh, *h1 = (char *)b->ai_hi[1] + (size_t)slot * sh;
+    rc = build_mass_blocks(hi, b->mass[slot], h0, h1, pd == 0 ? aD : nullptr, pd == 1 ? aD : nullptr, b->err, s);
+    if (rc == WC_OK)
+      wck::site_clover<double>(hi->g, 1, pd, aD, 0.0, half(guess, pd), y1, nullptr, hi->open_bc ? 1 : 0, s);
+    const hipError_t e = hipStreamSynchronize(s);
+    (void)hipFree(aD);
+    if (rc) return rc;
+    if (e != hipSuccess) WC_FAIL(WC_ERR_HIP, "wc_batch_admit: %s", hipGetErrorString(e));
+  } else if (guess) {
+    site_op<double>(hi, 1, pd, 0, half(guess, pd), y1, nullptr, s);
+  } else {
+    wck::zero<double>(n1, y1, s);
+  }
   // r_s = b_s - Mpc y ; r_h = Mpc^dag r_s
   if ((rc = mpc_impl<double>(hi, 1, pd, 0, mp1, y1, nullptr, s))) return rc;
   wck::axpy<double>(n1, -1.0, 0.0, mp1, bs1, rs1, s);
@@ -405,9 +518,21 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
   return WC_OK;
 }
 
+extern "C" {
+
+int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter) {
+  return admit_impl(b, slot, src, guess, tol, maxiter, nullptr);
+}
+
+int wc_batch_admit_mass(wc_batch *b, int slot, double mass, const void *src, const void *guess, double tol,
+                        long maxiter) {
+  return admit_impl(b, slot, src, guess, tol, maxiter, &mass);
+}
+
 int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished) {
   if (!b) WC_FAIL(WC_ERR_ARG, "wc_batch_step: null batch");
   if (iters < 0) WC_FAIL(WC_ERR_ARG, "wc_batch_step: iters must be >= 0");
+  SlotClover use(b, -1);
   const int N = b->N;
   hipStream_t s = b->s;
   wc_operator *lo = b->lo;
@@ -501,6 +626,7 @@ int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double 
   if (!b || !dst) WC_FAIL(WC_ERR_ARG, "wc_batch_extract: null argument");
   if (slot < 0 || slot >= b->N) WC_FAIL(WC_ERR_ARG, "wc_batch_extract: slot %d out of range", slot);
   if (!b->occupied[slot] || !b->finished[slot]) WC_FAIL(WC_ERR_ARG, "wc_batch_extract: slot %d has no finished solve", slot);
+  SlotClover use(b, slot);
   wc_operator *hi = b->hi;
   const int N = b->N, pd = b->pd, pc = 1 - pd;
   hipStream_t s = b->s;
@@ -516,7 +642,7 @@ int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double 
   site_op<double>(hi, 1, pc, 1, half(b->src[slot], pc), t1, nullptr, s);
   {
     HopLaunch a = hop_args<double>(hi, 1, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
-    a.clov = hi->has_clover ? hi->Ai[pc] : nullptr;
+    a.clov = clov_inv(hi, pc);
     a.y = t1;
     WC_HOP(double, a, s);
   }

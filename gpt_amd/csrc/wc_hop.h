@@ -155,7 +155,12 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
   const Geom &g = a.g;
   const uint32_t N = (uint32_t)a.nrhs;
   const uint32_t hh = (uint32_t)h;
-  const wc_rsrc in = make_rsrc(a.in, 12u * (uint32_t)g.Vh * N * (uint32_t)sizeof(SI));
+  // a field beyond one descriptor's 4 GB (a.big): each direction's gathers
+  // go through a descriptor based at the t-slice that holds the neighbours
+  // (a wave's sites share their t: a 64-site block lies in one slice), so
+  // the per-lane offsets stay within a slice
+  const wc_rsrc in_all = make_rsrc(a.in, 12u * (uint32_t)g.Vh * N * (uint32_t)sizeof(SI));
+  const uint32_t slice = (uint32_t)(g.Vh / g.L[3]);
   // a wave's sites are consecutive and lie in at most two aligned 64-site
   // blocks: its links are addressed from the first active lane's block
   // (unused, and removed by the compiler, when the links are staged)
@@ -204,7 +209,17 @@ __device__ __forceinline__ void hop_site(const HopLaunch &a, long h, const int r
       }
     }
     if (!staged) {
-      const uint32_t vs = (uint32_t)aidx_site(nb, 12) * N;
+      wc_rsrc in = in_all;
+      uint32_t vs;
+      if (a.big) {
+        const int tn = mu != 3 ? ct : (fb == 0 ? (ct == g.L[3] - 1 ? 0 : ct + 1) : (ct == 0 ? g.L[3] - 1 : ct - 1));
+        const long s0 = (long)__builtin_amdgcn_readfirstlane(tn) * (long)slice;  // first half-site of the slice
+        const long base = (s0 >> 6) * (12 * kBlk) * (long)N;                     // its element offset
+        in = make_rsrc((const SI *)a.in + base, 12u * slice * N * (uint32_t)sizeof(SI));
+        vs = (uint32_t)((long)aidx_site(nb, 12) * N - base);
+      } else {
+        vs = (uint32_t)aidx_site(nb, 12) * N;
+      }
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         const uint32_t vo = (vs + (uint32_t)rl[j]) * (uint32_t)sizeof(SI);
@@ -297,7 +312,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   C *stage = (C *)wc_dyn_lds;
   const long v0 = Gt == 1 ? q0 : (long)((uint32_t)q0 / (uint32_t)Gt);
   int ns = 0;
-  const bool clov_staged = LS && (EPI & EPI_CLOV) && a.clov;
+  const bool clov_staged = LS && (EPI & EPI_CLOV) && a.clov && a.clov_rstride == 0;
   if constexpr (LS) {
     long v1 = (long)((uint32_t)(q0 + (long)blockDim.x - 1) / (uint32_t)Gt);
     if (v1 > g.Vh - 1) v1 = g.Vh - 1;
@@ -370,6 +385,15 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   } else if (EPI & EPI_CLOV) {
     if (LS && clov_staged) {
       clov_apply_inplace_at<R, RB>(stage + 72 * ns + sv, (long)ns, acc);
+    } else if (a.clov && a.clov_rstride) {
+      // one set of blocks per right-hand side (a batch of several masses)
+#pragma unroll
+      for (int j = 0; j < RB; ++j) {
+        C o[12];
+        clov_apply<R>((const C *)a.clov + (long)rl[j] * a.clov_rstride, h, acc[j], o);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[j][k] = o[k];
+      }
     } else if (a.clov) {
       clov_apply_inplace<R, RB>((const C *)a.clov, h, acc);
     } else {
@@ -419,7 +443,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   }
 #pragma unroll
   for (int j = 0; j < RB; ++j)
-    if (rb + j < N) store12_stream((SO *)a.out, g.Vh, h, rb + j, N, acc[j]);
+    if (rb + j < N) store12_stream((SO *)a.out, g.Vh, h, rb + j, N, acc[j], a.big != 0);
 }
 
 // epilogues instantiated per storage: all of them in the working precision;
@@ -445,7 +469,7 @@ static bool hop_epi(const HopLaunch &a, hipStream_t s) {
   // the workgroups per CU at shape.occ
   size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
   if (a.shape.ls == 1) {
-    const size_t st = (size_t)(72 + ((a.epi & EPI_CLOV) && a.clov ? 36 : 0)) *
+    const size_t st = (size_t)(72 + ((a.epi & EPI_CLOV) && a.clov && !a.clov_rstride ? 36 : 0)) *
                       (size_t)stage_sites(threads, G) * sizeof(C);
     if (st > (size_t)kMaxStageBytes) return false;
     lds = std::max(lds, st);
@@ -493,8 +517,20 @@ template <typename R, typename SI, typename SO> static bool hop_links(const HopL
   return a.dag ? hop_rb<R, 6, 1, SI, SO>(a, s) : hop_rb<R, 6, 0, SI, SO>(a, s);
 }
 
-template <typename R> bool hop_launch(const HopLaunch &a, hipStream_t s) {
+template <typename R> bool hop_launch(const HopLaunch &a0, hipStream_t s) {
   using C = typename c2<R>::t;
+  HopLaunch a = a0;
+  {
+    // fields beyond one descriptor's 4 GB: per-slice gather and per-block
+    // store descriptors, which need whole 64-site blocks per t-slice and a
+    // slice within 4 GB
+    const double in_b = a.half_in ? 4.0 : (double)sizeof(C), out_b = a.half_out ? 4.0 : (double)sizeof(C);
+    const double e = 12.0 * (double)a.g.Vh * (double)a.nrhs;
+    a.big = (e * in_b >= 4294967296.0 || e * out_b >= 4294967296.0) ? 1 : 0;
+    if (a.big && ((a.g.Vh / a.g.L[3]) % kBlk != 0 || e / a.g.L[3] * in_b >= 4294967296.0 ||
+                  a.shape.ls == 2 || a.skip_t_faces))
+      return false;
+  }
   const HopShape &sh = a.shape;
   if (sh.tpb < 1 || sh.tpb > kMaxTpb || sh.rb < 1 || sh.rb > max_rb<R>() || sh.occ < 0 || sh.occ > kMaxOcc ||
       a.nrhs < 1 || a.nrhs > 64 || a.g.Vh % kBlk)

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kMmaThreads, 4) void k_hop_mma(const HopLaunch a) {
 #pragma unroll
     for (int e = 0; e < 9; ++e) lw[(s * 8 + dir) * 9 + e] = u[e];
   }
-  const bool clov_staged = (EPI & EPI_CLOV) && a.clov;
+  const bool clov_staged = (EPI & EPI_CLOV) && a.clov && a.clov_rstride == 0;
   if (clov_staged)
     for (int i = tid; i < 36 * kMmaSites; i += kMmaThreads)
       lc[i] = ((const C *)a.clov)[aidx(i / kMmaSites, site0 + i % kMmaSites, 36)];
@@ -289,6 +289,12 @@ __global__ __launch_bounds__(kMmaThreads, 4) void k_hop_mma(const HopLaunch a) {
       } else if (EPI & EPI_CLOV) {
         if (clov_staged) {
           clov_apply_inplace_at<float, 1>(lc + s, (long)kMmaSites, v);
+        } else if (a.clov) {
+          // one set of blocks per right-hand side (a batch of several masses)
+          C o[12];
+          clov_apply<float>((const C *)a.clov + (long)rhs * a.clov_rstride, h, v[0], o);
+#pragma unroll
+          for (int k = 0; k < 12; ++k) v[0][k] = o[k];
         } else {
 #pragma unroll
           for (int k = 0; k < 12; ++k) v[0][k] = cscale(v[0][k], (float)a.clov_scalar);
@@ -336,9 +342,12 @@ template <int NL, int DAG> static bool mma_epi(const HopLaunch &a, hipStream_t s
 }
 
 bool hop_mma_applies(const HopLaunch &a) {
+  // in and out within one descriptor's 4 GB (the FP16 field; an FP32 output
+  // of twice its size)
+  const double e = 12.0 * (double)a.g.Vh * (double)a.nrhs;
   return a.mma && a.half_in == kS16Half && (a.half_out == kS16Half || a.half_out == 0) && a.nrhs >= 4 &&
-         a.nrhs % 4 == 0 && !a.skip_t_faces && a.g.Vh % kMmaSites == 0 &&
-         (a.nl == 6 || a.nl == 9);
+         a.nrhs % 4 == 0 && !a.skip_t_faces && a.g.Vh % kMmaSites == 0 && (a.nl == 6 || a.nl == 9) &&
+         e * (a.half_out ? 4.0 : 8.0) < 4294967296.0;
 }
 
 bool hop_mma(const HopLaunch &a, hipStream_t s) {

@@ -87,6 +87,15 @@ inline int ensure_run(IterRun &ir) {
   return WC_OK;
 }
 
+// Clover-inverse blocks per right-hand side (a batch whose slots have
+// different masses): right-hand side r of parity p uses Ai[p] + r * rstride
+// (complex elements).  While an operator's `cov` points here, every
+// clover-inverse application of that operator uses these blocks.
+struct ClovSets {
+  const void *Ai[2] = {nullptr, nullptr};
+  long rstride = 0;
+};
+
 struct wc_operator {
   wc_params p{};
   Geom g{};
@@ -119,7 +128,18 @@ struct wc_operator {
   int mma = 2;
   const int *halt_cur = nullptr;  // halt word of the run being enqueued (null outside runs)
   IterRun run;                  // chunked iterations of this operator's CG and mixed solves
+  const ClovSets *cov = nullptr;  // per-RHS clover inverses in use (a multi-mass batch), or null
 };
+
+// the clover-inverse blocks of parity p as the operator applies them now
+// (null: the scalar 1 / diag), their stride between right-hand sides, and
+// the clover blocks themselves (never per right-hand side)
+inline const void *clov_inv(const wc_operator *op, int p) {
+  if (op->cov) return op->cov->Ai[p];
+  return op->has_clover ? op->Ai[p] : nullptr;
+}
+inline long clov_rstride(const wc_operator *op) { return op->cov ? op->cov->rstride : 0; }
+inline const void *clov_fwd(const wc_operator *op, int p) { return op->has_clover ? op->A[p] : nullptr; }
 
 inline size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
 
@@ -292,6 +312,11 @@ inline int mma_for(const wc_operator *op, int nrhs) {
   return it != g_mma_cache.end() ? it->second : 0;
 }
 
+// clover-inverse blocks of both parities (and optionally the blocks) of the
+// operator's clover term at another mass (wc_api.hip)
+int build_mass_blocks(const wc_operator *op, double mass, void *ai0, void *ai1, void *a0, void *a1, int *err,
+                      hipStream_t s);
+
 // ------------------------------------------------------------ compositions
 template <typename R>
 inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, int epi, const void *in, void *out) {
@@ -318,6 +343,7 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
   a.bps = op->bps;
   a.halt = op->halt_cur;
   a.mma = mma_for(op, nrhs);
+  a.clov_rstride = clov_rstride(op);
   return a;
 }
 
@@ -329,13 +355,15 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
               (a).shape.rb, (a).shape.tpb, (a).shape.occ, (a).shape.seg, (a).shape.ls);        \
   } while (0)
 
-// out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]
+// out = A_p in (inv = 0) or A_p^-1 in (inv = 1) [optionally y - that]; the
+// inverse follows the per-RHS sets in use (clov_inv), A_p is the operator's
 template <typename R>
 inline void site_op(const wc_operator *op, int nrhs, int p, int inv, const void *in, void *out, const void *y,
                     hipStream_t s, int half = 0) {
-  const void *B = op->has_clover ? (inv ? op->Ai[p] : op->A[p]) : nullptr;
+  const void *B = inv ? clov_inv(op, p) : clov_fwd(op, p);
   double sc = inv ? 1.0 / op->diag : op->diag;
-  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s, half);
+  wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s, half,
+                      inv ? clov_rstride(op) : 0);
 }
 
 // Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd on nrhs
@@ -353,8 +381,8 @@ inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, cons
   if (half && (sizeof(R) != 4 || which != 2))
     WC_FAIL(WC_ERR_ARG, "mpc: 16-bit storage is for the FP32 normal operator only");
   const int pc = 1 - pd;
-  const void *clovC = op->has_clover ? op->Ai[pc] : nullptr;
-  const void *clovD = op->has_clover ? op->Ai[pd] : nullptr;
+  const void *clovC = clov_inv(op, pc);
+  const void *clovD = clov_inv(op, pd);
   const void *src = in;
   void *dst_mpc = (which == 2) ? tY : out;
   if (which == 0 || which == 2) {
@@ -434,8 +462,8 @@ inline int inner_iteration(wc_operator *lo, int N, int pd, void *r, void *p, voi
   wck::cg_alpha(partials, wck::hop_blocks(lo->g), N, st, s, halt);
   wck::cg_update_r<R>(n, N, st, r, mmp, partials, s, halt);
   wck::cg_beta(partials, wck::kRedBlocks, N, st, s, halt);
-  wck::cg_update_xp<R>(lo->g, N, pd, st, r, p, x, lo->has_clover ? lo->Ai[pd] : nullptr, 1.0 / lo->diag, aD,
-                       lo->open_bc ? 1 : 0, s, half, halt);
+  wck::cg_update_xp<R>(lo->g, N, pd, st, r, p, x, clov_inv(lo, pd), 1.0 / lo->diag, aD,
+                       lo->open_bc ? 1 : 0, s, half, halt, clov_rstride(lo));
   WC_LAUNCHED();
   return WC_OK;
 }
@@ -451,8 +479,8 @@ inline void op_key(const wc_operator *op, int nrhs, std::vector<uintptr_t> &key)
   };
   const wck::HopShape sh = shape_for(op, nrhs);
   auto tmp = [&](size_t i) { return i < op->tmp.size() ? (uintptr_t)op->tmp[i] : (uintptr_t)0; };
-  for (uintptr_t v : {(uintptr_t)op, (uintptr_t)op->W[0], (uintptr_t)op->W[1], (uintptr_t)op->Ai[0],
-                      (uintptr_t)op->Ai[1], tmp(0), tmp(1), tmp(2), (uintptr_t)op->dots, (uintptr_t)op->partials, bits(op->diag), bits(op->cs), bits(op->ct), (uintptr_t)op->open_bc,
+  for (uintptr_t v : {(uintptr_t)op, (uintptr_t)op->W[0], (uintptr_t)op->W[1], (uintptr_t)clov_inv(op, 0),
+                      (uintptr_t)clov_inv(op, 1), (uintptr_t)clov_rstride(op), tmp(0), tmp(1), tmp(2), (uintptr_t)op->dots, (uintptr_t)op->partials, bits(op->diag), bits(op->cs), bits(op->ct), (uintptr_t)op->open_bc,
                       (uintptr_t)op->has_clover, (uintptr_t)sh.rb, (uintptr_t)sh.tpb, (uintptr_t)sh.occ,
                       (uintptr_t)sh.seg, (uintptr_t)sh.ls, (uintptr_t)op->bps})
     key.push_back(v);
@@ -583,7 +611,7 @@ inline int schur_solve(wc_operator *op, int nrhs, int pd, void *dst, const void 
   site_op<R>(op, nrhs, pc, 1, half(src, pc), tR, nullptr, s);
   {
     HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV | wck::EPI_AXPY, half(dst, pd), half(dst, pc));
-    a.clov = op->has_clover ? op->Ai[pc] : nullptr;
+    a.clov = clov_inv(op, pc);
     a.y = tR;
     WC_HOP(R, a, s);
   }
@@ -591,10 +619,19 @@ inline int schur_solve(wc_operator *op, int nrhs, int pd, void *dst, const void 
   return WC_OK;
 }
 
+// A half field of nrhs right-hand sides larger than one buffer descriptor's
+// 4 GB is addressed per t-slice by the hop (HopLaunch::big): that needs whole
+// 64-site blocks per slice and a slice of nrhs right-hand sides within 4 GB.
 inline int check_nrhs(const wc_operator *op, int nrhs, const char *fn) {
   if (nrhs < 1 || nrhs > 64) WC_FAIL(WC_ERR_ARG, "%s: nrhs = %d must be in [1, 64]", fn, nrhs);
-  if ((double)half_bytes(op) * nrhs >= 4294967296.0)
-    WC_FAIL(WC_ERR_ARG, "%s: a half field of %d right-hand sides exceeds the 4 GB gather window", fn, nrhs);
+  if ((double)half_bytes(op) * nrhs >= 4294967296.0) {
+    const long slice = op->g.Vh / op->g.L[3];
+    if (slice % 64 != 0)
+      WC_FAIL(WC_ERR_ARG, "%s: a half field of %d right-hand sides exceeds 4 GB and a t-slice of %ld half-sites "
+                          "is not a whole number of 64-site blocks", fn, nrhs, slice);
+    if ((double)half_bytes(op) / op->g.L[3] * nrhs >= 4294967296.0)
+      WC_FAIL(WC_ERR_ARG, "%s: a t-slice of %d right-hand sides exceeds 4 GB", fn, nrhs);
+  }
   return WC_OK;
 }
 

@@ -99,12 +99,25 @@ __device__ __forceinline__ void store12(S *f, long h, int r, int N, const C v[12
 #pragma unroll
   for (int k = 0; k < 12; ++k) p[(long)k * kBlk * N] = narrow<S>(v[k]);
 }
-// output of a hop: written once, not re-read by the kernel
+// output of a hop: written once, not re-read by the kernel.  big: the field
+// exceeds one descriptor's 4 GB; the descriptor is then based at the 64-site
+// block of the wave's first active lane (every lane of a wave stores into
+// that block)
 template <typename S, typename C>
-__device__ __forceinline__ void store12_stream(S *f, long Vh, long h, int r, int N, const C v[12]) {
+__device__ __forceinline__ void store12_stream(S *f, long Vh, long h, int r, int N, const C v[12],
+                                               bool big = false) {
 #if WC_NT_STORE
-  const wc_rsrc rs = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)N * (uint32_t)sizeof(S)));
-  const uint32_t vo = (uint32_t)(aidx_site(h, 12) * N + r) * (uint32_t)sizeof(S);
+  wc_rsrc rs;
+  uint32_t vo;
+  if (big) {
+    const long blk = (long)__builtin_amdgcn_readfirstlane((int)(h >> 6));
+    const long base = blk * (12 * kBlk) * (long)N;
+    rs = make_rsrc(f + base, (uint32_t)(12u * kBlk * (uint32_t)N * (uint32_t)sizeof(S)));
+    vo = (uint32_t)(aidx_site(h, 12) * N + r - base) * (uint32_t)sizeof(S);
+  } else {
+    rs = make_rsrc(f, (uint32_t)(12u * (uint32_t)Vh * (uint32_t)N * (uint32_t)sizeof(S)));
+    vo = (uint32_t)(aidx_site(h, 12) * N + r) * (uint32_t)sizeof(S);
+  }
 #pragma unroll
   for (int k = 0; k < 12; ++k)
     bstore_t<S, 2>::go(narrow<S>(v[k]), rs, vo, (uint32_t)(k * kBlk * N * sizeof(S)));

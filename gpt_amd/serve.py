@@ -6,10 +6,13 @@ one operator at a time.  Here requests (a gauge field, a mass, 1..12
 right-hand sides, a tolerance) arrive at any time and are served by
 continuous batching:
 
-* a *group* per (resident field, mass) owns an FP64 / FP32 operator pair and
-  a ``batch`` of solve slots (``wc_batch_*``: the mixed-precision eo2_ne CG
+* a *group* per resident field owns an FP64 / FP32 operator pair and a
+  ``batch`` of solve slots (``wc_batch_*``: the mixed-precision eo2_ne CG
   advancing all occupied slots with the multi-RHS kernels), sized to the
-  right-hand sides waiting when it starts (at most ``nslots``);
+  right-hand sides waiting when it starts (a multiple of 4, at most
+  ``nslots``); requests of any mass share it: each slot carries its own
+  clover-inverse blocks for its request's mass (``wc_batch_admit_mass``), so
+  same-field requests are batched whatever their masses;
 * right-hand sides enter free slots of their group whenever slots free up,
   while the group keeps iterating, and leave when their FP64 true residual
   |src - M dst| / |src| meets the request's tolerance;
@@ -58,10 +61,19 @@ class batch:
         self.free = list(range(self.nslots))
         self.slot_of = {}
 
-    def admit(self, src, tol, maxiter, guess=None):
+    def admit(self, src, tol, maxiter, guess=None, mass=None):
+        """put src into a free slot; mass: the slot's own mass (None: the
+        operators'), so one batch serves several masses"""
         slot = self.free.pop(0)
-        _lib.check(_lib.load().wc_batch_admit(self.handle, slot, src.ptr(), guess.ptr() if guess is not None else None,
-                                              float(tol), int(maxiter)))
+        gp = guess.ptr() if guess is not None else None
+        if mass is None:
+            rc = _lib.load().wc_batch_admit(self.handle, slot, src.ptr(), gp, float(tol), int(maxiter))
+        else:
+            rc = _lib.load().wc_batch_admit_mass(self.handle, slot, float(mass), src.ptr(), gp, float(tol),
+                                                 int(maxiter))
+        if rc:
+            self.free.insert(0, slot)
+            _lib.check(rc)
         return slot
 
     def step(self, iters):
@@ -126,17 +138,19 @@ class request:
         return None if self.t_done is None else self.t_done - self.t_arrival
 
 
-def group_bytes(fdims, nslots):
+def group_bytes(fdims, nslots, masses=True):
     """HBM held by one group: both operators (links -- 12-real in the inner
     one --, clover and inverse, the FP64 gauge copy each keeps, solver
-    scratch) and the batch state."""
+    scratch), the batch state and, for a batch of several masses, each
+    slot's clover inverses (both parities, FP64 and FP32)."""
     V = int(np.prod(fdims))
     Vh = V // 2
     h64, h32 = Vh * 12 * 16, Vh * 12 * 8
     op64 = 2 * Vh * 8 * 9 * 16 + 4 * Vh * 36 * 16 + V * 4 * 9 * 16 + h64 * (3 * nslots + 9)
     op32 = 2 * Vh * 8 * 6 * 8 + 4 * Vh * 36 * 8 + V * 4 * 9 * 16 + h32 * (3 * nslots + 5)
     state = 5 * h64 * nslots + 5 * h32 * nslots + 2 * h64 * nslots
-    return op64 + op32 + state
+    sets = nslots * 2 * Vh * 36 * (16 + 8) if masses else 0
+    return op64 + op32 + state + sets
 
 
 class _group:
@@ -144,8 +158,9 @@ class _group:
         self.server = server
         self.key = key
         self.U = U
-        self.mass = mass
+        self.mass = mass  # the operators' mass (the first request's); slots carry their own
         self.stream = stream
+        self.setup_s = None  # wall time of creating the operators and the batch
         self.queue = collections.deque()  # (request, rhs index)
         self.in_slot = {}  # slot -> (request, rhs index)
         self.batch = None
@@ -164,6 +179,7 @@ class _group:
         done = []
         with torch.cuda.stream(self.stream):
             if self.batch is None:
+                t0 = time.perf_counter()
                 p = dict(self.server.params, mass=self.mass)
                 self.op = fermion.wilson_clover(self.U, p)
                 # the inner operator only drives the recursion between FP64
@@ -174,9 +190,12 @@ class _group:
                     self.op.set_iteration_chunk(self.server.iteration_chunk)
                 self.batch = batch(self.op, self.inner, self.nslots, delta=self.server.delta,
                                    inner_storage=self.server.inner_storage)
+                torch.cuda.current_stream().synchronize()
+                self.setup_s = time.perf_counter() - t0
             while self.queue and self.batch.free:
                 req, j = self.queue.popleft()
-                slot = self.batch.admit(req.sources[j], req.tol, req.maxiter)
+                slot = self.batch.admit(req.sources[j], req.tol, req.maxiter,
+                                        mass=None if req.mass == self.mass else req.mass)
                 self.in_slot[slot] = (req, j)
             finished = self.batch.step(iters) if self.in_slot else []
             for slot in finished:
@@ -227,6 +246,7 @@ class server:
         self.completed = []
         self.peak_groups = 0
         self.waited_for_memory = 0
+        self.setup_times = []  # per group: seconds to create its operators and batch
         self.warmed = set()
 
     def add_field(self, key, U, warmup=True):
@@ -268,16 +288,17 @@ class server:
             op.release()
         self.warmed.add(fd)
 
-    # a group's batch has as many slots as the request that opens it has
-    # right-hand sides (at most nslots): a slot costs the same whether or not
-    # it is occupied; later requests of the same (field, mass) take slots as
-    # they free up
+    # one group per resident field: its batch has as many slots as the
+    # request that opens it has right-hand sides, rounded up to a multiple of 4
+    # (the matrix-core hop's unit) and at most nslots -- a slot costs the same
+    # whether or not it is occupied; later requests of the field, of any mass,
+    # take slots as they free up
     def _route(self, req):
-        key = (req.field, req.mass)
+        key = req.field
         g = self.groups.get(key)
         if g is None:
             U = self.fields[req.field]
-            nslots = max(1, min(self.nslots, len(req.sources)))
+            nslots = max(1, min(self.nslots, 4 * ((len(req.sources) + 3) // 4)))
             need = group_bytes(U[0].grid.fdimensions, nslots)
             if self.hbm_used + need > self.hbm_budget:
                 return False
@@ -343,6 +364,8 @@ class server:
                             on_done(req)
                 g = self.groups[key]
                 if not g.busy():
+                    if g.setup_s is not None:
+                        self.setup_times.append(g.setup_s)
                     g.release()
                     self.hbm_used -= g.bytes
                     del self.groups[key]

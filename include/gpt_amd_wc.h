@@ -326,6 +326,16 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
                     void *stream, wc_batch **out);
 int wc_batch_destroy(wc_batch *b);
 int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter);
+/* the same with the slot's own mass (no cgpt counterpart; gpt re-creates or
+ * re-masses an operator per mass, interface.py:41-48).  The first mass
+ * other than the operators' gives every slot its own clover-inverse blocks
+ * (both parities, FP64 and the inner precision: 36 Vh (16 + inner bytes) x
+ * 2 per slot), built from the operators' gauge field for that mass; from then
+ * on every clover-inverse application of the batch indexes them by slot, so
+ * requests of any mass on one gauge field share one batch.  A slot admitted
+ * with wc_batch_admit then gets the operators' mass. */
+int wc_batch_admit_mass(wc_batch *b, int slot, double mass, const void *src, const void *guess, double tol,
+                        long maxiter);
 int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);
 int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double *true_residual, int *converged);
 int wc_batch_slots(const wc_batch *b, int *occupied, int *active);

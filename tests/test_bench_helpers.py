@@ -19,11 +19,13 @@ def test_profiled_traffic_per_shape():
     assert bench.profiled_traffic((1, 2, 4, 1, 0)) is None
 
 
-def test_c4_batches_fit_the_offset_window():
+def test_c4_batches_fit_the_slice_window_and_memory():
     vh_big, vh_small = 32**3 * 64 // 2, 16**3 * 32 // 2
-    assert bench.c4_batch(vh_big, 12) == 12
-    assert bench.c4_batch(vh_big, 48) == 16
-    assert bench.c4_batch(vh_small, 48) == 48
-    for vh, n in [(vh_big, 12), (vh_big, 48), (vh_small, 48), (vh_big, 7)]:
-        nb = bench.c4_batch(vh, n)
-        assert n % nb == 0 and vh * 12 * 16 * nb < 1 << 32
+    free = 280 << 30
+    assert bench.c4_batch(vh_big, 12, 64, free) == 12
+    assert bench.c4_batch(vh_big, 48, 64, free) == 48  # a 9.7 GB half field, addressed per t-slice
+    assert bench.c4_batch(vh_big, 48, 64, 100 << 30) == 16  # memory caps it
+    assert bench.c4_batch(vh_small, 48, 32, free) == 48
+    for vh, n, lt in [(vh_big, 12, 64), (vh_big, 48, 64), (vh_small, 48, 32), (vh_big, 7, 64)]:
+        nb = bench.c4_batch(vh, n, lt, free)
+        assert n % nb == 0 and vh * 12 * 16 * nb // lt < 1 << 32 and 20 * vh * 12 * 16 * nb <= 0.85 * free

@@ -91,3 +91,38 @@ def test_meooe_32x32x32x64_per_site(big, prec, compress):
                 print(f"fullsize: shape {shape} pin {pin} dag {dag}: max site rel err {err:.2e}", flush=True)
                 assert err < tol, (shape, pin, dag, err)
     w.release()
+
+
+@pytest.mark.parametrize("prec", [g.double, g.single])
+def test_fields_beyond_the_4gb_window(big, prec):
+    """32^3 x 64 with 24 right-hand sides in FP64 (a 4.8 GB half field) and 48
+    in FP32 (4.8 GB): the hop addresses such fields per t-slice.  Meooe and
+    the normal operator (clover, axpy and inner-product epilogues) of the
+    packed field equal the single-RHS results for the first, a middle and the
+    last right-hand side, bitwise or to 1e-14 (FP32: 1e-6) of the field"""
+    U, ref, srcs = big
+    N = 24 if prec is g.double else 48
+    dev = torch.device("cuda", 0)
+    wN = g.qcd.fermion.wilson_clover(U, P, precision=prec, n_rhs=N)
+    w1 = g.qcd.fermion.wilson_clover(U, P, precision=prec)
+    assert 12 * (2 ** 20) * N * prec.nbytes * 2 > 2 ** 32  # Vh = 2^20 half-sites, complex = 2 reals
+    X = g.lattice(wN.F_grid_eo)
+    X.checkerboard(g.odd)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(2003)
+    X.data.copy_(torch.randn(X.data.shape, generator=gen, dtype=torch.float64, device=dev).to(X.data.dtype))
+    Y = wN.Meooe(X)
+    pc = g.qcd.fermion.preconditioner
+    Z = pc.eo2_ne()(wN).Mpc(X)
+    torch.cuda.synchronize()
+    tol = 1e-14 if prec is g.double else 1e-6
+    for r in (0, N // 2, N - 1):
+        x1 = X.rhs(r)
+        y1 = w1.Meooe(x1)
+        z1 = pc.eo2_ne()(w1).Mpc(x1)
+        for a, b in ((Y.rhs(r), y1), (Z.rhs(r), z1)):
+            d = float((a.data - b.data).abs().max() / b.data.abs().max())
+            print(f"beyond 4 GB: {prec.__name__} N={N} rhs {r}: max rel diff {d:.1e}", flush=True)
+            assert d <= tol, (r, d)
+    wN.release()
+    w1.release()

@@ -9,7 +9,12 @@ Checks, with the oracle's FP64 operator as the referee:
   batch, so the solutions agree to within a few condition numbers times tol);
 * the server serves requests for two resident fields and several masses,
   with an HBM budget small enough that requests must wait for memory;
-* a request served alone and among requests of other groups gives bitwise
+* requests of different masses on one field share one batch (one group per
+  field): each slot carries its own clover inverses, every solve meets its
+  tolerance through the oracle's M at its own mass and agrees with its solve
+  alone -- with clover, and for plain Wilson fermions;
+* fields of different volumes are served side by side;
+* a request served alone and among requests of other fields gives bitwise
   the same solutions.
 """
 import numpy as np
@@ -136,9 +141,9 @@ def test_server_mixed_fields_masses_and_memory_wait(setup):
     gr, U, srcs = setup
     Uh = {"a": U[0], "b": U[1]}
     nslots = 4
-    # room for the largest group of the plan (3 right-hand sides), not for two groups
-    budget = int(1.05 * serve.group_bytes([8, 8, 8, 16], 3))
-    assert 2 * serve.group_bytes([8, 8, 8, 16], 1) > budget
+    # room for one group (4 slots of several masses), not for two
+    budget = int(1.05 * serve.group_bytes([8, 8, 8, 16], nslots))
+    assert 2 * serve.group_bytes([8, 8, 8, 16], nslots) > budget
     srv = serve.server(P, nslots=nslots, streams=2, iters_per_step=10, hbm_budget=budget)
     for k, u in Uh.items():
         srv.add_field(k, u)
@@ -168,10 +173,10 @@ def test_server_mixed_fields_masses_and_memory_wait(setup):
 
 def test_request_alone_equals_request_among_others(setup):
     """determinism, alone vs mixed: a request served on its own and the same
-    request served while requests for another field and other masses run in
-    their own groups on other streams give bitwise the same solutions and
-    iteration counts (groups share no arithmetic; launch shapes and iteration
-    schedules do not change bits)"""
+    request served while requests for another field (two masses, sharing a
+    batch) run in their own group on another stream give bitwise the same
+    solutions and iteration counts (groups share no arithmetic; launch shapes
+    and iteration schedules do not change bits)"""
     gr, U, srcs = setup
     Uh = {"a": U[0], "b": U[1]}
     grid = g.grid([8, 8, 8, 16], g.double)
@@ -187,9 +192,82 @@ def test_request_alone_equals_request_among_others(setup):
         return reqs
 
     alone = serve_plan([("a", 0.1, [0, 1])])[0]
-    mixed = serve_plan([("b", 0.15, [3, 4, 5]), ("a", 0.1, [0, 1]), ("a", 0.2, [6]), ("b", 0.05, [7])])
+    mixed = serve_plan([("b", 0.15, [3, 4, 5]), ("a", 0.1, [0, 1]), ("b", 0.05, [7])])
     among = mixed[1]
     assert all(r.converged for r in mixed) and all(alone.converged)
     assert alone.iterations == among.iterations
     for x, y in zip(alone.results, among.results):
         assert np.array_equal(x.to_numpy(), y.to_numpy())
+
+
+@pytest.mark.parametrize("csw", [1.0, 0.0])
+def test_masses_share_one_batch(setup, csw):
+    """three masses admitted into one 4-slot batch (two while the first is
+    mid-solve): each solve meets its tolerance at its own mass and agrees
+    with the same solve alone on an operator of that mass"""
+    gr, U, srcs = setup
+    prm = dict(P, csw_r=csw, csw_t=csw)
+    masses = [0.1, 0.05, 0.2]
+    w = g.qcd.fermion.wilson_clover(U[0], dict(prm, mass=masses[0]))
+    inner = g.qcd.fermion.wilson_clover(U[0], dict(prm, mass=masses[0], compress_links=True), precision=g.single)
+    b = serve.batch(w, inner, 4)
+    S = [field(srcs[k], w.F_grid) for k in range(3)]
+    owner = {b.admit(S[0], TOL, 10000): 0}
+    assert b.step(5) == []
+    owner[b.admit(S[1], TOL, 10000, mass=masses[1])] = 1
+    owner[b.admit(S[2], TOL, 10000, mass=masses[2])] = 2
+    results = {}
+    for _ in range(400):
+        for sl in b.step(10):
+            x = g.lattice(w.F_grid)
+            it, tres, conv = b.extract(sl, x)
+            assert conv and 0 < tres <= TOL
+            results[owner.pop(sl)] = x.to_numpy()
+        if len(results) == 3:
+            break
+    b.close()
+    assert sorted(results) == [0, 1, 2]
+    for k in range(3):
+        ref = W.wilson_clover(U[0], mass=masses[k], **prm)
+        assert true_res(ref, results[k], srcs[k]) <= 1.05 * TOL, k
+        wk = g.qcd.fermion.wilson_clover(U[0], dict(prm, mass=masses[k]))
+        bk = serve.batch(wk, wk.converted(g.single), 1)
+        bk.admit(field(srcs[k], wk.F_grid), TOL, 10000)
+        while not bk.step(50):
+            pass
+        xk = g.lattice(wk.F_grid)
+        bk.extract(0, xk)
+        bk.close()
+        assert rel(results[k], xk.to_numpy()) < AGREE, k
+
+
+def test_server_one_group_per_field_and_mixed_volumes(setup):
+    """requests of four masses on one 8^3 x 16 field and of two masses on an
+    8^4 field: one group per field (peak 2), every result checked with the
+    oracle at its own mass and volume"""
+    gr, U, srcs = setup
+    gr4 = R.grid([8, 8, 8, 8])
+    rng = R.random("serve_8888")
+    U4 = G.random_links(gr4, rng, scale=0.5)
+    srcs4 = [rng.vspincolor(gr4) for _ in range(3)]
+    srv = serve.server(P, nslots=4, streams=2, iters_per_step=10)
+    srv.add_field("big", U[0])
+    srv.add_field("small", U4)
+    g16, g8 = g.grid([8, 8, 8, 16], g.double), g.grid([8, 8, 8, 8], g.double)
+    plan = [("big", 0.1, [0, 1], srcs, g16), ("small", 0.08, [0], srcs4, g8), ("big", 0.03, [2], srcs, g16),
+            ("big", 0.2, [3, 4], srcs, g16), ("small", 0.15, [1, 2], srcs4, g8), ("big", 0.12, [5], srcs, g16)]
+    reqs = []
+    for f, m, idx, pool, grid in plan:
+        r = serve.request(f, m, [field(pool[i], grid) for i in idx], tol=TOL, maxiter=10000)
+        r.idx, r.pool = idx, pool
+        reqs.append(r)
+    done = srv.run([(0.003 * n, r) for n, r in enumerate(reqs)])
+    srv.close()
+    assert len(done) == len(reqs) and srv.peak_groups == 2
+    assert all(t > 0 for t in srv.setup_times)
+    fields = {"big": U[0], "small": U4}
+    for r in reqs:
+        assert all(r.converged)
+        ref = W.wilson_clover(fields[r.field], mass=r.mass, **P)
+        for j, i in enumerate(r.idx):
+            assert true_res(ref, r.results[j].to_numpy(), r.pool[i]) <= 1.05 * TOL, (r.field, r.mass, i)
