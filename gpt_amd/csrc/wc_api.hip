@@ -656,21 +656,23 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
 }
 
 // ------------------------------------------------------------- hop tuner
-// Times the hop (output parity 0, no epilogue) of nrhs right-hand sides for
-// every valid launch shape and returns the fastest.  Scratch: tmp[0..1],
-// zeroed first so that no shape sees denormals or NaN.
+// Times the hop (output parity 0, clover epilogue) of nrhs right-hand sides
+// for every valid launch shape and returns the fastest.  Scratch: tmp[0..1];
+// the input is a fixed pattern of values in [-1, 1) (no denormals or NaN,
+// and not zero: all-zero input runs faster, by an amount that depends on the
+// shape), the output is overwritten.
 template <typename R>
 static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopShape &best, float &best_ms) {
   int rc = ensure_tmp(op, 2, N);
   if (rc) return rc;
   const long n = op->g.Vh * 12 * N;
-  wck::zero<R>(n, op->tmp[0], s);
+  wck::fill_pattern<R>(n, op->tmp[0], s);
   wck::zero<R>(n, op->tmp[1], s);
   std::vector<wck::HopShape> cand;
-  // t-marching order (seg > 1) has never won on MI355X and is left to
-  // wc_set_hop_shape; link staging (ls 1) only pays when a site has several
-  // items (more right-hand sides than rb), spinor-block staging (ls 2) is
-  // for one right-hand side
+  // link staging (ls 1) only pays when a site has several items (more
+  // right-hand sides than rb), spinor-block staging (ls 2) is for one
+  // right-hand side; the t-marching order (seg > 1) is tried second, on the
+  // fastest shape in natural order
   const int occs[] = {0, 2, 4, 8, 16};
   for (int rb = 1; rb <= max_rb_prec(op->prec) && rb <= N; ++rb)
     for (int ls = 0; ls <= 2; ++ls)
@@ -688,14 +690,14 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
   WC_HIP(hipEventCreate(&e0));
   WC_HIP(hipEventCreate(&e1));
   best_ms = -1.0f;
-  for (const wck::HopShape &sh : cand) {
+  auto try_shape = [&](const wck::HopShape &sh) -> int {
     // the hop of the normal operator's first step (clover-inverse epilogue),
     // whose blocks the staging shapes also stage
     HopLaunch a = hop_args<R>(op, N, 0, 0, op->has_clover ? wck::EPI_CLOV : 0, op->tmp[1], op->tmp[0]);
     a.clov = op->has_clover ? op->Ai[0] : nullptr;
     a.shape = sh;
     // shapes this device cannot launch, or whose staging area is too large
-    if (!wck::hop<R>(a, s) || hipGetLastError() != hipSuccess) continue;
+    if (!wck::hop<R>(a, s) || hipGetLastError() != hipSuccess) return WC_OK;
     WC_HIP(hipEventRecord(e0, s));
     for (int k = 0; k < reps; ++k) WC_HOP(R, a, s);
     WC_HIP(hipEventRecord(e1, s));
@@ -706,9 +708,29 @@ static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopSh
       best_ms = ms;
       best = sh;
     }
+    return WC_OK;
+  };
+  for (const wck::HopShape &sh : cand)
+    if ((rc = try_shape(sh))) break;
+  // t-marching runs of bps/seg blocks: with several right-hand sides a time
+  // slice outgrows L2, and shorter runs bring the +-t neighbours of a block
+  // back while they are still cached (32^3 x 64, 12 FP64 right-hand sides:
+  // 4 runs per slice, 11 % faster than natural order).  The natural-order
+  // winner is timed again beside its t-marching variants: its first time is
+  // the minimum over many noisy candidates and would be biased low.
+  if (!rc && best_ms >= 0.0f && best.ls != 2 && op->bps > 1) {
+    const wck::HopShape nat = best;
+    best_ms = -1.0f;
+    rc = try_shape(nat);
+    for (int sg = 2; sg <= 16 && sg <= op->bps && !rc; sg *= 2) {
+      wck::HopShape sh = nat;
+      sh.seg = sg;
+      if (shape_valid(op, sh)) rc = try_shape(sh);
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) return rc;
   best_ms /= (float)reps;
   return WC_OK;
 }

@@ -117,6 +117,7 @@ int rhs_block_threads(int nrhs);
 void accumulate_f2d(long n, const void *lo, void *hi, hipStream_t s);
 // x = 0 for n complex entries of R (a kernel, so it can sit in a captured graph)
 template <typename R> void zero(long n, void *x, hipStream_t s);
+template <typename R> void fill_pattern(long n, void *x, hipStream_t s);
 // x[e] *= f[e % nrhs] for n entries (per-RHS real factors on the device)
 template <typename R> void scale_rhs(long n, int nrhs, const double *f, void *x, hipStream_t s);
 

@@ -693,6 +693,30 @@ template <typename R> void zero(long n, void *x, hipStream_t s) {
 template void zero<double>(long, void *, hipStream_t);
 template void zero<float>(long, void *, hipStream_t);
 
+// x[i] = a deterministic pseudo-random complex number in [-1, 1)^2 with
+// every mantissa bit random (a 64-bit mix of i, splitmix64): tuner input
+// with the bit statistics of real spinors.  All-zero input runs the hop up to
+// 9 % faster than Gaussian data on MI355X, and not by the same amount for
+// every launch shape, so the tuner must not time on zeros.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+template <typename R> __global__ void k_fill_pattern(long n, typename c2<R>::t *x) {
+  using C = typename c2<R>::t;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t a = mix64(2 * (uint64_t)i + 0x9e3779b97f4a7c15ull), b = mix64(2 * (uint64_t)i + 1);
+  x[i] = cmk<C>((R)((double)(int64_t)a * 0x1p-63), (R)((double)(int64_t)b * 0x1p-63));
+}
+template <typename R> void fill_pattern(long n, void *x, hipStream_t s) {
+  using C = typename c2<R>::t;
+  if (n > 0) k_fill_pattern<R><<<elem_blocks(n), 256, 0, s>>>(n, (C *)x);
+}
+template void fill_pattern<double>(long, void *, hipStream_t);
+template void fill_pattern<float>(long, void *, hipStream_t);
+
 template <typename R>
 __global__ void k_scale_rhs(long n, int N, const double *f, typename c2<R>::t *x) {
   using C = typename c2<R>::t;
