@@ -206,6 +206,14 @@ class fine_operator(matrix_operator):
 
         self.G5M = matrix_operator(_G5M, make_dst=make_full)
 
+    def set_mass(self, mass):
+        """change the mass in place: wc_set_mass rebuilds the clover blocks
+        and their inverses (gpt re-masses a pooled operator the same way,
+        lib/gpt/qcd/fermion/operator/interface.py:41-48)"""
+        self.params["mass"] = float(mass)
+        self.params_constructor = dict(self.params_constructor, mass=float(mass))
+        _lib.check(_lib.load().wc_set_mass(self.handle, float(mass), core.stream_ptr()))
+
     # native handle for the fused Schur / CG kernels
     @property
     def handle(self):

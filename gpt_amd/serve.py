@@ -366,6 +366,12 @@ class server:
                 if not g.busy():
                     if g.setup_s is not None:
                         self.setup_times.append(g.setup_s)
+                    # hipFree synchronises the device, which a graph capture
+                    # running on another worker thread does not survive: free
+                    # only once the running steps have ended (their results
+                    # are collected on the next pass)
+                    if busy:
+                        cf.wait(list(busy.values()))
                     g.release()
                     self.hbm_used -= g.bytes
                     del self.groups[key]
