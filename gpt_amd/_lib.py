@@ -74,6 +74,7 @@ SIGNATURES = {
     "wc_get_hop_shape": (_I, [_P, _I, _P]),
     "wc_set_iteration_chunk": (_I, [_P, _I]),
     "wc_trim": (_I, [_P]),
+    "wc_release_cached_memory": (_I, [_P]),
     "wc_gauge_observables": (_I, [_P, _P, _P, _P, _P]),
     "wc_halo_pack": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),
     "wc_halo_unpack": (_I, [_P, _I, _I, _I, _P, _P, _P, _P]),

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libgpt_amd_wc.so")
 SOURCES = ["wc_kernels.hip", "wc_hop_f64.hip", "wc_hop_f32.hip", "wc_api.hip", "wc_batch.hip", "wc_block.hip",
-           "wc_halo.hip", "wc_probe.hip", "wc_noalloc.hip"]
+           "wc_halo.hip", "wc_probe.hip", "wc_noalloc.hip", "wc_alloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

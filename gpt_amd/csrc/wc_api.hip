@@ -174,14 +174,14 @@ int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out
   };
   const size_t Vh = (size_t)op->g.Vh;
   for (int q = 0; q < 2; ++q) {
-    chk(hipMalloc(&op->W[q], Vh * 8 * op->nl * op->csz), "hipMalloc(links)");
+    chk(wc_dev_malloc(&op->W[q], Vh * 8 * op->nl * op->csz), "wc_dev_malloc(links)");
     if (op->has_clover) {
-      chk(hipMalloc(&op->A[q], Vh * 36 * op->csz), "hipMalloc(clover)");
-      chk(hipMalloc(&op->Ai[q], Vh * 36 * op->csz), "hipMalloc(clover inverse)");
+      chk(wc_dev_malloc(&op->A[q], Vh * 36 * op->csz), "wc_dev_malloc(clover)");
+      chk(wc_dev_malloc(&op->Ai[q], Vh * 36 * op->csz), "wc_dev_malloc(clover inverse)");
     }
   }
-  chk(hipMalloc(&op->Unat, (size_t)op->g.V * 4 * 9 * 16), "hipMalloc(gauge copy)");
-  chk(hipMalloc((void **)&op->err, sizeof(int)), "hipMalloc(flag)");
+  chk(wc_dev_malloc(&op->Unat, (size_t)op->g.V * 4 * 9 * 16), "wc_dev_malloc(gauge copy)");
+  chk(wc_dev_malloc((void **)&op->err, sizeof(int)), "wc_dev_malloc(flag)");
   if (rc == WC_OK && ensure_red(op, 1) != WC_OK) rc = WC_ERR_HIP;
   if (rc != WC_OK) {
     free_op(op);
@@ -954,13 +954,13 @@ int wc_trim(wc_operator *op) {
   if (!op) WC_FAIL(WC_ERR_ARG, "wc_trim: null operator");
   WC_HIP(hipDeviceSynchronize());
   drop_graphs(op->run);
-  for (void *q : op->tmp) (void)hipFree(q);
+  for (void *q : op->tmp) (void)wc_dev_free(q);
   op->tmp.clear();
   op->tmp_bytes = 0;
-  (void)hipFree(op->partials);
+  (void)wc_dev_free(op->partials);
   op->partials = nullptr;
   op->npartials = 0;
-  (void)hipFree(op->dots);
+  (void)wc_dev_free(op->dots);
   op->dots = nullptr;
   op->ndots = 0;
   return WC_OK;
@@ -1170,7 +1170,7 @@ static thread_local double *g_red_host = nullptr;  // pinned host copy
 
 static int red_buffers() {
   if (!g_red) {
-    WC_HIP(hipMalloc((void **)&g_red, sizeof(double) * (2 * wck::kRedBlocks + 4)));
+    WC_HIP(wc_dev_malloc((void **)&g_red, sizeof(double) * (2 * wck::kRedBlocks + 4)));
     WC_HIP(hipHostMalloc((void **)&g_red_host, sizeof(double) * 4, hipHostMallocDefault));
   }
   return WC_OK;

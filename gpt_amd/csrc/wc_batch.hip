@@ -134,15 +134,15 @@ static size_t hbytes(const wc_operator *op) { return half_bytes(op); }
 
 static void free_batch(wc_batch *b) {
   if (!b) return;
-  for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh, b->rl, b->pl, b->ml, b->xl, b->aD}) (void)hipFree(q);
-  for (void *q : b->src) (void)hipFree(q);
-  (void)hipFree(b->st);
-  (void)hipFree(b->partials);
+  for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh, b->rl, b->pl, b->ml, b->xl, b->aD}) (void)wc_dev_free(q);
+  for (void *q : b->src) (void)wc_dev_free(q);
+  (void)wc_dev_free(b->st);
+  (void)wc_dev_free(b->partials);
   for (int p = 0; p < 2; ++p) {
-    (void)hipFree(b->ai_hi[p]);
-    (void)hipFree(b->ai_lo[p]);
+    (void)wc_dev_free(b->ai_hi[p]);
+    (void)wc_dev_free(b->ai_lo[p]);
   }
-  (void)hipFree(b->err);
+  (void)wc_dev_free(b->err);
   if (b->host) (void)hipHostFree(b->host);
   free_run(b->run);
   delete b;
@@ -177,10 +177,10 @@ static int enable_masses(wc_batch *b) {
   const int N = b->N;
   const size_t sh = (size_t)36 * hi->g.Vh * hi->csz, sl = (size_t)36 * lo->g.Vh * lo->csz;
   for (int p = 0; p < 2; ++p) {
-    WC_HIP(hipMalloc(&b->ai_hi[p], sh * N));
-    WC_HIP(hipMalloc(&b->ai_lo[p], sl * N));
+    WC_HIP(wc_dev_malloc(&b->ai_hi[p], sh * N));
+    WC_HIP(wc_dev_malloc(&b->ai_lo[p], sl * N));
   }
-  WC_HIP(hipMalloc((void **)&b->err, sizeof(int)));
+  WC_HIP(wc_dev_malloc((void **)&b->err, sizeof(int)));
   b->mass.assign(N, hi->mass);
   int rc;
   for (int i = 0; i < N; ++i)
@@ -325,7 +325,7 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
   const int N = nslots;
   const size_t hh = hbytes(op) * N, hl = hbytes(inner) * N;
   auto alloc = [&](void **q, size_t bytes) {
-    if (rc == WC_OK && hipMalloc(q, bytes) != hipSuccess) {
+    if (rc == WC_OK && wc_dev_malloc(q, bytes) != hipSuccess) {
       g_err = "wc_batch_create: hipMalloc failed";
       rc = WC_ERR_HIP;
     }
@@ -426,13 +426,13 @@ static int admit_impl(wc_batch *b, int slot, const void *src, const void *guess,
   if (guess && b->mm && b->mass[slot] != hi->mass) {
     void *aD = nullptr;
     const size_t sh = (size_t)36 * hi->g.Vh * hi->csz;
-    WC_HIP(hipMalloc(&aD, sh));
+    WC_HIP(wc_dev_malloc(&aD, sh));
     char *h0 = (char *)b->ai_hi[0] + (size_t)slot * sh, *h1 = (char *)b->ai_hi[1] + (size_t)slot * sh;
     rc = build_mass_blocks(hi, b->mass[slot], h0, h1, pd == 0 ? aD : nullptr, pd == 1 ? aD : nullptr, b->err, s);
     if (rc == WC_OK)
       wck::site_clover<double>(hi->g, 1, pd, aD, 0.0, half(guess, pd), y1, nullptr, hi->open_bc ? 1 : 0, s);
     const hipError_t e = hipStreamSynchronize(s);
-    (void)hipFree(aD);
+    (void)wc_dev_free(aD);
     if (rc) return rc;
     if (e != hipSuccess) WC_FAIL(WC_ERR_HIP, "wc_batch_admit: %s", hipGetErrorString(e));
   } else if (guess) {

@@ -199,8 +199,8 @@ struct BlockScratch {
   cplx *host = nullptr;                      // pinned, (1 + kSlots) N x N
   int next = 0;
   ~BlockScratch() {
-    (void)hipFree(part);
-    (void)hipFree(dmat);
+    (void)wc_dev_free(part);
+    (void)wc_dev_free(dmat);
     if (host) (void)hipHostFree(host);
   }
 };
@@ -221,8 +221,8 @@ int block_cg_impl(wc_operator *op, int N, int pd, void *x, const void *b, double
   void *T = op->tmp[3], *Q = op->tmp[4], *D = op->tmp[5], *Z = op->tmp[9];
   BlockScratch bs;
   const int NN = N * N;
-  WC_HIP(hipMalloc((void **)&bs.part, sizeof(double2) * (size_t)kGramBlocks * NN));
-  WC_HIP(hipMalloc((void **)&bs.dmat, sizeof(double2) * (size_t)NN * (1 + kSlots)));
+  WC_HIP(wc_dev_malloc((void **)&bs.part, sizeof(double2) * (size_t)kGramBlocks * NN));
+  WC_HIP(wc_dev_malloc((void **)&bs.dmat, sizeof(double2) * (size_t)NN * (1 + kSlots)));
   WC_HIP(hipHostMalloc((void **)&bs.host, sizeof(cplx) * (size_t)NN * (1 + kSlots), hipHostMallocDefault));
   *iterations = 0;
   *converged = 0;

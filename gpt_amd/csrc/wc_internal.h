@@ -21,6 +21,14 @@ using wck::HopLaunch;
 
 extern thread_local std::string g_err;  // message of the last failure (wc_api.hip)
 
+// device memory of the library's buffers: hipMalloc / hipFree behind an
+// exact-size cache of freed blocks (wc_alloc.hip)
+hipError_t wc_dev_malloc_raw(void **p, size_t bytes);
+hipError_t wc_dev_free(const void *p);
+template <typename T> inline hipError_t wc_dev_malloc(T **p, size_t bytes) {
+  return wc_dev_malloc_raw((void **)p, bytes);
+}
+
 #define WC_FAIL(code, ...)                       \
   do {                                           \
     char _b[512];                                \
@@ -68,14 +76,14 @@ inline void drop_graphs(IterRun &ir) {
 inline void free_run(IterRun &ir) {
   drop_graphs(ir);
   if (ir.cap) (void)hipStreamDestroy(ir.cap);
-  (void)hipFree(ir.dev);
+  (void)wc_dev_free(ir.dev);
   if (ir.host) (void)hipHostFree(ir.host);
   ir = IterRun();
 }
 
 inline int ensure_run(IterRun &ir) {
   if (ir.dev) return WC_OK;
-  WC_HIP(hipMalloc((void **)&ir.dev, IterRun::kBytes));
+  WC_HIP(wc_dev_malloc((void **)&ir.dev, IterRun::kBytes));
   WC_HIP(hipHostMalloc((void **)&ir.host, IterRun::kBytes, hipHostMallocDefault));
   memset(ir.host, 0, IterRun::kBytes);
   ir.ctl = (int *)ir.dev;
@@ -165,13 +173,13 @@ inline double s16_delta_floor(int code) {
 inline int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
   const size_t bytes = half_bytes(op) * (size_t)nrhs;
   if (bytes > op->tmp_bytes) {
-    for (void *q : op->tmp) (void)hipFree(q);
+    for (void *q : op->tmp) (void)wc_dev_free(q);
     op->tmp.clear();
     op->tmp_bytes = bytes;
   }
   while (op->tmp.size() < n) {
     void *q = nullptr;
-    WC_HIP(hipMalloc(&q, op->tmp_bytes));
+    WC_HIP(wc_dev_malloc(&q, op->tmp_bytes));
     op->tmp.push_back(q);
   }
   return WC_OK;
@@ -181,25 +189,25 @@ inline int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
 inline int ensure_red(wc_operator *op, int nrhs) {
   const size_t np = (size_t)std::max(wck::kRedBlocks, wck::hop_blocks(op->g)) * nrhs;
   if (np > op->npartials) {
-    (void)hipFree(op->partials);
+    (void)wc_dev_free(op->partials);
     op->partials = nullptr;
-    WC_HIP(hipMalloc((void **)&op->partials, sizeof(double) * np));
+    WC_HIP(wc_dev_malloc((void **)&op->partials, sizeof(double) * np));
     op->npartials = np;
   }
   const size_t nd = (size_t)op->g.Vh * nrhs;
   if (nd > op->ndots) {
-    (void)hipFree(op->dots);
+    (void)wc_dev_free(op->dots);
     op->dots = nullptr;
-    WC_HIP(hipMalloc((void **)&op->dots, sizeof(double) * nd));
+    WC_HIP(wc_dev_malloc((void **)&op->dots, sizeof(double) * nd));
     op->ndots = nd;
   }
   const size_t ns = (size_t)16 * nrhs;
   if (ns > op->nst) {
-    (void)hipFree(op->st);
+    (void)wc_dev_free(op->st);
     if (op->host) (void)hipHostFree(op->host);
     op->st = nullptr;
     op->host = nullptr;
-    WC_HIP(hipMalloc((void **)&op->st, sizeof(double) * ns));
+    WC_HIP(wc_dev_malloc((void **)&op->st, sizeof(double) * ns));
     WC_HIP(hipHostMalloc((void **)&op->host, sizeof(double) * ns, hipHostMallocDefault));
     op->nst = ns;
   }
@@ -209,16 +217,16 @@ inline int ensure_red(wc_operator *op, int nrhs) {
 inline void free_op(wc_operator *op) {
   if (!op) return;
   for (int p = 0; p < 2; ++p) {
-    (void)hipFree(op->W[p]);
-    (void)hipFree(op->A[p]);
-    (void)hipFree(op->Ai[p]);
+    (void)wc_dev_free(op->W[p]);
+    (void)wc_dev_free(op->A[p]);
+    (void)wc_dev_free(op->Ai[p]);
   }
-  (void)hipFree(op->Unat);
-  (void)hipFree(op->err);
-  for (void *q : op->tmp) (void)hipFree(q);
-  (void)hipFree(op->partials);
-  (void)hipFree(op->dots);
-  (void)hipFree(op->st);
+  (void)wc_dev_free(op->Unat);
+  (void)wc_dev_free(op->err);
+  for (void *q : op->tmp) (void)wc_dev_free(q);
+  (void)wc_dev_free(op->partials);
+  (void)wc_dev_free(op->dots);
+  (void)wc_dev_free(op->st);
   if (op->host) (void)hipHostFree(op->host);
   free_run(op->run);
   delete op;
