@@ -6,8 +6,8 @@
 // seconds right after large frees, and hipFree synchronises the device, which
 // a graph capture running on another thread does not survive.  Freed blocks
 // are therefore kept per (device, size) and handed out again to a request of
-// the same size; at most a quarter of the device's memory (and 64 GB) is kept
-// that way, beyond which blocks are freed.  An allocation that fails returns
+// the same size; at most an eighth of the device's memory (and 32 GB) is
+// kept that way, beyond which blocks are freed.  An allocation that fails returns
 // every cached block of the device to the runtime and tries once more, and
 // wc_release_cached_memory does the same on request.
 #include <hip/hip_runtime.h>
@@ -22,6 +22,8 @@
 #include "wc_internal.h"
 
 namespace {
+
+constexpr size_t kCacheMax = (size_t)32 << 30;  // bytes kept per device at most
 
 struct Live {
   int dev;
@@ -38,8 +40,8 @@ size_t cap_of(int dev) {
   auto it = g_cap.find(dev);
   if (it != g_cap.end()) return it->second;
   size_t free_b = 0, total = 0;
-  size_t cap = (hipMemGetInfo(&free_b, &total) == hipSuccess) ? total / 4 : 0;
-  cap = std::min(cap, (size_t)64 << 30);
+  size_t cap = (hipMemGetInfo(&free_b, &total) == hipSuccess) ? total / 8 : 0;
+  cap = std::min(cap, kCacheMax);
   g_cap[dev] = cap;
   return cap;
 }

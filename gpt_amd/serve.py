@@ -18,7 +18,8 @@ continuous batching:
   |src - M dst| / |src| meets the request's tolerance;
 * a new group is created only if its HBM footprint fits the budget left
   (``group_bytes``); otherwise the request waits, and idle groups are
-  released;
+  released (their device blocks stay in the library's cache, so the next
+  group of the same lattice and size allocates nothing);
 * groups step on their own HIP streams from worker threads (ctypes releases
   the GIL), so the work of different groups overlaps on the device.
 
@@ -221,7 +222,7 @@ class server:
     phases, ...); nslots: slots per group; streams: HIP streams the groups
     are spread over; iters_per_step: inner iterations a group runs before the
     scheduler looks again; hbm_budget: bytes the groups may hold (default 80%
-    of the device memory free at start)."""
+    of the device memory free at start, less the library's block cache)."""
 
     def __init__(self, params, nslots=12, streams=4, iters_per_step=25, hbm_budget=None,
                  inner_precision=core.single, delta=0.01, inner_storage="auto", iteration_chunk=None):
@@ -234,8 +235,10 @@ class server:
         # cap on the inner iterations per captured graph run (None: automatic; 1: no graphs)
         self.iteration_chunk = iteration_chunk
         if hbm_budget is None:
-            free, _ = torch.cuda.mem_get_info()
-            hbm_budget = int(0.8 * free)
+            # 80 % of the free memory, less what the library may keep cached
+            # (an eighth of the device, at most 32 GB)
+            free, total = torch.cuda.mem_get_info()
+            hbm_budget = int(0.8 * free) - min(total // 8, 32 << 30)
         self.hbm_budget = int(hbm_budget)
         self.hbm_used = 0
         self.fields = {}

@@ -300,8 +300,8 @@ int wc_set_iteration_chunk(wc_operator *op, int chunk);
 int wc_trim(wc_operator *op);
 int wc_iteration_runs(const wc_operator *op, long *graph_runs, long *direct_runs);
 /* The library keeps device blocks it frees (operators, scratch, batches) for
- * a later request of the same size, up to a quarter of the device's memory
- * (at most 64 GB): re-creating an operator or batch then allocates nothing,
+ * a later request of the same size, up to an eighth of the device's memory
+ * (at most 32 GB): re-creating an operator or batch then allocates nothing,
  * and freeing one does not synchronise the device.  An allocation that
  * fails returns the cached blocks first and tries again.  This returns
  * every cached block of the current device to the runtime now and reports
