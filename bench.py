@@ -783,6 +783,19 @@ def c1_measure(torch, g, dev, reps=3):
     return out
 
 
+def release_device_caches(torch):
+    """return the blocks torch and the library keep cached to the device, so
+    that the free memory reported is what a new allocation can get"""
+    import ctypes
+
+    from gpt_amd import _lib
+
+    torch.cuda.synchronize()
+    n = ctypes.c_long()
+    _lib.check(_lib.load().wc_release_cached_memory(ctypes.byref(n)))
+    torch.cuda.empty_cache()
+
+
 def c4_batch(Vh, nrhs, lt, free_bytes):
     """the largest batch that divides nrhs such that a t-slice of an FP64 half
     field of that many right-hand sides fits one buffer descriptor (4 GB; the
@@ -808,7 +821,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
     inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
     grid = g.grid(fd, g.double)
     Vh = int(np.prod(fd)) // 2
-    torch.cuda.empty_cache()
+    release_device_caches(torch)
     nb = c4_batch(Vh, nrhs, fd[3], torch.cuda.mem_get_info()[0])
     srcs = []
     for j in range(nrhs):
@@ -884,7 +897,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
         finally:
             for op in (wN, mh.inner_operator(wN)):
                 op.release()
-            torch.cuda.empty_cache()
+            release_device_caches(torch)
         return res_out
 
     while True:
