@@ -25,6 +25,7 @@ thread_local std::string g_err;
 std::mutex g_shape_mu;
 std::map<ShapeKey, wck::HopShape> g_shape_cache;
 std::map<ShapeKey, int> g_mma_cache;
+std::map<ShapeKey, wck::HopShape> g_shape16_cache;
 
 // gamma matrices of gpt's chiral basis (lib/gpt/core/gamma.py:28-42)
 typedef std::complex<double> cdbl;
@@ -820,6 +821,7 @@ static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *m
   if (rc) return rc;
   op->shape[nrhs] = best;
   op->shape_set[nrhs] = true;
+  op->shape_user[nrhs] = false;
   {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     g_shape_cache[shape_key(op, nrhs)] = best;
@@ -829,21 +831,25 @@ static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *m
   return WC_OK;
 }
 
-// Automatic hop form (op->mma == 2) of an FP32 operator whose 16-bit inner
-// hops can run on the matrix cores (a multiple of 4 right-hand sides): the
-// normal operator's first 16-bit hop (clover epilogue, FP16 in and out) is
-// timed in both forms and the faster is kept process-wide for the lattice,
-// link format and class of right-hand-side counts.  Both forms give results
-// within the FP16 storage's rounding of each other, so the choice changes
-// speed, not the solve's tolerance.  The input is a pseudo-random FP16
-// pattern (all-zero input runs faster, and not by the same amount in both
-// forms); the forms are timed in two alternating rounds, the better round of
-// each counting.  Scratch: tmp[0..1].
-static int pick_mma(wc_operator *op, int N, hipStream_t s) {
-  if (op->mma != 2 || op->prec != WC_SINGLE || N < 4 || N % 4) return WC_OK;
+// Launch shape and form of the hops with 16-bit spinors (FP32 operator),
+// once per lattice, link format and class of right-hand-side counts
+// (process-wide): the normal operator's first 16-bit hop (clover epilogue,
+// FP16 in and out) is timed with the FP32 hop's tuned shape and a few
+// others (64-512 threads, occupancy caps), and, in automatic form
+// (op->mma == 2, a multiple of 4 right-hand sides), on the matrix cores.
+// The 16-bit hops move a third of the FP32 hops' spinor bytes and are bound
+// by the VALUs instead, so their best shape is not always the FP32 one.  All
+// candidates give results within the FP16 storage's rounding of each other
+// (the vector shapes bitwise the same), so the choice changes speed, not the
+// solve's tolerance.  The input is a pseudo-random FP16 pattern (all-zero
+// input runs faster, and not by the same amount for every candidate); the
+// candidates are timed in two alternating rounds, the better round of each
+// counting.  Scratch: tmp[0..1].
+static int pick_s16(wc_operator *op, int N, hipStream_t s) {
+  if (op->prec != WC_SINGLE) return WC_OK;
   {
     std::lock_guard<std::mutex> lk(g_shape_mu);
-    if (g_mma_cache.count(shape_key(op, N))) return WC_OK;
+    if (g_shape16_cache.count(shape_key(op, N))) return WC_OK;
   }
   int rc = ensure_tmp(op, 2, N);
   if (rc) return rc;
@@ -851,29 +857,69 @@ static int pick_mma(wc_operator *op, int N, hipStream_t s) {
   wck::fill_pattern<float>(n, op->tmp[1], s);
   wck::convert_f2h(n, op->tmp[1], op->tmp[0], kS16Half, s);
   wck::zero<float>(n, op->tmp[1], s);
+  // candidates: (shape, form)
+  std::vector<std::pair<wck::HopShape, int>> cand;
+  auto add = [&](const wck::HopShape &sh, int form) {
+    if (!shape_valid(op, sh) || !stage_fits(op, N, sh) || sh.rb > N) return;
+    for (auto &c : cand)
+      if (c.second == form && c.first.rb == sh.rb && c.first.tpb == sh.tpb && c.first.occ == sh.occ &&
+          c.first.seg == sh.seg && c.first.ls == sh.ls)
+        return;
+    cand.push_back({sh, form});
+  };
+  const wck::HopShape base = shape_for(op, N);
+  add(base, 0);
+  for (int tpb : {2, 4, 8})
+    for (int oc : {0, 4, 8}) {
+      wck::HopShape sh;
+      sh.rb = 1;
+      sh.tpb = tpb;
+      sh.occ = oc;
+      sh.seg = 1;
+      sh.ls = N > 1 ? 1 : 0;
+      add(sh, 0);
+    }
+  const bool try_mma = op->mma == 2 && N >= 4 && N % 4 == 0;
+  if (try_mma) add(base, 1);
   hipEvent_t e0, e1;
   WC_HIP(hipEventCreate(&e0));
   WC_HIP(hipEventCreate(&e1));
-  float best[2] = {-1.0f, -1.0f};
-  for (int round = 0; round < 2; ++round)
-    for (int form = 0; form < 2; ++form) {
+  std::vector<float> best(cand.size(), -1.0f);
+  for (int round = 0; round < 2 && !rc; ++round)
+    for (size_t i = 0; i < cand.size() && !rc; ++i) {
       HopLaunch a = hop_args<float>(op, N, 0, 0, op->has_clover ? wck::EPI_CLOV : 0, op->tmp[1], op->tmp[0]);
       a.clov = op->has_clover ? op->Ai[0] : nullptr;
       a.half_in = a.half_out = kS16Half;
-      a.mma = form;
-      if (round == 0) WC_HOP(float, a, s);  // warm-up (first launch of the form)
-      WC_HIP(hipEventRecord(e0, s));
-      for (int k = 0; k < 3; ++k) WC_HOP(float, a, s);
-      WC_HIP(hipEventRecord(e1, s));
-      WC_HIP(hipEventSynchronize(e1));
-      float ms = 0.0f;
-      WC_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (best[form] < 0.0f || ms < best[form]) best[form] = ms;
+      a.shape = cand[i].first;
+      a.mma = cand[i].second;
+      if (cand[i].second == 1 && !wck::hop_mma_applies(a)) continue;
+      // shapes this device cannot launch, or whose staging area is too large
+      if (round == 0 && (!wck::hop<float>(a, s) || hipGetLastError() != hipSuccess)) continue;
+      if (round == 1 && best[i] < 0.0f) continue;
+      auto timed = [&]() -> int {
+        WC_HIP(hipEventRecord(e0, s));
+        for (int k = 0; k < 3; ++k) WC_HOP(float, a, s);
+        WC_HIP(hipEventRecord(e1, s));
+        WC_HIP(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        WC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (best[i] < 0.0f || ms < best[i]) best[i] = ms;
+        return WC_OK;
+      };
+      rc = timed();
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  int bv = -1, bm = -1;  // fastest vector shape, fastest candidate overall
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (best[i] < 0.0f) continue;
+    if (cand[i].second == 0 && (bv < 0 || best[i] < best[bv])) bv = (int)i;
+    if (bm < 0 || best[i] < best[bm]) bm = (int)i;
+  }
   std::lock_guard<std::mutex> lk(g_shape_mu);
-  g_mma_cache[shape_key(op, N)] = best[1] < best[0] ? 1 : 0;
+  g_shape16_cache[shape_key(op, N)] = bv >= 0 ? cand[bv].first : base;
+  if (try_mma && bm >= 0) g_mma_cache[shape_key(op, N)] = cand[bm].second;
   return WC_OK;
 }
 
@@ -890,7 +936,7 @@ int maybe_tune(wc_operator *op, int nrhs, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     have = g_shape_cache.count(shape_key(op, nrhs)) > 0;
   }
-  if (have) return pick_mma(op, nrhs, s);
+  if (have) return pick_s16(op, nrhs, s);
   wck::HopShape sh;
   int rc = WC_OK;
   if (load_tuned(op, nrhs, sh)) {
@@ -899,7 +945,7 @@ int maybe_tune(wc_operator *op, int nrhs, hipStream_t s) {
   } else {
     rc = tune_hop(op, nrhs, 3, s, nullptr);
   }
-  if (rc == WC_OK) rc = pick_mma(op, nrhs, s);
+  if (rc == WC_OK) rc = pick_s16(op, nrhs, s);
   return rc;
 }
 
@@ -927,6 +973,7 @@ int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape) {
   if (rc) return rc;
   if (!shape) {  // back to automatic
     op->shape_set[nrhs] = false;
+    op->shape_user[nrhs] = false;
     return WC_OK;
   }
   wck::HopShape sh;
@@ -943,6 +990,7 @@ int wc_set_hop_shape(wc_operator *op, int nrhs, const int *shape) {
             wck::kMaxStageBytes);
   op->shape[nrhs] = sh;
   op->shape_set[nrhs] = true;
+  op->shape_user[nrhs] = true;
   return WC_OK;
 }
 
@@ -1062,6 +1110,7 @@ int wc_apply_block_s16(wc_operator *op, int opcode, int nrhs, int storage, void 
   hipStream_t s = (hipStream_t)stream;
   HopLaunch a = hop_args<float>(op, nrhs, 1 - src_cb, opcode == 2004, 0, src, dst);
   a.half_in = a.half_out = code;
+  a.shape = shape16_for(op, nrhs);
   WC_HOP(float, a, s);
   WC_LAUNCHED();
   return WC_OK;

@@ -129,6 +129,7 @@ struct wc_operator {
   int bps = 0;                  // 64-site blocks per time slice (0: not a whole number)
   wck::HopShape shape[65];      // hop launch shape per right-hand-side count
   bool shape_set[65] = {};      // shape chosen (by the tuner or the caller)
+  bool shape_user[65] = {};     // shape set by the caller (wc_set_hop_shape): used for 16-bit hops too
   int chunk = 0;                // iterations per run: 0 automatic, 1 one at a time, else at most this many
   // FP16-storage hops of 4k right-hand sides: 0 vector ALUs, 1 matrix cores,
   // 2 automatic (the faster of the two, timed per lattice and class of
@@ -310,6 +311,26 @@ inline wck::HopShape shape_for(const wc_operator *op, int nrhs) {
   return default_shape(op, nrhs);
 }
 
+// launch shape of the hops with 16-bit spinors (FP32 operator): the caller's
+// shape if one was set, else the one timed fastest for those hops (pick_s16,
+// process-wide per lattice, link format and class of right-hand-side
+// counts), else the FP32 hop's
+extern std::map<ShapeKey, wck::HopShape> g_shape16_cache;
+inline wck::HopShape shape16_for(const wc_operator *op, int nrhs) {
+  if (nrhs >= 1 && nrhs <= 64 && op->shape_user[nrhs]) return op->shape[nrhs];
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    auto it = g_shape16_cache.find(shape_key(op, nrhs));
+    if (it != g_shape16_cache.end()) {
+      wck::HopShape sh = it->second;
+      if (sh.rb > nrhs) sh.rb = 1;
+      if (!stage_fits(op, nrhs, sh)) sh.ls = 0;
+      return sh;
+    }
+  }
+  return shape_for(op, nrhs);
+}
+
 // the hop form timed faster for the operator's lattice, link format and
 // class of right-hand-side counts (process-wide, like the shapes), or -1
 extern std::map<ShapeKey, int> g_mma_cache;
@@ -402,6 +423,7 @@ inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, cons
     HopLaunch a = hop_args<R>(op, nrhs, pc, 0, wck::EPI_CLOV, aD, tC);
     a.clov = clovC;
     a.half_in = a.half_out = half;
+    if (half) a.shape = shape16_for(op, nrhs);
     WC_HOP(R, a, s);
     HopLaunch b = hop_args<R>(op, nrhs, pd, 0, wck::EPI_AXPY | (which == 0 && dotv ? wck::EPI_DOT : 0), tC,
                               dst_mpc);
@@ -409,6 +431,7 @@ inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, cons
     b.dotv = dotv;
     b.dots = dots;
     b.half_in = b.half_out = half;
+    if (half) b.shape = a.shape;
     WC_HOP(R, b, s);
     if (which == 0 && dotv) wck::dot_blocks(op->g, nrhs, dots, partials, s, op->halt_cur);
     src = dst_mpc;
@@ -417,6 +440,7 @@ inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, cons
     HopLaunch a = hop_args<R>(op, nrhs, pc, 1, wck::EPI_CLOV, src, tC);
     a.clov = clovC;
     a.half_in = a.half_out = half;
+    if (half) a.shape = shape16_for(op, nrhs);
     WC_HOP(R, a, s);
     HopLaunch b = hop_args<R>(op, nrhs, pd, 1, wck::EPI_CLOV | wck::EPI_AXPY | (dotv ? wck::EPI_DOT : 0), tC,
                               out);
@@ -425,6 +449,7 @@ inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, cons
     b.dotv = dotv;
     b.dots = dots;
     b.half_in = half;  // the product itself stays in the working precision
+    if (half) b.shape = a.shape;
     WC_HOP(R, b, s);
     if (dotv) wck::dot_blocks(op->g, nrhs, dots, partials, s, op->halt_cur);
   }
@@ -485,12 +510,13 @@ inline void op_key(const wc_operator *op, int nrhs, std::vector<uintptr_t> &key)
     memcpy(&u, &d, sizeof u);
     return (uintptr_t)u;
   };
-  const wck::HopShape sh = shape_for(op, nrhs);
+  const wck::HopShape sh = shape_for(op, nrhs), s16 = shape16_for(op, nrhs);
   auto tmp = [&](size_t i) { return i < op->tmp.size() ? (uintptr_t)op->tmp[i] : (uintptr_t)0; };
   for (uintptr_t v : {(uintptr_t)op, (uintptr_t)op->W[0], (uintptr_t)op->W[1], (uintptr_t)clov_inv(op, 0),
                       (uintptr_t)clov_inv(op, 1), (uintptr_t)clov_rstride(op), tmp(0), tmp(1), tmp(2), (uintptr_t)op->dots, (uintptr_t)op->partials, bits(op->diag), bits(op->cs), bits(op->ct), (uintptr_t)op->open_bc,
                       (uintptr_t)op->has_clover, (uintptr_t)sh.rb, (uintptr_t)sh.tpb, (uintptr_t)sh.occ,
-                      (uintptr_t)sh.seg, (uintptr_t)sh.ls, (uintptr_t)op->bps})
+                      (uintptr_t)sh.seg, (uintptr_t)sh.ls, (uintptr_t)op->bps, (uintptr_t)s16.rb, (uintptr_t)s16.tpb,
+                      (uintptr_t)s16.occ, (uintptr_t)s16.seg, (uintptr_t)s16.ls, (uintptr_t)mma_for(op, nrhs)})
     key.push_back(v);
   for (int mu = 0; mu < 4; ++mu) {
     key.push_back(bits(op->p.boundary_phases[mu][0]));
