@@ -784,13 +784,18 @@ def c1_measure(torch, g, dev, reps=3):
 
 
 def release_device_caches(torch):
-    """return the blocks torch and the library keep cached to the device, so
-    that the free memory reported is what a new allocation can get"""
+    """return the blocks torch and the library keep cached, and the pool of
+    parked operators, to the device, so that the free memory reported is what
+    a new allocation can get"""
     import ctypes
+    import gc
 
     from gpt_amd import _lib
+    from gpt_amd.qcd.fermion import operator as fop
 
+    gc.collect()  # operators dropped earlier park their handles in the pool
     torch.cuda.synchronize()
+    fop.release_pool()
     n = ctypes.c_long()
     _lib.check(_lib.load().wc_release_cached_memory(ctypes.byref(n)))
     torch.cuda.empty_cache()

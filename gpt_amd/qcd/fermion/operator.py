@@ -35,6 +35,19 @@ operator_tag = {}
 operator_limbo = {}
 
 
+def release_pool():
+    """destroy every parked operator handle (their links and clover); the
+    pool is refilled as operators are dropped.  Returns how many there were."""
+    n = 0
+    for tag, handles in operator_limbo.items():
+        while handles:
+            h = handles.pop()
+            operator_tag.pop(h, None)
+            _lib.check(_lib.load().wc_destroy(h))
+            n += 1
+    return n
+
+
 def _phases(bp):
     out = []
     for b in bp:

@@ -63,3 +63,24 @@ def test_released_operator_blocks_are_reused_then_flushed():
     assert released >= links_bytes
     assert free3 - free2 >= links_bytes
     assert release_cache() == 0
+
+
+def test_parked_operators_are_destroyed_by_release_pool():
+    """an operator dropped without release() parks its handle for reuse
+    (gpt's operator pool, interface.py:23-80); release_pool destroys the
+    parked handles, and a new operator then computes the same bits"""
+    import gc
+
+    from gpt_amd.qcd.fermion import operator as fop
+
+    rng = R.random("alloc_pool")
+    U = G.random_links(R.grid(FD), rng, scale=0.5)
+    src = rng.vspincolor(R.grid(FD))
+    w, ref = meooe(U, src)
+    del w
+    gc.collect()
+    assert fop.release_pool() >= 1
+    assert fop.release_pool() == 0
+    w, out = meooe(U, src)
+    assert torch.equal(out, ref)
+    w.release()
