@@ -114,6 +114,129 @@ __global__ __launch_bounds__(256) void k_lincomb(long rows, int N, int rpb, cons
   out[o].y = (decltype(out[o].y))r.y;
 }
 
+// ------------------------------------------- the same on the matrix cores
+// FP64 fields with 8 <= N <= 48 right-hand sides: both are tall-skinny
+// complex GEMMs (rows x N by N x N, or its transpose), O(N^2) per row, which
+// at N = 48 the vector ALUs ran at about 10 TFLOP/s.  v_mfma_f64_16x16x4f64
+// lane layout (checked on MI355X with a probe): operand A[i][k] in lane
+// i + 16 k, operand B[k][j] in lane j + 16 k, result D[i][j] in lane
+// j + 16 (i % 4), register i / 4.  Every MFMA adds its 4 products in a fixed
+// order, so the sums are the same on every run.
+typedef double wc_v4d __attribute__((ext_vector_type(4)));
+constexpr int kGramBlocksMfma = 2048;  // fixed grid, one wave each
+
+__device__ __forceinline__ wc_v4d mfma64(double a, double b, wc_v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// part[b * N * N + i * N + j] = sum over the rows of workgroup b of
+// conj(A[row][i]) B[row][j]; T = ceil(N / 16) tiles per dimension.  The rows
+// go in steps of 4 (an MFMA's K): lane c + 16 q loads row base + q, RHS
+// 16 t + c of each field.
+template <int T>
+__global__ __launch_bounds__(64) void k_gram_mfma(long rows, int N, const double2 *A, const double2 *B,
+                                                  double2 *part) {
+  const int l = threadIdx.x, c = l & 15, q = l >> 4;
+  const long r0 = rows * blockIdx.x / gridDim.x, r1 = rows * (blockIdx.x + 1) / gridDim.x;
+  wc_v4d re[T][T], im[T][T];
+#pragma unroll
+  for (int ti = 0; ti < T; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < T; ++tj) {
+      re[ti][tj] = wc_v4d{0.0, 0.0, 0.0, 0.0};
+      im[ti][tj] = wc_v4d{0.0, 0.0, 0.0, 0.0};
+    }
+  for (long base = r0; base < r1; base += 4) {
+    const long row = base + q;
+    double2 a[T], b[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int idx = 16 * t + c;
+      const bool ok = row < r1 && idx < N;
+      a[t] = ok ? A[row * N + idx] : make_double2(0.0, 0.0);
+      b[t] = ok ? B[row * N + idx] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int ti = 0; ti < T; ++ti)
+#pragma unroll
+      for (int tj = 0; tj < T; ++tj) {
+        re[ti][tj] = mfma64(a[ti].x, b[tj].x, re[ti][tj]);
+        re[ti][tj] = mfma64(a[ti].y, b[tj].y, re[ti][tj]);
+        im[ti][tj] = mfma64(a[ti].x, b[tj].y, im[ti][tj]);
+        im[ti][tj] = mfma64(-a[ti].y, b[tj].x, im[ti][tj]);
+      }
+  }
+  const long NN = (long)N * N;
+#pragma unroll
+  for (int ti = 0; ti < T; ++ti)
+#pragma unroll
+    for (int tj = 0; tj < T; ++tj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = 16 * ti + q + 4 * r, j = 16 * tj + c;
+        if (i < N && j < N) part[(long)blockIdx.x * NN + (long)i * N + j] = make_double2(re[ti][tj][r], im[ti][tj][r]);
+      }
+}
+
+// out[row][j] = (Y ? Y[row][j] : 0) + sgn * sum_i X[row][i] M[i * N + j]: one
+// wave per workgroup and 16 rows, staged in LDS (row stride N + 1, against
+// bank conflicts) before anything is written, so out may be X or Y.  K step s
+// covers i = 4 s .. 4 s + 3: lane c + 16 q takes X[row c][4 s + q] and
+// M[4 s + q][16 t + c]; the result row q + 4 r, column 16 t + c.
+template <int T>
+__global__ __launch_bounds__(64) void k_lincomb_mfma(long rows, int N, const double2 *X, const double2 *M,
+                                                     const double2 *Y, double sgn, double2 *out) {
+  extern __shared__ double2 xsh[];
+  const int l = threadIdx.x, c = l & 15, q = l >> 4;
+  const long row0 = (long)blockIdx.x * 16;
+  const int nr = (int)((rows - row0) < 16 ? (rows - row0) : 16);
+  const int ld = N + 1;
+  for (int e = l; e < 16 * N; e += 64) {
+    const int rr = e / N, ii = e - rr * N;
+    xsh[rr * ld + ii] = rr < nr ? X[(row0 + rr) * N + ii] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  wc_v4d re[T], im[T];
+#pragma unroll
+  for (int t = 0; t < T; ++t) {
+    re[t] = wc_v4d{0.0, 0.0, 0.0, 0.0};
+    im[t] = wc_v4d{0.0, 0.0, 0.0, 0.0};
+  }
+  const int ks = (N + 3) / 4;
+  for (int st = 0; st < ks; ++st) {
+    const int i = 4 * st + q;
+    const double2 x = i < N ? xsh[c * ld + i] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const int j = 16 * t + c;
+      const double2 m = (i < N && j < N) ? M[i * N + j] : make_double2(0.0, 0.0);
+      re[t] = mfma64(x.x, m.x, re[t]);
+      re[t] = mfma64(-x.y, m.y, re[t]);
+      im[t] = mfma64(x.x, m.y, im[t]);
+      im[t] = mfma64(x.y, m.x, im[t]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int rr = q + 4 * r, j = 16 * t + c;
+      if (rr < nr && j < N) {
+        const long o = (row0 + rr) * N + j;
+        double2 v = make_double2(sgn * re[t][r], sgn * im[t][r]);
+        if (Y) {
+          const double2 y = Y[o];
+          v.x += y.x;
+          v.y += y.y;
+        }
+        out[o] = v;
+      }
+    }
+}
+
+// whether the block recursion's Gram and update kernels run on the matrix cores
+template <typename R> inline bool block_mfma(int N) { return sizeof(R) == 8 && N >= 8 && N <= 48; }
+
 // ----------------------------------------------------- host linear algebra
 // inverse by Gauss-Jordan elimination with partial pivoting; false if singular
 bool inverse(int n, const std::vector<cplx> &a, std::vector<cplx> &inv) {
@@ -221,7 +344,10 @@ int block_cg_impl(wc_operator *op, int N, int pd, void *x, const void *b, double
   void *T = op->tmp[3], *Q = op->tmp[4], *D = op->tmp[5], *Z = op->tmp[9];
   BlockScratch bs;
   const int NN = N * N;
-  WC_HIP(wc_dev_malloc((void **)&bs.part, sizeof(double2) * (size_t)kGramBlocks * NN));
+  const bool mf = block_mfma<R>(N);
+  const int ntile = (N + 15) / 16;  // 16 x 16 tiles per dimension
+  const int nparts = mf ? kGramBlocksMfma : kGramBlocks;
+  WC_HIP(wc_dev_malloc((void **)&bs.part, sizeof(double2) * (size_t)nparts * NN));
   WC_HIP(wc_dev_malloc((void **)&bs.dmat, sizeof(double2) * (size_t)NN * (1 + kSlots)));
   WC_HIP(hipHostMalloc((void **)&bs.host, sizeof(cplx) * (size_t)NN * (1 + kSlots), hipHostMallocDefault));
   *iterations = 0;
@@ -229,9 +355,16 @@ int block_cg_impl(wc_operator *op, int N, int pd, void *x, const void *b, double
 
   // N x N Gram matrix of two fields to the host, G[i][j] = <A_i, B_j>
   auto gram = [&](const void *A, const void *B, std::vector<cplx> &G) -> int {
-    const size_t lds = sizeof(double2) * 2 * kGramTile * N;
-    k_gram<C><<<kGramBlocks, 256, lds, s>>>(rows, N, (const C *)A, (const C *)B, bs.part);
-    k_gram_sum<<<(NN + 255) / 256, 256, 0, s>>>(NN, kGramBlocks, bs.part, bs.dmat);
+    if (mf) {
+      const double2 *a2 = (const double2 *)A, *b2 = (const double2 *)B;
+      if (ntile == 1) k_gram_mfma<1><<<kGramBlocksMfma, 64, 0, s>>>(rows, N, a2, b2, bs.part);
+      else if (ntile == 2) k_gram_mfma<2><<<kGramBlocksMfma, 64, 0, s>>>(rows, N, a2, b2, bs.part);
+      else k_gram_mfma<3><<<kGramBlocksMfma, 64, 0, s>>>(rows, N, a2, b2, bs.part);
+    } else {
+      const size_t lds = sizeof(double2) * 2 * kGramTile * N;
+      k_gram<C><<<kGramBlocks, 256, lds, s>>>(rows, N, (const C *)A, (const C *)B, bs.part);
+    }
+    k_gram_sum<<<(NN + 255) / 256, 256, 0, s>>>(NN, nparts, bs.part, bs.dmat);
     WC_LAUNCHED();
     WC_HIP(hipMemcpyAsync(bs.host, bs.dmat, sizeof(cplx) * NN, hipMemcpyDeviceToHost, s));
     WC_HIP(hipStreamSynchronize(s));
@@ -245,10 +378,20 @@ int block_cg_impl(wc_operator *op, int N, int pd, void *x, const void *b, double
     bs.next = (bs.next + 1) % kSlots;
     memcpy(bs.host + slot, M.data(), sizeof(cplx) * NN);
     WC_HIP(hipMemcpyAsync(bs.dmat + slot, bs.host + slot, sizeof(cplx) * NN, hipMemcpyHostToDevice, s));
-    const int rpb = 256 / N;
-    const unsigned grid = (unsigned)((rows + rpb - 1) / rpb);
-    k_lincomb<C><<<grid, rpb * N, sizeof(double2) * rpb * N, s>>>(rows, N, rpb, (const C *)X, bs.dmat + slot,
-                                                                 (const C *)Y, sgn, (C *)out);
+    if (mf) {
+      const unsigned grid = (unsigned)((rows + 15) / 16);
+      const size_t lds = sizeof(double2) * 16 * (N + 1);
+      const double2 *x2 = (const double2 *)X, *y2 = (const double2 *)Y, *m2 = bs.dmat + slot;
+      double2 *o2 = (double2 *)out;
+      if (ntile == 1) k_lincomb_mfma<1><<<grid, 64, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
+      else if (ntile == 2) k_lincomb_mfma<2><<<grid, 64, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
+      else k_lincomb_mfma<3><<<grid, 64, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
+    } else {
+      const int rpb = 256 / N;
+      const unsigned grid = (unsigned)((rows + rpb - 1) / rpb);
+      k_lincomb<C><<<grid, rpb * N, sizeof(double2) * rpb * N, s>>>(rows, N, rpb, (const C *)X, bs.dmat + slot,
+                                                                   (const C *)Y, sgn, (C *)out);
+    }
     WC_LAUNCHED();
     return WC_OK;
   };
