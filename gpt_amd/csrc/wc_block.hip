@@ -178,60 +178,64 @@ __global__ __launch_bounds__(64) void k_gram_mfma(long rows, int N, const double
       }
 }
 
-// out[row][j] = (Y ? Y[row][j] : 0) + sgn * sum_i X[row][i] M[i * N + j]: one
-// wave per workgroup and 16 rows, staged in LDS (row stride N + 1, against
-// bank conflicts) before anything is written, so out may be X or Y.  K step s
-// covers i = 4 s .. 4 s + 3: lane c + 16 q takes X[row c][4 s + q] and
-// M[4 s + q][16 t + c]; the result row q + 4 r, column 16 t + c.
+// out[row][j] = (Y ? Y[row][j] : 0) + sgn * sum_i X[row][i] M[i * N + j]: a
+// workgroup of 4 waves stages M in LDS once and covers kLincombRows rows,
+// each wave groups of 16 rows in turn.  A wave reads all X values of a group
+// before it writes the group's rows, and no other wave touches them, so out
+// may be X or Y.  K step s covers i = 4 s .. 4 s + 3: lane c + 16 q takes
+// X[row c][4 s + q] and M[4 s + q][16 t + c]; the result row q + 4 r,
+// column 16 t + c.
+constexpr int kLincombRows = 512;  // rows per workgroup: 8 groups of 16 per wave
+
 template <int T>
-__global__ __launch_bounds__(64) void k_lincomb_mfma(long rows, int N, const double2 *X, const double2 *M,
-                                                     const double2 *Y, double sgn, double2 *out) {
-  extern __shared__ double2 xsh[];
-  const int l = threadIdx.x, c = l & 15, q = l >> 4;
-  const long row0 = (long)blockIdx.x * 16;
-  const int nr = (int)((rows - row0) < 16 ? (rows - row0) : 16);
-  const int ld = N + 1;
-  for (int e = l; e < 16 * N; e += 64) {
-    const int rr = e / N, ii = e - rr * N;
-    xsh[rr * ld + ii] = rr < nr ? X[(row0 + rr) * N + ii] : make_double2(0.0, 0.0);
-  }
+__global__ __launch_bounds__(256) void k_lincomb_mfma(long rows, int N, const double2 *X, const double2 *M,
+                                                      const double2 *Y, double sgn, double2 *out) {
+  extern __shared__ double2 msh[];
+  for (int e = threadIdx.x; e < N * N; e += blockDim.x) msh[e] = M[e];
   __syncthreads();
-  wc_v4d re[T], im[T];
-#pragma unroll
-  for (int t = 0; t < T; ++t) {
-    re[t] = wc_v4d{0.0, 0.0, 0.0, 0.0};
-    im[t] = wc_v4d{0.0, 0.0, 0.0, 0.0};
-  }
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, c = l & 15, q = l >> 4;
   const int ks = (N + 3) / 4;
-  for (int st = 0; st < ks; ++st) {
-    const int i = 4 * st + q;
-    const double2 x = i < N ? xsh[c * ld + i] : make_double2(0.0, 0.0);
+  const long wg0 = (long)blockIdx.x * kLincombRows;
+  for (int grp = w; grp < kLincombRows / 16; grp += 4) {
+    const long row0 = wg0 + 16L * grp;
+    if (row0 >= rows) break;
+    const int nr = (int)((rows - row0) < 16 ? (rows - row0) : 16);
+    wc_v4d re[T], im[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
-      const int j = 16 * t + c;
-      const double2 m = (i < N && j < N) ? M[i * N + j] : make_double2(0.0, 0.0);
-      re[t] = mfma64(x.x, m.x, re[t]);
-      re[t] = mfma64(-x.y, m.y, re[t]);
-      im[t] = mfma64(x.x, m.y, im[t]);
-      im[t] = mfma64(x.y, m.x, im[t]);
+      re[t] = wc_v4d{0.0, 0.0, 0.0, 0.0};
+      im[t] = wc_v4d{0.0, 0.0, 0.0, 0.0};
     }
-  }
+    for (int st = 0; st < ks; ++st) {
+      const int i = 4 * st + q;
+      const double2 x = (i < N && c < nr) ? X[(row0 + c) * N + i] : make_double2(0.0, 0.0);
 #pragma unroll
-  for (int t = 0; t < T; ++t)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int rr = q + 4 * r, j = 16 * t + c;
-      if (rr < nr && j < N) {
-        const long o = (row0 + rr) * N + j;
-        double2 v = make_double2(sgn * re[t][r], sgn * im[t][r]);
-        if (Y) {
-          const double2 y = Y[o];
-          v.x += y.x;
-          v.y += y.y;
-        }
-        out[o] = v;
+      for (int t = 0; t < T; ++t) {
+        const int j = 16 * t + c;
+        const double2 m = (i < N && j < N) ? msh[i * N + j] : make_double2(0.0, 0.0);
+        re[t] = mfma64(x.x, m.x, re[t]);
+        re[t] = mfma64(-x.y, m.y, re[t]);
+        im[t] = mfma64(x.x, m.y, im[t]);
+        im[t] = mfma64(x.y, m.x, im[t]);
       }
     }
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = q + 4 * r, j = 16 * t + c;
+        if (rr < nr && j < N) {
+          const long o = (row0 + rr) * N + j;
+          double2 v = make_double2(sgn * re[t][r], sgn * im[t][r]);
+          if (Y) {
+            const double2 y = Y[o];
+            v.x += y.x;
+            v.y += y.y;
+          }
+          out[o] = v;
+        }
+      }
+  }
 }
 
 // whether the block recursion's Gram and update kernels run on the matrix cores
@@ -379,13 +383,13 @@ int block_cg_impl(wc_operator *op, int N, int pd, void *x, const void *b, double
     memcpy(bs.host + slot, M.data(), sizeof(cplx) * NN);
     WC_HIP(hipMemcpyAsync(bs.dmat + slot, bs.host + slot, sizeof(cplx) * NN, hipMemcpyHostToDevice, s));
     if (mf) {
-      const unsigned grid = (unsigned)((rows + 15) / 16);
-      const size_t lds = sizeof(double2) * 16 * (N + 1);
+      const unsigned grid = (unsigned)((rows + kLincombRows - 1) / kLincombRows);
+      const size_t lds = sizeof(double2) * (size_t)N * N;
       const double2 *x2 = (const double2 *)X, *y2 = (const double2 *)Y, *m2 = bs.dmat + slot;
       double2 *o2 = (double2 *)out;
-      if (ntile == 1) k_lincomb_mfma<1><<<grid, 64, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
-      else if (ntile == 2) k_lincomb_mfma<2><<<grid, 64, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
-      else k_lincomb_mfma<3><<<grid, 64, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
+      if (ntile == 1) k_lincomb_mfma<1><<<grid, 256, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
+      else if (ntile == 2) k_lincomb_mfma<2><<<grid, 256, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
+      else k_lincomb_mfma<3><<<grid, 256, lds, s>>>(rows, N, x2, m2, y2, sgn, o2);
     } else {
       const int rpb = 256 / N;
       const unsigned grid = (unsigned)((rows + rpb - 1) / rpb);
