@@ -66,6 +66,9 @@ def parse():
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--serve-streams", type=int, default=4, help="HIP streams (worker threads) of the server")
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="Poisson arrival rate in requests/s (default: --serve-load times the capacity)")
+    ap.add_argument("--serve-min-slots", type=int, default=1, help="smallest batch a serving group opens with")
     ap.add_argument("--no-c1", action="store_true", help="skip the 8^4 point-source propagator")
     ap.add_argument("--c4", default="16x16x16x32:12,16x16x16x32:48,32x32x32x64:12,32x32x32x64:48",
                     help="multi-RHS cases LXxLYxLZxLT:NRHS, comma-separated ('' to skip)")
@@ -501,7 +504,7 @@ def main():
     serve_res = None
     if args.serve > 0:
         serve_res = serve_measure(torch, g, dev, args.serve, npoisson=args.serve_poisson, load=args.serve_load,
-                                  streams=args.serve_streams)
+                                  streams=args.serve_streams, rate=args.serve_rate, min_slots=args.serve_min_slots)
 
     cg_res = None
     if not args.no_cg:
@@ -923,7 +926,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
 
 
 def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10, iteration_chunk=None,
-                  npoisson=200, streams=4):
+                  npoisson=200, streams=4, rate=None, min_slots=1):
     """Serving stream: requests over volumes {8^4, 16^3x32, 24^3x48, 32^3x64},
     3 resident fields per volume (seeds 5101..5103), mass U[0.02, 0.2], 1..12
     Gaussian right-hand sides, tolerance tol on the true residual.  First nreq
@@ -939,7 +942,7 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
 
     vols = [[8, 8, 8, 8], [16, 16, 16, 32], [24, 24, 24, 48], [32, 32, 32, 64]]
     prm = dict(csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0, boundary_phases=[1, 1, 1, -1])
-    srv = serve.server(prm, nslots=nslots, streams=streams, iteration_chunk=iteration_chunk)
+    srv = serve.server(prm, nslots=nslots, streams=streams, iteration_chunk=iteration_chunk, min_slots=min_slots)
     for vi, fd in enumerate(vols):
         for k in range(3):
             srv.add_field((vi, k), smooth_links_device(torch, fd, 0.3, 5101 + k, dev))
@@ -996,7 +999,7 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
     srv.peak_groups = 0
     reqs = make(npoisson)
     mean_rhs = float(np.mean([len(r.sources) for r in reqs]))
-    lam = load * cap["solves_per_s"] / mean_rhs
+    lam = rate if rate else load * cap["solves_per_s"] / mean_rhs
     arrivals = np.cumsum(rng.exponential(1.0 / lam, npoisson))
     t0 = time.perf_counter()
     done = srv.run(list(zip(arrivals.tolist(), reqs)), on_done=progress)

@@ -225,9 +225,11 @@ class server:
     of the device memory free at start, less the library's block cache)."""
 
     def __init__(self, params, nslots=12, streams=4, iters_per_step=25, hbm_budget=None,
-                 inner_precision=core.single, delta=0.01, inner_storage="auto", iteration_chunk=None):
+                 inner_precision=core.single, delta=0.01, inner_storage="auto", iteration_chunk=None,
+                 min_slots=1):
         self.params = {k: v for k, v in dict(params).items() if k not in ("mass", "kappa")}
         self.nslots = int(nslots)
+        self.min_slots = max(1, min(int(min_slots), self.nslots))  # smallest batch a group opens with
         self.iters_per_step = int(iters_per_step)
         self.inner_precision = inner_precision
         self.delta = float(delta)
@@ -301,7 +303,7 @@ class server:
         g = self.groups.get(key)
         if g is None:
             U = self.fields[req.field]
-            nslots = max(1, min(self.nslots, 4 * ((len(req.sources) + 3) // 4)))
+            nslots = max(self.min_slots, min(self.nslots, 4 * ((len(req.sources) + 3) // 4)))
             need = group_bytes(U[0].grid.fdimensions, nslots)
             if self.hbm_used + need > self.hbm_budget:
                 return False

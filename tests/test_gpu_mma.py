@@ -249,3 +249,28 @@ def test_automatic_form_on_a_tuned_lattice():
     for r in range(N):
         tres = np.linalg.norm(srcs[r] - ref.M(dst[r].to_numpy())) / np.linalg.norm(srcs[r])
         assert tres <= 1.05e-10, (r, tres)
+
+
+def test_vector_16bit_hop_same_bits_for_every_shape(links):
+    """the 16-bit hops take their own launch shape (timed when the solvers
+    tune); a caller's set_hop_shape applies to them too, and every vector
+    shape gives the same bits, so the choice changes speed only"""
+    p = CASES["clover"]
+    N = 8
+    srcs = fp16_sources(N, 7800)
+    w = inner_op(links, p, N, True)
+    H = half_field(srcs, w, g.odd)
+    s16 = H.data.to(torch.float16)
+    w.set_hop_mma(False)
+    ref = None
+    for sh in [(1, 1, 0, 1, 0), (1, 2, 4, 1, 1), (1, 8, 0, 1, 1), (1, 4, 8, 1, 0), (1, 4, 4, 2, 1)]:
+        w.set_hop_shape(sh)
+        out = w.hop_s16(s16, 1, n_rhs=N).clone()
+        Y = w.mpc_ne_s16(s16, 1, n_rhs=N).clone()
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = (out, Y)
+        else:
+            assert torch.equal(out, ref[0]) and torch.equal(Y, ref[1]), sh
+    w.set_hop_shape(None)
+    w.set_hop_mma("auto")
