@@ -36,6 +36,7 @@ std::map<std::pair<int, size_t>, std::vector<void *>> g_free;  // cached, by (de
 std::map<int, size_t> g_cached;                           // cached bytes per device
 std::map<int, size_t> g_cap;                              // cache limit per device
 
+// the cache limit of dev, computed on its first allocation (dev current then)
 size_t cap_of(int dev) {
   auto it = g_cap.find(dev);
   if (it != g_cap.end()) return it->second;
@@ -73,6 +74,7 @@ hipError_t wc_dev_malloc_raw(void **p, size_t bytes) {
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lk(g_mu);
+  cap_of(dev);  // the device's cache limit, taken while it is the current one
   auto it = g_free.find({dev, bytes});
   if (it != g_free.end() && !it->second.empty()) {
     *p = it->second.back();
