@@ -835,7 +835,8 @@ static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *m
 // once per lattice, link format and class of right-hand-side counts
 // (process-wide): the normal operator's first 16-bit hop (clover epilogue,
 // FP16 in and out) is timed with the FP32 hop's tuned shape and a few
-// others (64-512 threads, occupancy caps), and, in automatic form
+// others (one or two right-hand sides per thread, 128-512 threads,
+// occupancy caps), and, in automatic form
 // (op->mma == 2, a multiple of 4 right-hand sides), on the matrix cores.
 // The 16-bit hops move a third of the FP32 hops' spinor bytes and are bound
 // by the VALUs instead, so their best shape is not always the FP32 one.  All
@@ -869,16 +870,17 @@ static int pick_s16(wc_operator *op, int N, hipStream_t s) {
   };
   const wck::HopShape base = shape_for(op, N);
   add(base, 0);
-  for (int tpb : {2, 4, 8})
-    for (int oc : {0, 4, 8}) {
-      wck::HopShape sh;
-      sh.rb = 1;
-      sh.tpb = tpb;
-      sh.occ = oc;
-      sh.seg = 1;
-      sh.ls = N > 1 ? 1 : 0;
-      add(sh, 0);
-    }
+  for (int rb = 1; rb <= 2 && rb <= N; ++rb)
+    for (int tpb : {2, 4, 8})
+      for (int oc : {0, 4, 8}) {
+        wck::HopShape sh;
+        sh.rb = rb;
+        sh.tpb = tpb;
+        sh.occ = oc;
+        sh.seg = 1;
+        sh.ls = N > rb ? 1 : 0;
+        add(sh, 0);
+      }
   const bool try_mma = op->mma == 2 && N >= 4 && N % 4 == 0;
   if (try_mma) add(base, 1);
   hipEvent_t e0, e1;
