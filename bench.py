@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--serve-poisson", type=int, default=200, help="serving stream: Poisson-arrival requests")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson arrival rate as a fraction of the measured capacity")
-    ap.add_argument("--serve-streams", type=int, default=8, help="HIP streams (worker threads) of the server")
+    ap.add_argument("--serve-streams", type=int, default=4, help="HIP streams (worker threads) of the server")
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="Poisson arrival rate in requests/s (default: --serve-load times the capacity)")
     ap.add_argument("--serve-min-slots", type=int, default=1, help="smallest batch a serving group opens with")
@@ -926,7 +926,7 @@ def c4_measure(torch, g, dev, fd, nrhs, tol=1e-10, seed=4001):
 
 
 def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10, iteration_chunk=None,
-                  npoisson=200, streams=8, rate=None, min_slots=1):
+                  npoisson=200, streams=4, rate=None, min_slots=1):
     """Serving stream: requests over volumes {8^4, 16^3x32, 24^3x48, 32^3x64},
     3 resident fields per volume (seeds 5101..5103), mass U[0.02, 0.2], 1..12
     Gaussian right-hand sides, tolerance tol on the true residual.  First nreq
