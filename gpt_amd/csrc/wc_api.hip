@@ -503,7 +503,7 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
       } else {
         void *aD = hi->tmp[0];
         wck::fold_site_clover(hi->g, N, pd, clov_inv(hi, pd), 1.0 / hi->diag,
-                              h16 ? sth + 13 * N : nullptr, xl, xh, aD, hi->open_bc ? 1 : 0, s, clov_rstride(hi));
+                              h16 ? sth + 13 * N : nullptr, xl, xh, aD, hi->open_bc ? 1 : 0, s, clov_roff(hi));
         if ((rc2 = mpc_impl<double>(hi, N, pd, 0, mp, xh, nullptr, s, aD))) return rc2;
         wck::resid_d2f(n, N, bs, mp, rl, hi->partials, s);
       }

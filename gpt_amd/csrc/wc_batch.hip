@@ -92,35 +92,62 @@ struct wc_batch {
   bool dirty = true;
   std::vector<double> cref, clast, rate;
   std::vector<long> since;
-  // several masses (wc_batch_admit_mass): each slot has its own clover-inverse
-  // blocks of both parities, FP64 for the outer operator (ai_hi) and in the
-  // inner precision (ai_lo), slot r of parity p at ai_*[p] + r * 36 Vh
-  // complex values; while the batch works, the operators use them through
-  // their `cov` (every clover-inverse application indexes them by slot)
+  // several masses (wc_batch_admit_mass): N sets of clover-inverse blocks of
+  // both parities, FP64 for the outer operator (ai_hi) and in the inner
+  // precision (ai_lo), set k of parity p at ai_*[p] + k * 36 Vh complex
+  // values.  Slot r uses set set_of[r]; the slots of one mass share a set
+  // (set_mass: the mass a set holds, NaN before it is built).  While the
+  // batch works, the operators use them through their `cov`: roff[r] =
+  // set_of[r] * 36 Vh on the device, or one set for all when every occupied
+  // slot has the same one.
   bool mm = false;
   void *ai_hi[2] = {nullptr, nullptr}, *ai_lo[2] = {nullptr, nullptr};
+  long *roff = nullptr;
   int *err = nullptr;
-  std::vector<double> mass;
+  std::vector<double> mass, set_mass;
+  std::vector<int> set_of;
   ClovSets cov_hi, cov_lo;
 };
 
+// the clover-inverse set k of both operators, parity p
+static const void *set_hi(const wc_batch *b, int p, int k) {
+  return (const char *)b->ai_hi[p] + (size_t)k * 36 * b->hi->g.Vh * b->hi->csz;
+}
+static const void *set_lo(const wc_batch *b, int p, int k) {
+  return (const char *)b->ai_lo[p] + (size_t)k * 36 * b->lo->g.Vh * b->lo->csz;
+}
+
 // Sets the operators' per-slot clover inverses for the duration of a batch
 // call (all slots: slot < 0; one slot's blocks as a single set: slot >= 0)
-// and clears them again.
+// and clears them again.  When every occupied slot uses one set, all slots
+// run on that set alone: the kernels then stage its blocks per site rather
+// than load them per right-hand side (the unoccupied slots' results are not
+// used).
 struct SlotClover {
   wc_batch *b;
   ClovSets one_hi, one_lo;
   SlotClover(wc_batch *b_, int slot) : b(b_) {
     if (!b->mm) return;
+    int k = slot >= 0 ? b->set_of[slot] : -1;
     if (slot < 0) {
-      b->hi->cov = &b->cov_hi;
-      b->lo->cov = &b->cov_lo;
-      return;
+      for (int i = 0; i < b->N; ++i) {
+        if (!b->occupied[i]) continue;
+        if (k < 0) {
+          k = b->set_of[i];
+        } else if (k != b->set_of[i]) {
+          k = -1;
+          break;
+        }
+      }
+      if (k < 0) {
+        b->hi->cov = &b->cov_hi;
+        b->lo->cov = &b->cov_lo;
+        return;
+      }
     }
-    const size_t sh = (size_t)36 * b->hi->g.Vh * b->hi->csz, sl = (size_t)36 * b->lo->g.Vh * b->lo->csz;
     for (int p = 0; p < 2; ++p) {
-      one_hi.Ai[p] = (const char *)b->ai_hi[p] + (size_t)slot * sh;
-      one_lo.Ai[p] = (const char *)b->ai_lo[p] + (size_t)slot * sl;
+      one_hi.Ai[p] = set_hi(b, p, k);
+      one_lo.Ai[p] = set_lo(b, p, k);
     }
     b->hi->cov = &one_hi;
     b->lo->cov = &one_lo;
@@ -142,19 +169,37 @@ static void free_batch(wc_batch *b) {
     (void)wc_dev_free(b->ai_hi[p]);
     (void)wc_dev_free(b->ai_lo[p]);
   }
+  (void)wc_dev_free(b->roff);
   (void)wc_dev_free(b->err);
   if (b->host) (void)hipHostFree(b->host);
   free_run(b->run);
   delete b;
 }
 
-// slot's clover inverses for `mass` (both operators; the operator's own
-// blocks are copied when the mass is the operator's and it has a clover term)
-static int set_slot_mass(wc_batch *b, int slot, double mass) {
+// the per-slot set offsets to the device (by value: nothing on the host
+// has to outlive the copy)
+struct SetOffsets {
+  long v[wck::kMaxRhs];
+};
+__global__ void k_set_offsets(SetOffsets o, int N, long *roff) {
+  if ((int)threadIdx.x < N) roff[threadIdx.x] = o.v[threadIdx.x];
+}
+static int send_offsets(wc_batch *b) {
+  SetOffsets o{};
+  for (int i = 0; i < b->N; ++i) o.v[i] = (long)b->set_of[i] * 36 * b->hi->g.Vh;
+  k_set_offsets<<<1, wck::kMaxRhs, 0, b->s>>>(o, b->N, b->roff);
+  WC_LAUNCHED();
+  return WC_OK;
+}
+
+// set k of both operators for `mass` (the operator's own blocks are copied
+// when the mass is the operator's and it has a clover term)
+static int build_set(wc_batch *b, int k, double mass) {
   wc_operator *hi = b->hi, *lo = b->lo;
   const size_t sh = (size_t)36 * hi->g.Vh * hi->csz, sl = (size_t)36 * lo->g.Vh * lo->csz;
-  char *h0 = (char *)b->ai_hi[0] + (size_t)slot * sh, *h1 = (char *)b->ai_hi[1] + (size_t)slot * sh;
-  char *l0 = (char *)b->ai_lo[0] + (size_t)slot * sl, *l1 = (char *)b->ai_lo[1] + (size_t)slot * sl;
+  void *h0 = (void *)set_hi(b, 0, k), *h1 = (void *)set_hi(b, 1, k);
+  void *l0 = (void *)set_lo(b, 0, k), *l1 = (void *)set_lo(b, 1, k);
+  b->set_mass[k] = NAN;  // until it is built
   int rc;
   if (mass == hi->mass && hi->has_clover && lo->has_clover && mass == lo->mass) {
     WC_HIP(hipMemcpyAsync(h0, hi->Ai[0], sh, hipMemcpyDeviceToDevice, b->s));
@@ -165,12 +210,36 @@ static int set_slot_mass(wc_batch *b, int slot, double mass) {
     if ((rc = build_mass_blocks(hi, mass, h0, h1, nullptr, nullptr, b->err, b->s))) return rc;
     if ((rc = build_mass_blocks(lo, mass, l0, l1, nullptr, nullptr, b->err, b->s))) return rc;
   }
-  b->mass[slot] = mass;
+  b->set_mass[k] = mass;
   return WC_OK;
 }
 
-// switch a batch to one clover-inverse set per slot, every slot at the
-// operator's mass to begin with
+// slot's clover inverses for `mass`: the set of another slot of that mass,
+// else a set that no other occupied slot uses, built for it
+static int set_slot_mass(wc_batch *b, int slot, double mass) {
+  const int N = b->N;
+  int k = -1, rc;
+  for (int j = 0; j < N && k < 0; ++j)
+    if (b->set_mass[j] == mass) k = j;
+  if (k < 0) {
+    std::vector<char> used(N, 0);
+    for (int i = 0; i < N; ++i)
+      if (b->occupied[i] && i != slot) used[b->set_of[i]] = 1;
+    // N sets and at most N - 1 other occupied slots: one is free
+    for (int j = 0; j < N && k < 0; ++j)
+      if (!used[j]) k = j;
+    if ((rc = build_set(b, k, mass))) return rc;
+  }
+  b->mass[slot] = mass;
+  if (b->set_of[slot] != k) {
+    b->set_of[slot] = k;
+    return send_offsets(b);
+  }
+  return WC_OK;
+}
+
+// switch a batch to clover-inverse sets per slot, every slot on set 0 at
+// the operator's mass to begin with
 static int enable_masses(wc_batch *b) {
   if (b->mm) return WC_OK;
   wc_operator *hi = b->hi, *lo = b->lo;
@@ -180,17 +249,19 @@ static int enable_masses(wc_batch *b) {
     WC_HIP(wc_dev_malloc(&b->ai_hi[p], sh * N));
     WC_HIP(wc_dev_malloc(&b->ai_lo[p], sl * N));
   }
+  WC_HIP(wc_dev_malloc(&b->roff, sizeof(long) * N));
   WC_HIP(wc_dev_malloc((void **)&b->err, sizeof(int)));
   b->mass.assign(N, hi->mass);
+  b->set_mass.assign(N, NAN);
+  b->set_of.assign(N, 0);
   int rc;
-  for (int i = 0; i < N; ++i)
-    if ((rc = set_slot_mass(b, i, hi->mass))) return rc;
+  if ((rc = build_set(b, 0, hi->mass))) return rc;
+  if ((rc = send_offsets(b))) return rc;
   for (int p = 0; p < 2; ++p) {
     b->cov_hi.Ai[p] = b->ai_hi[p];
     b->cov_lo.Ai[p] = b->ai_lo[p];
   }
-  b->cov_hi.rstride = 36 * hi->g.Vh;
-  b->cov_lo.rstride = 36 * lo->g.Vh;
+  b->cov_hi.roff = b->cov_lo.roff = b->roff;
   b->mm = true;
   return WC_OK;
 }
@@ -226,7 +297,7 @@ static int batch_reliable(wc_batch *b) {
     void *aD = hi->tmp[0];
     wck::fold_site_clover(hi->g, N, b->pd, clov_inv(hi, b->pd), 1.0 / hi->diag,
                           b->h16 ? b->st + 13 * N : nullptr, b->xl, b->xh, aD, hi->open_bc ? 1 : 0, s,
-                          clov_rstride(hi));
+                          clov_roff(hi));
     if ((rc = mpc_impl<double>(hi, N, b->pd, 0, b->mp, b->xh, nullptr, s, aD, b->partials))) return rc;
     wck::resid_d2f(n, N, b->bs, b->mp, b->rl, b->partials, s);
     wck::reduce_final_rhs(b->partials, wck::kRedBlocks, N, b->st + 8 * N, s);
@@ -427,7 +498,9 @@ static int admit_impl(wc_batch *b, int slot, const void *src, const void *guess,
     void *aD = nullptr;
     const size_t sh = (size_t)36 * hi->g.Vh * hi->csz;
     WC_HIP(wc_dev_malloc(&aD, sh));
-    char *h0 = (char *)b->ai_hi[0] + (size_t)slot * sh, *h1 = (char *)b->ai_hi[1] + (size_t)slot * sh;
+    // the slot's inverse set is written again with the values it holds
+    // (the same build), in stream order with every kernel that reads it
+    void *h0 = (void *)set_hi(b, 0, b->set_of[slot]), *h1 = (void *)set_hi(b, 1, b->set_of[slot]);
     rc = build_mass_blocks(hi, b->mass[slot], h0, h1, pd == 0 ? aD : nullptr, pd == 1 ? aD : nullptr, b->err, s);
     if (rc == WC_OK)
       wck::site_clover<double>(hi->g, 1, pd, aD, 0.0, half(guess, pd), y1, nullptr, hi->open_bc ? 1 : 0, s);

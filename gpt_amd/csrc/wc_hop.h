@@ -312,7 +312,7 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   C *stage = (C *)wc_dyn_lds;
   const long v0 = Gt == 1 ? q0 : (long)((uint32_t)q0 / (uint32_t)Gt);
   int ns = 0;
-  const bool clov_staged = LS && (EPI & EPI_CLOV) && a.clov && a.clov_rstride == 0;
+  const bool clov_staged = LS && (EPI & EPI_CLOV) && a.clov && !a.clov_roff;
   if constexpr (LS) {
     long v1 = (long)((uint32_t)(q0 + (long)blockDim.x - 1) / (uint32_t)Gt);
     if (v1 > g.Vh - 1) v1 = g.Vh - 1;
@@ -385,12 +385,12 @@ __global__ __launch_bounds__(512, (hop_min_waves<R, RB>())) void k_hop(const Hop
   } else if (EPI & EPI_CLOV) {
     if (LS && clov_staged) {
       clov_apply_inplace_at<R, RB>(stage + 72 * ns + sv, (long)ns, acc);
-    } else if (a.clov && a.clov_rstride) {
+    } else if (a.clov && a.clov_roff) {
       // one set of blocks per right-hand side (a batch of several masses)
 #pragma unroll
       for (int j = 0; j < RB; ++j) {
         C o[12];
-        clov_apply<R>((const C *)a.clov + (long)rl[j] * a.clov_rstride, h, acc[j], o);
+        clov_apply<R>((const C *)a.clov + a.clov_roff[rl[j]], h, acc[j], o);
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[j][k] = o[k];
       }
@@ -469,7 +469,7 @@ static bool hop_epi(const HopLaunch &a, hipStream_t s) {
   // the workgroups per CU at shape.occ
   size_t lds = a.shape.occ > 0 ? (size_t)((kLdsPerCU / a.shape.occ) & ~1023) : 0;
   if (a.shape.ls == 1) {
-    const size_t st = (size_t)(72 + ((a.epi & EPI_CLOV) && a.clov && !a.clov_rstride ? 36 : 0)) *
+    const size_t st = (size_t)(72 + ((a.epi & EPI_CLOV) && a.clov && !a.clov_roff ? 36 : 0)) *
                       (size_t)stage_sites(threads, G) * sizeof(C);
     if (st > (size_t)kMaxStageBytes) return false;
     lds = std::max(lds, st);

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kMmaThreads, 4) void k_hop_mma(const HopLaunch a) {
 #pragma unroll
     for (int e = 0; e < 9; ++e) lw[(s * 8 + dir) * 9 + e] = u[e];
   }
-  const bool clov_staged = (EPI & EPI_CLOV) && a.clov && a.clov_rstride == 0;
+  const bool clov_staged = (EPI & EPI_CLOV) && a.clov && !a.clov_roff;
   if (clov_staged)
     for (int i = tid; i < 36 * kMmaSites; i += kMmaThreads)
       lc[i] = ((const C *)a.clov)[aidx(i / kMmaSites, site0 + i % kMmaSites, 36)];
@@ -292,7 +292,7 @@ __global__ __launch_bounds__(kMmaThreads, 4) void k_hop_mma(const HopLaunch a) {
         } else if (a.clov) {
           // one set of blocks per right-hand side (a batch of several masses)
           C o[12];
-          clov_apply<float>((const C *)a.clov + (long)rhs * a.clov_rstride, h, v[0], o);
+          clov_apply<float>((const C *)a.clov + a.clov_roff[rhs], h, v[0], o);
 #pragma unroll
           for (int k = 0; k < 12; ++k) v[0][k] = o[k];
         } else {

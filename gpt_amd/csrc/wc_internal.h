@@ -96,12 +96,14 @@ inline int ensure_run(IterRun &ir) {
 }
 
 // Clover-inverse blocks per right-hand side (a batch whose slots have
-// different masses): right-hand side r of parity p uses Ai[p] + r * rstride
-// (complex elements).  While an operator's `cov` points here, every
+// different masses): right-hand side r of parity p uses Ai[p] + roff[r]
+// (roff: device array of offsets in complex elements; several right-hand
+// sides of one mass share a set).  roff null: Ai[p] serves every
+// right-hand side.  While an operator's `cov` points here, every
 // clover-inverse application of that operator uses these blocks.
 struct ClovSets {
   const void *Ai[2] = {nullptr, nullptr};
-  long rstride = 0;
+  const long *roff = nullptr;
 };
 
 struct wc_operator {
@@ -141,13 +143,13 @@ struct wc_operator {
 };
 
 // the clover-inverse blocks of parity p as the operator applies them now
-// (null: the scalar 1 / diag), their stride between right-hand sides, and
+// (null: the scalar 1 / diag), their per-RHS offsets (null: one set), and
 // the clover blocks themselves (never per right-hand side)
 inline const void *clov_inv(const wc_operator *op, int p) {
   if (op->cov) return op->cov->Ai[p];
   return op->has_clover ? op->Ai[p] : nullptr;
 }
-inline long clov_rstride(const wc_operator *op) { return op->cov ? op->cov->rstride : 0; }
+inline const long *clov_roff(const wc_operator *op) { return op->cov ? op->cov->roff : nullptr; }
 inline const void *clov_fwd(const wc_operator *op, int p) { return op->has_clover ? op->A[p] : nullptr; }
 
 inline size_t half_bytes(const wc_operator *op) { return (size_t)op->g.Vh * 12 * op->csz; }
@@ -372,7 +374,7 @@ inline HopLaunch hop_args(const wc_operator *op, int nrhs, int pout, int dag, in
   a.bps = op->bps;
   a.halt = op->halt_cur;
   a.mma = mma_for(op, nrhs);
-  a.clov_rstride = clov_rstride(op);
+  a.clov_roff = clov_roff(op);
   return a;
 }
 
@@ -392,7 +394,7 @@ inline void site_op(const wc_operator *op, int nrhs, int p, int inv, const void 
   const void *B = inv ? clov_inv(op, p) : clov_fwd(op, p);
   double sc = inv ? 1.0 / op->diag : op->diag;
   wck::site_clover<R>(op->g, nrhs, p, B, sc, in, out, y, op->open_bc ? 1 : 0, s, half,
-                      inv ? clov_rstride(op) : 0);
+                      inv ? clov_roff(op) : nullptr);
 }
 
 // Mpc (which 0), Mpc^dag (1) or Mpc^dag Mpc (2) with D = pd on nrhs
@@ -496,7 +498,7 @@ inline int inner_iteration(wc_operator *lo, int N, int pd, void *r, void *p, voi
   wck::cg_update_r<R>(n, N, st, r, mmp, partials, s, halt);
   wck::cg_beta(partials, wck::kRedBlocks, N, st, s, halt);
   wck::cg_update_xp<R>(lo->g, N, pd, st, r, p, x, clov_inv(lo, pd), 1.0 / lo->diag, aD,
-                       lo->open_bc ? 1 : 0, s, half, halt, clov_rstride(lo));
+                       lo->open_bc ? 1 : 0, s, half, halt, clov_roff(lo));
   WC_LAUNCHED();
   return WC_OK;
 }
@@ -513,7 +515,7 @@ inline void op_key(const wc_operator *op, int nrhs, std::vector<uintptr_t> &key)
   const wck::HopShape sh = shape_for(op, nrhs), s16 = shape16_for(op, nrhs);
   auto tmp = [&](size_t i) { return i < op->tmp.size() ? (uintptr_t)op->tmp[i] : (uintptr_t)0; };
   for (uintptr_t v : {(uintptr_t)op, (uintptr_t)op->W[0], (uintptr_t)op->W[1], (uintptr_t)clov_inv(op, 0),
-                      (uintptr_t)clov_inv(op, 1), (uintptr_t)clov_rstride(op), tmp(0), tmp(1), tmp(2), (uintptr_t)op->dots, (uintptr_t)op->partials, bits(op->diag), bits(op->cs), bits(op->ct), (uintptr_t)op->open_bc,
+                      (uintptr_t)clov_inv(op, 1), (uintptr_t)clov_roff(op), tmp(0), tmp(1), tmp(2), (uintptr_t)op->dots, (uintptr_t)op->partials, bits(op->diag), bits(op->cs), bits(op->ct), (uintptr_t)op->open_bc,
                       (uintptr_t)op->has_clover, (uintptr_t)sh.rb, (uintptr_t)sh.tpb, (uintptr_t)sh.occ,
                       (uintptr_t)sh.seg, (uintptr_t)sh.ls, (uintptr_t)op->bps, (uintptr_t)s16.rb, (uintptr_t)s16.tpb,
                       (uintptr_t)s16.occ, (uintptr_t)s16.seg, (uintptr_t)s16.ls, (uintptr_t)mma_for(op, nrhs)})

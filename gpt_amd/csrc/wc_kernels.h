@@ -69,8 +69,8 @@ struct HopLaunch {
   int skip_t_faces;  // 1: leave out the t-terms through the links that wrap in t (the interior
                      // part of a T-slab rank's hop; the halo kernels add them)
   int mma;           // 1: FP16-storage hops of 4k right-hand sides run on the matrix cores (k_hop_mma)
-  long clov_rstride; // > 0: right-hand side r uses the EPI_CLOV blocks at clov + r * clov_rstride
-                     // (complex elements; a batch of several masses), 0: one set for all
+  const long *clov_roff; // non-null: right-hand side r uses the EPI_CLOV blocks at clov + clov_roff[r]
+                         // (device array, complex elements; a batch of several masses), null: one set
   int big;           // set by the launch: in or out exceeds one buffer descriptor's 4 GB, so
                      // gathers go through a descriptor based at the neighbours' t-slice and
                      // stores through one based at the wave's 64-site block
@@ -102,9 +102,9 @@ bool hop_mma(const HopLaunch &a, hipStream_t s);
 // sides; zeroes the open boundary; optional axpy (out = y - A in).  half:
 // FP32 only, in and out stored as 16-bit (kS16Half / kS16Bf16; y must be null).
 template <typename R>
-// rstride > 0: right-hand side r uses the blocks at blocks + r * rstride
+// roff non-null: right-hand side r uses the blocks at blocks + roff[r] (device array, complex elements)
 void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
-                 const void *y, int open_bc, hipStream_t s, int half = 0, long rstride = 0);
+                 const void *y, int open_bc, hipStream_t s, int half = 0, const long *roff = nullptr);
 
 // Per-right-hand-side reductions over fields whose innermost index is the
 // RHS (nrhs of them): partials[r * kRedBlocks + b] for b < kRedBlocks, then
@@ -183,7 +183,7 @@ void cg_update_r(long n, int nrhs, const double *st, void *r, const void *mmp, d
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
                   const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half = 0,
-                  const int *halt = nullptr, long rstride = 0);
+                  const int *halt = nullptr, const long *roff = nullptr);
 
 // Chunked CG iterations: a run of up to K iterations is enqueued (or
 // replayed from a captured graph) without the host looking in between.
@@ -231,7 +231,7 @@ void convert_f2h(long n, const void *src, void *dst, int half, hipStream_t s);
 // dst = (float) f[r] src.
 void resid_d2f(long n, int nrhs, const void *bs, const void *mp, void *rl, double *partials, hipStream_t s);
 void fold_site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const double *f, void *xl,
-                      void *xh, void *out, int open_bc, hipStream_t s, long rstride = 0);
+                      void *xh, void *out, int open_bc, hipStream_t s, const long *roff = nullptr);
 void scale_copy_rhs(long n, int nrhs, const double *f, const void *src, void *dst, hipStream_t s);
 
 }  // namespace wck

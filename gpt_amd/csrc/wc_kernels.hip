@@ -16,7 +16,7 @@ int site_blocks(long Vh) { return (int)((Vh + kSiteThreads - 1) / kSiteThreads);
 template <typename R, typename S>
 __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int p, const typename c2<R>::t *A,
                                                              double diag, const S *in, S *out, const S *y,
-                                                             int open_bc, long rstride) {
+                                                             int open_bc, const long *roff) {
   using C = typename c2<R>::t;
   const long t = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
   const long h = (N == 1) ? t : t / N;
@@ -34,7 +34,7 @@ __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = cmk<C>(0, 0);
   } else if (A) {
-    clov_apply<R>(A + r * rstride, h, v, o);
+    clov_apply<R>(roff ? A + roff[r] : A, h, v, o);
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = cscale(v[k], (R)diag);
@@ -50,28 +50,28 @@ __global__ __launch_bounds__(kSiteThreads) void k_site_clover(Geom g, int N, int
 
 template <typename R>
 void site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const void *in, void *out,
-                 const void *y, int open_bc, hipStream_t s, int half, long rstride) {
+                 const void *y, int open_bc, hipStream_t s, int half, const long *roff) {
   using C = typename c2<R>::t;
   const int nb = site_blocks(g.Vh * nrhs);
   if constexpr (sizeof(R) == 4) {
     if (half == kS16Half) {
       k_site_clover<R, c2h><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const c2h *)in,
-                                                        (c2h *)out, (const c2h *)y, open_bc, rstride);
+                                                        (c2h *)out, (const c2h *)y, open_bc, roff);
       return;
     }
     if (half == kS16Bf16) {
       k_site_clover<R, c2b><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const c2b *)in,
-                                                        (c2b *)out, (const c2b *)y, open_bc, rstride);
+                                                        (c2b *)out, (const c2b *)y, open_bc, roff);
       return;
     }
   }
   k_site_clover<R, C><<<nb, kSiteThreads, 0, s>>>(g, nrhs, p, (const C *)blocks, diag, (const C *)in, (C *)out,
-                                                  (const C *)y, open_bc, rstride);
+                                                  (const C *)y, open_bc, roff);
 }
 template void site_clover<double>(const Geom &, int, int, const void *, double, const void *, void *,
-                                  const void *, int, hipStream_t, int, long);
+                                  const void *, int, hipStream_t, int, const long *);
 template void site_clover<float>(const Geom &, int, int, const void *, double, const void *, void *,
-                                 const void *, int, hipStream_t, int, long);
+                                 const void *, int, hipStream_t, int, const long *);
 
 // ------------------------------------------------------------------ BLAS-1
 static int elem_blocks(long n) { return (int)((n + 255) / 256); }
@@ -810,7 +810,7 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
                                                               const typename c2<R>::t *r, SP *p,
                                                               typename c2<R>::t *x, const typename c2<R>::t *A,
                                                               double diag, SP *t, int open_bc,
-                                                              const int *halt, int stage, long rstride) {
+                                                              const int *halt, int stage, const long *roff) {
   using C = typename c2<R>::t;
   if (halt && *halt) return;
   const long t0 = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads;
@@ -860,7 +860,7 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
     for (int k = 0; k < 12; ++k) v[k] = cmk<C>(0, 0);
   } else if (A) {
     if (stage) clov_apply_at<R>(cl + (h - s0), (long)ns, pv, v);
-    else clov_apply<R>(A + rr * rstride, h, pv, v);
+    else clov_apply<R>(roff ? A + roff[rr] : A, h, pv, v);
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) v[k] = cscale(pv[k], (R)diag);
@@ -870,36 +870,36 @@ __global__ __launch_bounds__(kSiteThreads) void k_cg_update_xp(Geom g, int N, in
 template <typename R>
 void cg_update_xp(const Geom &g, int nrhs, int par, const double *st, const void *r, void *p, void *x,
                   const void *blocks, double diag, void *t, int open_bc, hipStream_t s, int half,
-                  const int *halt, long rstride) {
+                  const int *halt, const long *roff) {
   using C = typename c2<R>::t;
   const int nb = site_blocks(g.Vh * nrhs);
   // with several right-hand sides sharing one set of clover blocks, the
   // blocks are staged (the items of a site would each load them through the
   // texture path)
-  const int stage = (blocks && nrhs >= 4 && rstride == 0) ? 1 : 0;
+  const int stage = (blocks && nrhs >= 4 && !roff) ? 1 : 0;
   const size_t lds = stage ? sizeof(C) * 36 * (size_t)stage_sites(kSiteThreads, nrhs) : 0;
   if constexpr (sizeof(R) == 4) {
     if (half == kS16Half) {
       k_cg_update_xp<R, c2h><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (c2h *)p, (C *)x,
                                                            (const C *)blocks, diag, (c2h *)t, open_bc, halt,
-                                                           stage, rstride);
+                                                           stage, roff);
       return;
     }
     if (half == kS16Bf16) {
       k_cg_update_xp<R, c2b><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (c2b *)p, (C *)x,
                                                            (const C *)blocks, diag, (c2b *)t, open_bc, halt,
-                                                           stage, rstride);
+                                                           stage, roff);
       return;
     }
   }
   k_cg_update_xp<R, C><<<nb, kSiteThreads, lds, s>>>(g, nrhs, par, st, (const C *)r, (C *)p, (C *)x,
                                                      (const C *)blocks, diag, (C *)t, open_bc, halt, stage,
-                                                     rstride);
+                                                     roff);
 }
 template void cg_update_xp<double>(const Geom &, int, int, const double *, const void *, void *, void *,
-                                   const void *, double, void *, int, hipStream_t, int, const int *, long);
+                                   const void *, double, void *, int, hipStream_t, int, const int *, const long *);
 template void cg_update_xp<float>(const Geom &, int, int, const double *, const void *, void *, void *,
-                                  const void *, double, void *, int, hipStream_t, int, const int *, long);
+                                  const void *, double, void *, int, hipStream_t, int, const int *, const long *);
 
 // one wave; lane r handles right-hand side r (nrhs <= 64)
 __global__ __launch_bounds__(64) void k_iter_check(int N, int mode, double delta2, double *st, double *dat,
@@ -1010,7 +1010,7 @@ void resid_d2f(long n, int nrhs, const void *bs, const void *mp, void *rl, doubl
 __global__ __launch_bounds__(kSiteThreads) void k_fold_site_clover(Geom g, int N, int p, const double2 *A,
                                                                   double diag, const double *f, float2 *xl,
                                                                   double2 *xh, double2 *out, int open_bc,
-                                                                  long rstride) {
+                                                                  const long *roff) {
   const long t = xcd_tile(blockIdx.x, gridDim.x) * kSiteThreads + threadIdx.x;
   const long h = (N == 1) ? t : t / N;
   const int r = (int)(t - h * N);
@@ -1037,7 +1037,7 @@ __global__ __launch_bounds__(kSiteThreads) void k_fold_site_clover(Geom g, int N
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = make_double2(0.0, 0.0);
   } else if (A) {
-    clov_apply<double>(A + r * rstride, h, v, o);
+    clov_apply<double>(roff ? A + roff[r] : A, h, v, o);
   } else {
 #pragma unroll
     for (int k = 0; k < 12; ++k) o[k] = cscale(v[k], diag);
@@ -1045,10 +1045,10 @@ __global__ __launch_bounds__(kSiteThreads) void k_fold_site_clover(Geom g, int N
   store12(out, h, r, N, o);
 }
 void fold_site_clover(const Geom &g, int nrhs, int p, const void *blocks, double diag, const double *f, void *xl,
-                      void *xh, void *out, int open_bc, hipStream_t s, long rstride) {
+                      void *xh, void *out, int open_bc, hipStream_t s, const long *roff) {
   k_fold_site_clover<<<site_blocks(g.Vh * nrhs), kSiteThreads, 0, s>>>(
       g, nrhs, p, (const double2 *)blocks, diag, f, (float2 *)xl, (double2 *)xh, (double2 *)out, open_bc,
-      rstride);
+      roff);
 }
 
 // dst = (float) f[r] * src (FP32 fields)

@@ -336,12 +336,13 @@ int wc_batch_destroy(wc_batch *b);
 int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter);
 /* the same with the slot's own mass (no cgpt counterpart; gpt re-creates or
  * re-masses an operator per mass, interface.py:41-48).  The first mass
- * other than the operators' gives every slot its own clover-inverse blocks
- * (both parities, FP64 and the inner precision: 36 Vh (16 + inner bytes) x
- * 2 per slot), built from the operators' gauge field for that mass; from then
- * on every clover-inverse application of the batch indexes them by slot, so
- * requests of any mass on one gauge field share one batch.  A slot admitted
- * with wc_batch_admit then gets the operators' mass. */
+ * other than the operators' gives the batch one set of clover-inverse
+ * blocks per slot (both parities, FP64 and the inner precision: 36 Vh
+ * (16 + inner bytes) x 2 per set), built from the operators' gauge field
+ * for a mass; slots of one mass share a set.  From then on every
+ * clover-inverse application of the batch uses each slot's set, so requests
+ * of any mass on one gauge field share one batch.  A slot admitted with
+ * wc_batch_admit then gets the operators' mass. */
 int wc_batch_admit_mass(wc_batch *b, int slot, double mass, const void *src, const void *guess, double tol,
                         long maxiter);
 int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);

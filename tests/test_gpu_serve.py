@@ -12,7 +12,8 @@ Checks, with the oracle's FP64 operator as the referee:
 * requests of different masses on one field share one batch (one group per
   field): each slot carries its own clover inverses, every solve meets its
   tolerance through the oracle's M at its own mass and agrees with its solve
-  alone -- with clover, and for plain Wilson fermions;
+  alone -- with clover, and for plain Wilson fermions; slots of one mass
+  share one set;
 * fields of different volumes are served side by side;
 * a request served alone and among requests of other fields gives bitwise
   the same solutions.
@@ -239,6 +240,50 @@ def test_masses_share_one_batch(setup, csw):
         bk.extract(0, xk)
         bk.close()
         assert rel(results[k], xk.to_numpy()) < AGREE, k
+
+
+def test_repeated_masses_share_clover_sets(setup):
+    """slots of one mass share a clover-inverse set: three slots at a mass
+    other than the operators' (all occupied slots on one set, which is then
+    applied as a single set), then slots of the operators' mass, a repeated
+    mass and a new one (sets per slot), and slots admitted into freed sets
+    as others finish.  Every solve meets its tolerance through the oracle at
+    its own mass and agrees with the same solve alone."""
+    gr, U, srcs = setup
+    w = g.qcd.fermion.wilson_clover(U[0], dict(P, mass=0.1))
+    b = serve.batch(w, w.converted(g.single), 8)
+    S = [field(srcs[k], w.F_grid) for k in range(8)]
+    first = [0.05, 0.05, 0.05]
+    later = [0.1, 0.2, 0.05]
+    last = [0.2, 0.15]
+    mass_of, owner, results = {}, {}, {}
+
+    def admit(k, m):
+        owner[b.admit(S[k], TOL, 10000, mass=m)] = k
+        mass_of[k] = m
+
+    for k, m in enumerate(first):
+        admit(k, m)
+    assert b.step(5) == []
+    for k, m in enumerate(later):
+        admit(3 + k, m)
+    pending = list(enumerate(last, start=6))
+    for _ in range(600):
+        for sl in b.step(10):
+            x = g.lattice(w.F_grid)
+            it, tres, conv = b.extract(sl, x)
+            assert conv and 0 < tres <= TOL
+            results[owner.pop(sl)] = x.to_numpy()
+            if pending:
+                admit(*pending.pop(0))
+        if len(results) == 8:
+            break
+    b.close()
+    assert sorted(results) == list(range(8))
+    for k in range(8):
+        ref = W.wilson_clover(U[0], mass=mass_of[k], **P)
+        assert true_res(ref, results[k], srcs[k]) <= 1.05 * TOL, k
+        assert rel(results[k], solve_alone(U[0], mass_of[k], srcs[k])) < AGREE, k
 
 
 def test_server_one_group_per_field_and_mixed_volumes(setup):
