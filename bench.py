@@ -932,8 +932,8 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
     Gaussian right-hand sides, tolerance tol on the true residual.  First nreq
     requests at once (capacity, right-hand sides per second), then npoisson
     new ones with Poisson arrivals at load x that capacity: p50/p99 latency
-    and solves/s.  One group per field serves every mass (per-slot clover
-    inverses); the time to set a group up (operators and batch) is reported
+    and solves/s.  One group per field serves every mass (clover inverses
+    per slot, one set per mass); the time to set a group up (operators and batch) is reported
     apart from the latencies, which include it.  Sources come from a pool of
     12 Gaussian fields per volume (a request takes distinct ones), and
     results are dropped once checked, so that hundreds of requests fit in

@@ -10,9 +10,9 @@ continuous batching:
   ``batch`` of solve slots (``wc_batch_*``: the mixed-precision eo2_ne CG
   advancing all occupied slots with the multi-RHS kernels), sized to the
   right-hand sides waiting when it starts (a multiple of 4, at most
-  ``nslots``); requests of any mass share it: each slot carries its own
-  clover-inverse blocks for its request's mass (``wc_batch_admit_mass``), so
-  same-field requests are batched whatever their masses;
+  ``nslots``); requests of any mass share it: each slot uses clover-inverse
+  blocks for its request's mass (``wc_batch_admit_mass``; slots of one mass
+  share one set), so same-field requests are batched whatever their masses;
 * right-hand sides enter free slots of their group whenever slots free up,
   while the group keeps iterating, and leave when their FP64 true residual
   |src - M dst| / |src| meets the request's tolerance;
@@ -142,8 +142,8 @@ class request:
 def group_bytes(fdims, nslots, masses=True):
     """HBM held by one group: both operators (links -- 12-real in the inner
     one --, clover and inverse, the FP64 gauge copy each keeps, solver
-    scratch), the batch state and, for a batch of several masses, each
-    slot's clover inverses (both parities, FP64 and FP32)."""
+    scratch), the batch state and, for a batch of several masses, room for
+    one set of clover inverses per slot (both parities, FP64 and FP32)."""
     V = int(np.prod(fdims))
     Vh = V // 2
     h64, h32 = Vh * 12 * 16, Vh * 12 * 8
