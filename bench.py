@@ -505,6 +505,7 @@ def main():
     if args.serve > 0:
         serve_res = serve_measure(torch, g, dev, args.serve, npoisson=args.serve_poisson, load=args.serve_load,
                                   streams=args.serve_streams, rate=args.serve_rate, min_slots=args.serve_min_slots)
+        serve_res["multi_mass"] = multi_mass_measure(torch, g, dev)
 
     cg_res = None
     if not args.no_cg:
@@ -1012,6 +1013,60 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
            "resident_fields": len(srv.fields), "grouping": "one group per resident field, any mass",
            "hbm_budget_gb": srv.hbm_budget / 2**30, "scheduler_passes_waiting_for_hbm": srv.waited_for_memory}
     srv.close()
+    return out
+
+
+def multi_mass_measure(torch, g, dev, fd=(24, 24, 24, 48), nslots=12, tol=1e-10):
+    """Cost of several masses in one serving batch: the same 12 Gaussian
+    sources in a 12-slot batch (FP64 outer, 12-real FP32 inner operator at
+    mass 0.1, csw 1) with one mass (the operators'), one other mass, three
+    masses of 4 slots each, and twelve masses; seconds from the first
+    admission until all 12 solves are extracted (tolerance tol on the true
+    residual).  Slots of one mass share a clover-inverse set, and a batch
+    whose occupied slots all use one set stages it per site, so the cost
+    grows with the number of distinct masses, not with the slots."""
+    from gpt_amd import serve
+
+    fd = list(fd)
+    U = smooth_links_device(torch, fd, 0.3, 5101, dev)
+    prm = dict(mass=0.1, csw_r=1.0, csw_t=1.0, xi_0=1.0, nu=1.0, boundary_phases=[1, 1, 1, -1])
+    op = g.qcd.fermion.wilson_clover(U, prm)
+    inner = g.qcd.fermion.wilson_clover(U, dict(prm, compress_links=True), precision=g.single)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(7)
+    grid = g.grid(fd, g.double)
+    srcs = []
+    for _ in range(nslots):
+        x = g.lattice(grid)
+        x.data.copy_(torch.randn(x.data.shape, generator=gen, dtype=torch.float64, device=dev) / np.sqrt(2))
+        srcs.append(x)
+    third = nslots // 3
+    cases = [("warm-up", [None] * nslots), ("one mass", [None] * nslots), ("one other mass", [0.08] * nslots),
+             ("three masses", [0.08] * third + [None] * third + [0.12] * (nslots - 2 * third)),
+             ("twelve masses", [0.1 + 0.005 * (j - 6) if j != 6 else None for j in range(nslots)])]
+    out = {"lattice": "x".join(map(str, fd)), "slots": nslots, "tol": tol}
+    dst = g.lattice(op.F_grid)
+    for label, masses in cases:
+        b = serve.batch(op, inner, nslots)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for x, m in zip(srcs, masses):
+            b.admit(x, tol, 20000, mass=m)
+        done, worst, conv = 0, 0.0, True
+        while done < nslots:
+            for sl in b.step(25):
+                it, tres, c = b.extract(sl, dst)
+                worst, conv, done = max(worst, tres), conv and bool(c), done + 1
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        b.close()
+        if label != "warm-up":
+            out[label.replace(" ", "_") + "_s"] = t
+            out["max_true_residual"] = max(out.get("max_true_residual", 0.0), worst)
+            out["all_converged"] = out.get("all_converged", True) and conv
+        log(f"bench: multi-mass batch, {label}: {t:.3f} s for {nslots} solves")
+    op.release()
+    inner.release()
     return out
 
 
