@@ -90,6 +90,7 @@ SIGNATURES = {
     "wc_batch_step": (_I, [_P, _I, _P, _P]),
     "wc_batch_extract": (_I, [_P, _I, _P, _P, _P, _P]),
     "wc_batch_slots": (_I, [_P, _P, _P]),
+    "wc_batch_set_mass_sets": (_I, [_P, _I]),
     "wc_set_hop_mma": (_I, [_P, _I]),
     "wc_get_hop_mma": (_I, [_P, _I, _P]),
     "wc_apply_block_s16": (_I, [_P, _I, _I, _I, _P, _P, _I, _P]),

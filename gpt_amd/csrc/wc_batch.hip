@@ -92,8 +92,8 @@ struct wc_batch {
   bool dirty = true;
   std::vector<double> cref, clast, rate;
   std::vector<long> since;
-  // several masses (wc_batch_admit_mass): N sets of clover-inverse blocks of
-  // both parities, FP64 for the outer operator (ai_hi) and in the inner
+  // several masses (wc_batch_admit_mass): nsets (default N) sets of
+  // clover-inverse blocks of both parities, FP64 for the outer operator (ai_hi) and in the inner
   // precision (ai_lo), set k of parity p at ai_*[p] + k * 36 Vh complex
   // values.  Slot r uses set set_of[r]; the slots of one mass share a set
   // (set_mass: the mass a set holds, NaN before it is built).  While the
@@ -101,6 +101,7 @@ struct wc_batch {
   // set_of[r] * 36 Vh on the device, or one set for all when every occupied
   // slot has the same one.
   bool mm = false;
+  int nsets = 0;  // sets a multi-mass batch holds (0: one per slot)
   void *ai_hi[2] = {nullptr, nullptr}, *ai_lo[2] = {nullptr, nullptr};
   long *roff = nullptr;
   int *err = nullptr;
@@ -215,19 +216,22 @@ static int build_set(wc_batch *b, int k, double mass) {
 }
 
 // slot's clover inverses for `mass`: the set of another slot of that mass,
-// else a set that no other occupied slot uses, built for it
+// else a set that no other occupied slot uses, built for it (an error when
+// the other occupied slots use every set: as many other masses as sets)
 static int set_slot_mass(wc_batch *b, int slot, double mass) {
-  const int N = b->N;
+  const int N = b->N, K = (int)b->set_mass.size();
   int k = -1, rc;
-  for (int j = 0; j < N && k < 0; ++j)
+  for (int j = 0; j < K && k < 0; ++j)
     if (b->set_mass[j] == mass) k = j;
   if (k < 0) {
-    std::vector<char> used(N, 0);
+    std::vector<char> used(K, 0);
     for (int i = 0; i < N; ++i)
       if (b->occupied[i] && i != slot) used[b->set_of[i]] = 1;
-    // N sets and at most N - 1 other occupied slots: one is free
-    for (int j = 0; j < N && k < 0; ++j)
+    for (int j = 0; j < K && k < 0; ++j)
       if (!used[j]) k = j;
+    if (k < 0)
+      WC_FAIL(WC_ERR_ARG, "wc_batch_admit_mass: the batch's %d clover-inverse sets are in use by %d other masses", K,
+              K);
     if ((rc = build_set(b, k, mass))) return rc;
   }
   b->mass[slot] = mass;
@@ -243,16 +247,16 @@ static int set_slot_mass(wc_batch *b, int slot, double mass) {
 static int enable_masses(wc_batch *b) {
   if (b->mm) return WC_OK;
   wc_operator *hi = b->hi, *lo = b->lo;
-  const int N = b->N;
+  const int N = b->N, K = b->nsets > 0 ? b->nsets : N;
   const size_t sh = (size_t)36 * hi->g.Vh * hi->csz, sl = (size_t)36 * lo->g.Vh * lo->csz;
   for (int p = 0; p < 2; ++p) {
-    WC_HIP(wc_dev_malloc(&b->ai_hi[p], sh * N));
-    WC_HIP(wc_dev_malloc(&b->ai_lo[p], sl * N));
+    WC_HIP(wc_dev_malloc(&b->ai_hi[p], sh * K));
+    WC_HIP(wc_dev_malloc(&b->ai_lo[p], sl * K));
   }
   WC_HIP(wc_dev_malloc(&b->roff, sizeof(long) * N));
   WC_HIP(wc_dev_malloc((void **)&b->err, sizeof(int)));
   b->mass.assign(N, hi->mass);
-  b->set_mass.assign(N, NAN);
+  b->set_mass.assign(K, NAN);
   b->set_of.assign(N, 0);
   int rc;
   if ((rc = build_set(b, 0, hi->mass))) return rc;
@@ -592,6 +596,14 @@ static int admit_impl(wc_batch *b, int slot, const void *src, const void *guess,
 }
 
 extern "C" {
+
+int wc_batch_set_mass_sets(wc_batch *b, int nsets) {
+  if (!b) WC_FAIL(WC_ERR_ARG, "wc_batch_set_mass_sets: null batch");
+  if (nsets < 1 || nsets > b->N) WC_FAIL(WC_ERR_ARG, "wc_batch_set_mass_sets: nsets %d not in [1, %d]", nsets, b->N);
+  if (b->mm) WC_FAIL(WC_ERR_ARG, "wc_batch_set_mass_sets: the batch already holds several masses");
+  b->nsets = nsets;
+  return WC_OK;
+}
 
 int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, double tol, long maxiter) {
   return admit_impl(b, slot, src, guess, tol, maxiter, nullptr);

@@ -45,9 +45,11 @@ class batch:
     operator pair (op FP64, inner in the inner precision).  inner_storage
     "half" (the default for an FP32 inner operator; IEEE FP16) or "bf16"
     keeps the inner search direction and Schur intermediates in 16-bit
-    storage."""
+    storage.  mass_sets: clover-inverse sets the batch holds once it serves
+    several masses (None: one per slot); with fewer, it holds at most that
+    many masses at a time (``admit`` of one more raises)."""
 
-    def __init__(self, op, inner, nslots, parity=core.odd, delta=0.01, inner_storage="auto"):
+    def __init__(self, op, inner, nslots, parity=core.odd, delta=0.01, inner_storage="auto", mass_sets=None):
         self.op, self.inner = op, inner
         self.nslots = int(nslots)
         if inner_storage == "auto":
@@ -59,6 +61,11 @@ class batch:
         _lib.check(_lib.load().wc_batch_create(op.handle, inner.handle, parity.tag, self.nslots, float(delta), code,
                                                core.stream_ptr(), ctypes.byref(h)))
         self.handle = h.value
+        if mass_sets is not None:
+            rc = _lib.load().wc_batch_set_mass_sets(self.handle, max(1, min(int(mass_sets), self.nslots)))
+            if rc:
+                self.close()
+                _lib.check(rc)
         self.free = list(range(self.nslots))
         self.slot_of = {}
 
@@ -139,18 +146,20 @@ class request:
         return None if self.t_done is None else self.t_done - self.t_arrival
 
 
-def group_bytes(fdims, nslots, masses=True):
+def group_bytes(fdims, nslots, masses=True, mass_sets=None):
     """HBM held by one group: both operators (links -- 12-real in the inner
     one --, clover and inverse, the FP64 gauge copy each keeps, solver
-    scratch), the batch state and, for a batch of several masses, room for
-    one set of clover inverses per slot (both parities, FP64 and FP32)."""
+    scratch), the batch state and, for a batch of several masses, its sets
+    of clover inverses (both parities, FP64 and FP32; mass_sets of them, or
+    one per slot)."""
     V = int(np.prod(fdims))
     Vh = V // 2
     h64, h32 = Vh * 12 * 16, Vh * 12 * 8
     op64 = 2 * Vh * 8 * 9 * 16 + 4 * Vh * 36 * 16 + V * 4 * 9 * 16 + h64 * (3 * nslots + 9)
     op32 = 2 * Vh * 8 * 6 * 8 + 4 * Vh * 36 * 8 + V * 4 * 9 * 16 + h32 * (3 * nslots + 5)
     state = 5 * h64 * nslots + 5 * h32 * nslots + 2 * h64 * nslots
-    sets = nslots * 2 * Vh * 36 * (16 + 8) if masses else 0
+    nsets = nslots if mass_sets is None else min(nslots, int(mass_sets))
+    sets = nsets * 2 * Vh * 36 * (16 + 8) if masses else 0
     return op64 + op32 + state + sets
 
 
@@ -167,7 +176,7 @@ class _group:
         self.batch = None
         self.op = self.inner = None
         self.nslots = nslots
-        self.bytes = group_bytes(U[0].grid.fdimensions, nslots)
+        self.bytes = group_bytes(U[0].grid.fdimensions, nslots, mass_sets=server.mass_sets)
 
     def busy(self):
         return bool(self.queue) or bool(self.in_slot)
@@ -190,11 +199,19 @@ class _group:
                 if self.server.iteration_chunk is not None:
                     self.op.set_iteration_chunk(self.server.iteration_chunk)
                 self.batch = batch(self.op, self.inner, self.nslots, delta=self.server.delta,
-                                   inner_storage=self.server.inner_storage)
+                                   inner_storage=self.server.inner_storage, mass_sets=self.server.mass_sets)
                 torch.cuda.current_stream().synchronize()
                 self.setup_s = time.perf_counter() - t0
+            cap = self.server.mass_sets
             while self.queue and self.batch.free:
-                req, j = self.queue.popleft()
+                req, j = self.queue[0]
+                # a batch holds at most mass_sets masses at a time: a new
+                # mass waits (in arrival order) until one leaves
+                if cap is not None:
+                    used = {r.mass for r, _ in self.in_slot.values()}
+                    if req.mass not in used and len(used) >= min(cap, self.nslots):
+                        break
+                self.queue.popleft()
                 slot = self.batch.admit(req.sources[j], req.tol, req.maxiter,
                                         mass=None if req.mass == self.mass else req.mass)
                 self.in_slot[slot] = (req, j)
@@ -222,14 +239,17 @@ class server:
     phases, ...); nslots: slots per group; streams: HIP streams the groups
     are spread over; iters_per_step: inner iterations a group runs before the
     scheduler looks again; hbm_budget: bytes the groups may hold (default 80%
-    of the device memory free at start, less the library's block cache)."""
+    of the device memory free at start, less the library's block cache);
+    mass_sets: the clover-inverse sets of a group's batch (at most that many
+    masses in a batch at a time; None: one set per slot)."""
 
     def __init__(self, params, nslots=12, streams=4, iters_per_step=25, hbm_budget=None,
                  inner_precision=core.single, delta=0.01, inner_storage="auto", iteration_chunk=None,
-                 min_slots=1):
+                 min_slots=1, mass_sets=4):
         self.params = {k: v for k, v in dict(params).items() if k not in ("mass", "kappa")}
         self.nslots = int(nslots)
         self.min_slots = max(1, min(int(min_slots), self.nslots))  # smallest batch a group opens with
+        self.mass_sets = None if mass_sets is None else max(2, int(mass_sets))
         self.iters_per_step = int(iters_per_step)
         self.inner_precision = inner_precision
         self.delta = float(delta)
@@ -304,7 +324,7 @@ class server:
         if g is None:
             U = self.fields[req.field]
             nslots = max(self.min_slots, min(self.nslots, 4 * ((len(req.sources) + 3) // 4)))
-            need = group_bytes(U[0].grid.fdimensions, nslots)
+            need = group_bytes(U[0].grid.fdimensions, nslots, mass_sets=self.mass_sets)
             if self.hbm_used + need > self.hbm_budget:
                 return False
             g = _group(self, key, U, req.mass, self.streams[self._next_stream % len(self.streams)], nslots)

@@ -345,6 +345,13 @@ int wc_batch_admit(wc_batch *b, int slot, const void *src, const void *guess, do
  * wc_batch_admit then gets the operators' mass. */
 int wc_batch_admit_mass(wc_batch *b, int slot, double mass, const void *src, const void *guess, double tol,
                         long maxiter);
+/* the number of clover-inverse sets a batch of several masses holds (no
+ * cgpt counterpart), 1 <= nsets <= nslots, before its first admission at
+ * another mass (default: nslots).  With fewer sets than slots the batch
+ * holds at most nsets masses at a time: an admission whose mass would need
+ * one more set returns WC_ERR_ARG and leaves the batch as it was, so a
+ * caller keeps such a request waiting until a mass leaves. */
+int wc_batch_set_mass_sets(wc_batch *b, int nsets);
 int wc_batch_step(wc_batch *b, int iters, int *finished, int *nfinished);
 int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double *true_residual, int *converged);
 int wc_batch_slots(const wc_batch *b, int *occupied, int *active);

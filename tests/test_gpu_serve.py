@@ -13,7 +13,8 @@ Checks, with the oracle's FP64 operator as the referee:
   field): each slot carries its own clover inverses, every solve meets its
   tolerance through the oracle's M at its own mass and agrees with its solve
   alone -- with clover, and for plain Wilson fermions; slots of one mass
-  share one set;
+  share one set, and a batch of fewer sets than slots holds that many
+  masses at a time;
 * fields of different volumes are served side by side;
 * a request served alone and among requests of other fields gives bitwise
   the same solutions.
@@ -284,6 +285,56 @@ def test_repeated_masses_share_clover_sets(setup):
         ref = W.wilson_clover(U[0], mass=mass_of[k], **P)
         assert true_res(ref, results[k], srcs[k]) <= 1.05 * TOL, k
         assert rel(results[k], solve_alone(U[0], mass_of[k], srcs[k])) < AGREE, k
+
+
+def test_mass_sets_cap(setup):
+    """a batch of two clover-inverse sets holds two masses at a time: a third
+    mass is refused while both are in use and admitted once one has left;
+    a server whose groups hold two sets serves requests of four masses on
+    one field, each checked by the oracle at its own mass"""
+    gr, U, srcs = setup
+    w = g.qcd.fermion.wilson_clover(U[0], dict(P, mass=0.1))
+    b = serve.batch(w, w.converted(g.single), 4, mass_sets=2)
+    S = [field(srcs[k], w.F_grid) for k in range(4)]
+    sa = b.admit(S[0], TOL, 10000)
+    sb = b.admit(S[1], TOL, 10000, mass=0.05)
+    with pytest.raises(RuntimeError):
+        b.admit(S[2], TOL, 10000, mass=0.2)
+    assert len(b.free) == 2
+    owner = {sa: (0, 0.1), sb: (1, 0.05)}
+    results = {}
+    waiting = [(2, 0.2), (3, 0.05)]
+    for _ in range(600):
+        for sl in b.step(10):
+            x = g.lattice(w.F_grid)
+            it, tres, conv = b.extract(sl, x)
+            assert conv and 0 < tres <= TOL
+            k, m = owner.pop(sl)
+            results[k] = (m, x.to_numpy())
+            while waiting:
+                k2, m2 = waiting[0]
+                if m2 not in {mm for _, mm in owner.values()} and len({mm for _, mm in owner.values()}) >= 2:
+                    break
+                waiting.pop(0)
+                owner[b.admit(S[k2], TOL, 10000, mass=m2)] = (k2, m2)
+        if len(results) == 4:
+            break
+    b.close()
+    assert sorted(results) == [0, 1, 2, 3]
+    for k, (m, x) in results.items():
+        ref = W.wilson_clover(U[0], mass=m, **P)
+        assert true_res(ref, x, srcs[k]) <= 1.05 * TOL, k
+
+    srv = serve.server(P, nslots=4, streams=2, mass_sets=2)
+    srv.add_field("a", U[0], warmup=False)
+    reqs = [serve.request("a", m, [field(srcs[k], w.F_grid)], tol=TOL, maxiter=10000)
+            for k, m in enumerate([0.1, 0.05, 0.2, 0.15, 0.05])]
+    done = srv.run([(0.0, r) for r in reqs])
+    srv.close()
+    assert len(done) == 5
+    for k, r in enumerate(reqs):
+        ref = W.wilson_clover(U[0], mass=r.mass, **P)
+        assert r.converged[0] and true_res(ref, r.results[0].to_numpy(), srcs[k]) <= 1.05 * TOL, k
 
 
 def test_server_one_group_per_field_and_mixed_volumes(setup):
