@@ -243,16 +243,22 @@ def test_masses_share_one_batch(setup, csw):
         assert rel(results[k], xk.to_numpy()) < AGREE, k
 
 
-def test_repeated_masses_share_clover_sets(setup):
+@pytest.mark.parametrize("mma", [False, True])
+def test_repeated_masses_share_clover_sets(setup, mma):
     """slots of one mass share a clover-inverse set: three slots at a mass
     other than the operators' (all occupied slots on one set, which is then
     applied as a single set), then slots of the operators' mass, a repeated
     mass and a new one (sets per slot), and slots admitted into freed sets
     as others finish.  Every solve meets its tolerance through the oracle at
-    its own mass and agrees with the same solve alone."""
+    its own mass and agrees with the same solve alone.  mma: the inner
+    16-bit hops on the matrix cores (their per-RHS clover epilogue)."""
     gr, U, srcs = setup
     w = g.qcd.fermion.wilson_clover(U[0], dict(P, mass=0.1))
-    b = serve.batch(w, w.converted(g.single), 8)
+    inner = w.converted(g.single)
+    if mma:
+        inner.set_hop_mma(True)
+        assert inner.hop_mma(8)
+    b = serve.batch(w, inner, 8, inner_storage="half")
     S = [field(srcs[k], w.F_grid) for k in range(8)]
     first = [0.05, 0.05, 0.05]
     later = [0.1, 0.2, 0.05]
