@@ -10,6 +10,14 @@
 // kept that way, beyond which blocks are freed.  An allocation that fails returns
 // every cached block of the device to the runtime and tries once more, and
 // wc_release_cached_memory does the same on request.
+//
+// hipFree waits for all work on the device before the block can be used
+// again.  A cached block is handed out again at once, possibly to work on
+// another stream, so a free waits for the block's device just the same
+// before it caches the block: kernels still queued on the old owner's
+// streams cannot run into the new owner's data.  Frees happen when
+// operators and batches are destroyed or their scratch grows, not inside
+// solves.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -100,9 +108,22 @@ hipError_t wc_dev_malloc_raw(void **p, size_t bytes) {
 hipError_t wc_dev_free(const void *cp) {
   void *p = const_cast<void *>(cp);
   if (!p) return hipSuccess;
+  int dev = -1;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_live.find(p);
+    if (it == g_live.end()) return hipFree(p);  // not ours: hand it straight back
+    dev = it->second.dev;
+  }
+  // the work still using the block ends first, as hipFree would wait for it
+  int cur = dev;
+  (void)hipGetDevice(&cur);
+  if (cur != dev) (void)hipSetDevice(dev);
+  (void)hipDeviceSynchronize();
+  if (cur != dev) (void)hipSetDevice(cur);
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_live.find(p);
-  if (it == g_live.end()) return hipFree(p);  // not ours: hand it straight back
+  if (it == g_live.end()) return hipErrorInvalidValue;  // freed twice meanwhile
   const Live l = it->second;
   g_live.erase(it);
   if (g_cached[l.dev] + l.bytes <= cap_of(l.dev)) {
