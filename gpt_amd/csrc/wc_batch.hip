@@ -75,6 +75,9 @@ struct wc_batch {
   // N-RHS half fields in the inner precision
   void *rl = nullptr, *pl = nullptr, *ml = nullptr, *xl = nullptr, *aD = nullptr;
   std::vector<void *> src;  // per slot: the request's full FP64 source (kept for S src)
+  // single-RHS half fields for admission and extraction: FP64 (one) and in
+  // the inner precision (lone)
+  void *one[6] = {}, *lone[2] = {};
   double *st = nullptr, *host = nullptr, *partials = nullptr;  // CG scalars, reduction partials
   // host-side slot state
   std::vector<int> occupied, active, finished, conv;
@@ -164,6 +167,8 @@ static void free_batch(wc_batch *b) {
   if (!b) return;
   for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh, b->rl, b->pl, b->ml, b->xl, b->aD}) (void)wc_dev_free(q);
   for (void *q : b->src) (void)wc_dev_free(q);
+  for (void *q : b->one) (void)wc_dev_free(q);
+  for (void *q : b->lone) (void)wc_dev_free(q);
   (void)wc_dev_free(b->st);
   (void)wc_dev_free(b->partials);
   for (int p = 0; p < 2; ++p) {
@@ -270,14 +275,15 @@ static int enable_masses(wc_batch *b) {
   return WC_OK;
 }
 
-// single-RHS FP64 temporaries of the outer operator (tmp[3..8]) and of the
-// inner one (tmp[3..4]) used by admission and extraction
-static int ensure_single(wc_batch *b) {
+// the operators' scratch at the batch's size, so that no later call of the
+// batch grows (frees and re-allocates) it: N-RHS temporaries tmp[0..2] and
+// reductions of both operators
+static int ensure_scratch(wc_batch *b) {
   int rc;
-  if ((rc = ensure_tmp(b->hi, 9, 1))) return rc;
-  if ((rc = ensure_tmp(b->lo, 5, 1))) return rc;
-  if ((rc = ensure_red(b->hi, 1))) return rc;
-  return ensure_red(b->lo, 1);
+  if ((rc = ensure_tmp(b->hi, 3, b->N))) return rc;
+  if ((rc = ensure_tmp(b->lo, 3, b->N))) return rc;
+  if ((rc = ensure_red(b->hi, b->N))) return rc;
+  return ensure_red(b->lo, b->N);
 }
 
 // FP64: y += x_inner / s for every slot, x_inner = 0, then r_s = b_s - Mpc y,
@@ -409,13 +415,15 @@ int wc_batch_create(wc_operator *op, wc_operator *inner, int parity, int nslots,
   for (void **q : {&b->rl, &b->pl, &b->ml, &b->xl, &b->aD}) alloc(q, hl);
   b->src.assign(N, nullptr);
   for (int i = 0; i < N; ++i) alloc(&b->src[i], 2 * hbytes(op));
+  for (void *&q : b->one) alloc(&q, hbytes(op));
+  for (void *&q : b->lone) alloc(&q, hbytes(inner));
   alloc((void **)&b->st, sizeof(double) * 16 * N);
   alloc((void **)&b->partials, sizeof(double) * std::max(wck::kRedBlocks, wck::hop_blocks(op->g)) * N);
   if (rc == WC_OK && hipHostMalloc((void **)&b->host, sizeof(double) * 16 * N, hipHostMallocDefault) != hipSuccess) {
     g_err = "wc_batch_create: hipHostMalloc failed";
     rc = WC_ERR_HIP;
   }
-  if (rc == WC_OK) rc = ensure_single(b);
+  if (rc == WC_OK) rc = ensure_scratch(b);
   if (rc == WC_OK) rc = maybe_tune(op, N, b->s);
   if (rc == WC_OK) rc = maybe_tune(inner, N, b->s);
   if (rc != WC_OK) {
@@ -478,16 +486,15 @@ static int admit_impl(wc_batch *b, int slot, const void *src, const void *guess,
   const int N = b->N, pd = b->pd, pc = 1 - pd;
   hipStream_t s = b->s;
   int rc;
-  if ((rc = ensure_single(b))) return rc;
+  if ((rc = ensure_scratch(b))) return rc;
   // a mass other than the operators' makes the batch one of several masses
   if (mass && *mass != hi->mass && (rc = enable_masses(b))) return rc;
   if (b->mm && (rc = set_slot_mass(b, slot, mass ? *mass : hi->mass))) return rc;
   SlotClover use(b, slot);
   const size_t hb = hbytes(hi);
   const long n1 = hi->g.Vh * 12;
-  void *t1 = hi->tmp[3], *bs1 = hi->tmp[4], *y1 = hi->tmp[5], *mp1 = hi->tmp[6], *rs1 = hi->tmp[7],
-       *rh1 = hi->tmp[8];
-  void *pl1 = lo->tmp[3], *aD1 = lo->tmp[4];
+  void *t1 = b->one[0], *bs1 = b->one[1], *y1 = b->one[2], *mp1 = b->one[3], *rs1 = b->one[4], *rh1 = b->one[5];
+  void *pl1 = b->lone[0], *aD1 = b->lone[1];
   auto half = [&](const void *f, int p) { return (const void *)((const char *)f + (size_t)p * hb); };
   WC_HIP(hipMemcpyAsync(b->src[slot], src, 2 * hb, hipMemcpyDefault, s));
   // b_s = R src = src_D - Meooe A_C^-1 src_C
@@ -716,10 +723,10 @@ int wc_batch_extract(wc_batch *b, int slot, void *dst, long *iterations, double 
   const int N = b->N, pd = b->pd, pc = 1 - pd;
   hipStream_t s = b->s;
   int rc;
-  if ((rc = ensure_single(b))) return rc;
+  if ((rc = ensure_scratch(b))) return rc;
   const size_t hb = hbytes(hi);
   const long n1 = hi->g.Vh * 12;
-  void *y1 = hi->tmp[5], *t1 = hi->tmp[3];
+  void *y1 = b->one[2], *t1 = b->one[0];
   auto half = [&](void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
   k_from_slot<double2><<<nblk(n1), 256, 0, s>>>(n1, N, slot, (const double2 *)b->xh, (double2 *)y1);
   // dst = L y + S src: dst_D = A_D^-1 y ; dst_C = A_C^-1 src_C - A_C^-1 Meooe dst_D

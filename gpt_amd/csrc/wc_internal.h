@@ -25,6 +25,24 @@ extern thread_local std::string g_err;  // message of the last failure (wc_api.h
 // exact-size cache of freed blocks (wc_alloc.hip)
 hipError_t wc_dev_malloc_raw(void **p, size_t bytes);
 hipError_t wc_dev_free(const void *p);
+// Frees by this thread between begin and end are held back; end(synced)
+// caches them, at once when synced (the stream their last users ran on has
+// been synchronized), else through wc_dev_free's wait for the device.
+void wc_alloc_defer_begin();
+void wc_alloc_defer_end(bool synced);
+// a graph capture in progress on this thread (device-wide waits and frees
+// of the allocator wait for it to end)
+void wc_capture_begin();
+void wc_capture_end();
+struct CaptureGuard {
+  CaptureGuard() { wc_capture_begin(); }
+  ~CaptureGuard() { wc_capture_end(); }
+};
+struct AllocDefer {
+  bool synced = false;
+  AllocDefer() { wc_alloc_defer_begin(); }
+  ~AllocDefer() { wc_alloc_defer_end(synced); }
+};
 template <typename T> inline hipError_t wc_dev_malloc(T **p, size_t bytes) {
   return wc_dev_malloc_raw((void **)p, bytes);
 }
@@ -553,6 +571,9 @@ template <typename F>
 inline int run_iterations(IterRun &ir, const std::vector<uintptr_t> &key, int K, int nrhs, hipStream_t s,
                           F &&enqueue) {
   int rc;
+  // scratch that grows inside the run is freed once the run has ended on s
+  // (a device-wide wait inside a capture would break it)
+  AllocDefer defer;
   if ((rc = ensure_run(ir))) return rc;
   if (ir.key != key) {
     drop_graphs(ir);
@@ -572,10 +593,14 @@ inline int run_iterations(IterRun &ir, const std::vector<uintptr_t> &key, int K,
     while ((2 << j) < K) ++j;
     if (!ir.exec[j]) {
       if (!ir.cap) WC_HIP(hipStreamCreateWithFlags(&ir.cap, hipStreamNonBlocking));
-      WC_HIP(hipStreamBeginCapture(ir.cap, hipStreamCaptureModeRelaxed));
-      for (int i = 0; i < K && !rc; ++i) rc = enqueue(ir.cap);
       hipGraph_t gr = nullptr;
-      const hipError_t e = hipStreamEndCapture(ir.cap, &gr);
+      hipError_t e;
+      {
+        CaptureGuard cg;
+        WC_HIP(hipStreamBeginCapture(ir.cap, hipStreamCaptureModeRelaxed));
+        for (int i = 0; i < K && !rc; ++i) rc = enqueue(ir.cap);
+        e = hipStreamEndCapture(ir.cap, &gr);
+      }
       if (rc || e != hipSuccess) {
         if (gr) (void)hipGraphDestroy(gr);
         if (rc) return rc;
@@ -594,6 +619,7 @@ inline int run_iterations(IterRun &ir, const std::vector<uintptr_t> &key, int K,
   WC_HIP(hipMemcpyAsync(ir.hctl, ir.ctl, IterRun::kCtlBytes, hipMemcpyDeviceToHost, s));
   WC_HIP(hipMemcpyAsync(ir.hhist, ir.hist, sizeof(double) * (size_t)K * nrhs, hipMemcpyDeviceToHost, s));
   WC_HIP(hipStreamSynchronize(s));
+  defer.synced = true;
   const int done = ir.hctl[wck::kCtlIter];
   if (done < 1 || done > K) WC_FAIL(WC_ERR_HIP, "CG run: %d iterations reported for a run of %d", done, K);
   return WC_OK;
