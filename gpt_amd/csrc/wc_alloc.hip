@@ -25,7 +25,9 @@
 // new ones from starting meanwhile.  A thread's own frees inside a run of
 // iterations (scratch that grows on the run's first iteration, possibly
 // inside its capture) are held until the run has ended on its stream
-// (wc_alloc_defer_*).
+// (wc_alloc_defer_*).  Scratch that grows outside a run was last used on
+// the stream of the call that grows it: wc_dev_free_on waits for that stream
+// alone.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -203,6 +205,40 @@ hipError_t wc_dev_free(const void *cp) {
     return hipSuccess;
   }
   return hipFree(p);
+}
+
+void wc_device_wait() {
+  std::unique_lock<std::shared_mutex> q(g_capmu);
+  (void)hipDeviceSynchronize();
+}
+
+hipError_t wc_dev_free_idle(const void *cp) {
+  void *p = const_cast<void *>(cp);
+  if (!p) return hipSuccess;
+  bool ours;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ours = g_live.count(p) > 0;
+  }
+  if (!ours || t_defer > 0 || t_capturing > 0) return wc_dev_free(p);
+  return release_block(p);
+}
+
+hipError_t wc_dev_free_on(const void *cp, hipStream_t s) {
+  void *p = const_cast<void *>(cp);
+  if (!p) return hipSuccess;
+  bool ours;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    ours = g_live.count(p) > 0;
+  }
+  if (!ours || t_defer > 0) return wc_dev_free(p);
+  if (t_capturing > 0) return hipErrorInvalidValue;
+  // its last users ran on s: a stream-local wait, which no capture on
+  // another stream minds
+  const hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+  return release_block(p);
 }
 
 extern "C" int wc_release_cached_memory(long *released_bytes) {

@@ -183,7 +183,7 @@ int wc_create(const wc_params *p, const void *U, void *stream, wc_operator **out
   }
   chk(wc_dev_malloc(&op->Unat, (size_t)op->g.V * 4 * 9 * 16), "wc_dev_malloc(gauge copy)");
   chk(wc_dev_malloc((void **)&op->err, sizeof(int)), "wc_dev_malloc(flag)");
-  if (rc == WC_OK && ensure_red(op, 1) != WC_OK) rc = WC_ERR_HIP;
+  if (rc == WC_OK && ensure_red(op, 1, (hipStream_t)stream) != WC_OK) rc = WC_ERR_HIP;
   if (rc != WC_OK) {
     free_op(op);
     return rc;
@@ -291,9 +291,9 @@ static int apply_impl(wc_operator *op, int nrhs, int opcode, void *dst, const vo
 template <typename R>
 static int cg_impl(wc_operator *op, int nrhs, int pd, void *x, const void *b, double eps, long maxiter,
                    double *history, long *iters, int *conv, double *r2, double *b2, hipStream_t s) {
-  int rc = ensure_tmp(op, 10, nrhs);
+  int rc = ensure_tmp(op, 10, nrhs, s);
   if (rc) return rc;
-  if ((rc = ensure_red(op, nrhs))) return rc;
+  if ((rc = ensure_red(op, nrhs, s))) return rc;
   const int N = nrhs;
   void *r = op->tmp[3], *p = op->tmp[4], *mmp = op->tmp[5], *aDp = op->tmp[9];
   const long n = op->g.Vh * 12 * N;
@@ -424,10 +424,10 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
                       hipStream_t s) {
   const int pc = 1 - pd;
   int rc;
-  if ((rc = ensure_tmp(hi, 9, N))) return rc;
-  if ((rc = ensure_tmp(lo, 8, N))) return rc;
-  if ((rc = ensure_red(hi, N))) return rc;
-  if ((rc = ensure_red(lo, N))) return rc;
+  if ((rc = ensure_tmp(hi, 9, N, s))) return rc;
+  if ((rc = ensure_tmp(lo, 8, N, s))) return rc;
+  if ((rc = ensure_red(hi, N, s))) return rc;
+  if ((rc = ensure_red(lo, N, s))) return rc;
   const size_t hb = half_bytes(hi) * (size_t)N;
   auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
   void *bs = hi->tmp[3], *xh = hi->tmp[4], *rs = hi->tmp[5], *rh = hi->tmp[6], *tC = hi->tmp[7], *mp = hi->tmp[8];
@@ -664,7 +664,7 @@ static int mixed_impl(wc_operator *hi, wc_operator *lo, int N, int pd, void *dst
 // shape), the output is overwritten.
 template <typename R>
 static int tune_impl(wc_operator *op, int N, int reps, hipStream_t s, wck::HopShape &best, float &best_ms) {
-  int rc = ensure_tmp(op, 2, N);
+  int rc = ensure_tmp(op, 2, N, s);
   if (rc) return rc;
   const long n = op->g.Vh * 12 * N;
   wck::fill_pattern<R>(n, op->tmp[0], s);
@@ -852,7 +852,7 @@ static int pick_s16(wc_operator *op, int N, hipStream_t s) {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     if (g_shape16_cache.count(shape_key(op, N))) return WC_OK;
   }
-  int rc = ensure_tmp(op, 2, N);
+  int rc = ensure_tmp(op, 2, N, s);
   if (rc) return rc;
   const long n = op->g.Vh * 12 * N;
   wck::fill_pattern<float>(n, op->tmp[1], s);
@@ -1010,15 +1010,15 @@ int wc_get_hop_shape(const wc_operator *op, int nrhs, int *shape) {
 
 int wc_trim(wc_operator *op) {
   if (!op) WC_FAIL(WC_ERR_ARG, "wc_trim: null operator");
-  WC_HIP(hipDeviceSynchronize());
+  wc_device_wait();
   drop_graphs(op->run);
-  for (void *q : op->tmp) (void)wc_dev_free(q);
+  for (void *q : op->tmp) (void)wc_dev_free_idle(q);
   op->tmp.clear();
   op->tmp_bytes = 0;
-  (void)wc_dev_free(op->partials);
+  (void)wc_dev_free_idle(op->partials);
   op->partials = nullptr;
   op->npartials = 0;
-  (void)wc_dev_free(op->dots);
+  (void)wc_dev_free_idle(op->dots);
   op->dots = nullptr;
   op->ndots = 0;
   return WC_OK;

@@ -165,18 +165,19 @@ static size_t hbytes(const wc_operator *op) { return half_bytes(op); }
 
 static void free_batch(wc_batch *b) {
   if (!b) return;
-  for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh, b->rl, b->pl, b->ml, b->xl, b->aD}) (void)wc_dev_free(q);
-  for (void *q : b->src) (void)wc_dev_free(q);
-  for (void *q : b->one) (void)wc_dev_free(q);
-  for (void *q : b->lone) (void)wc_dev_free(q);
-  (void)wc_dev_free(b->st);
-  (void)wc_dev_free(b->partials);
+  wc_device_wait();  // the batch's work has ended
+  for (void *q : {b->xh, b->bs, b->mp, b->rs, b->rh, b->rl, b->pl, b->ml, b->xl, b->aD}) (void)wc_dev_free_idle(q);
+  for (void *q : b->src) (void)wc_dev_free_idle(q);
+  for (void *q : b->one) (void)wc_dev_free_idle(q);
+  for (void *q : b->lone) (void)wc_dev_free_idle(q);
+  (void)wc_dev_free_idle(b->st);
+  (void)wc_dev_free_idle(b->partials);
   for (int p = 0; p < 2; ++p) {
-    (void)wc_dev_free(b->ai_hi[p]);
-    (void)wc_dev_free(b->ai_lo[p]);
+    (void)wc_dev_free_idle(b->ai_hi[p]);
+    (void)wc_dev_free_idle(b->ai_lo[p]);
   }
-  (void)wc_dev_free(b->roff);
-  (void)wc_dev_free(b->err);
+  (void)wc_dev_free_idle(b->roff);
+  (void)wc_dev_free_idle(b->err);
   if (b->host) (void)hipHostFree(b->host);
   free_run(b->run);
   delete b;
@@ -280,10 +281,10 @@ static int enable_masses(wc_batch *b) {
 // reductions of both operators
 static int ensure_scratch(wc_batch *b) {
   int rc;
-  if ((rc = ensure_tmp(b->hi, 3, b->N))) return rc;
-  if ((rc = ensure_tmp(b->lo, 3, b->N))) return rc;
-  if ((rc = ensure_red(b->hi, b->N))) return rc;
-  return ensure_red(b->lo, b->N);
+  if ((rc = ensure_tmp(b->hi, 3, b->N, b->s))) return rc;
+  if ((rc = ensure_tmp(b->lo, 3, b->N, b->s))) return rc;
+  if ((rc = ensure_red(b->hi, b->N, b->s))) return rc;
+  return ensure_red(b->lo, b->N, b->s);
 }
 
 // FP64: y += x_inner / s for every slot, x_inner = 0, then r_s = b_s - Mpc y,
@@ -303,7 +304,7 @@ static int batch_reliable(wc_batch *b) {
     // Mpc y); r_s = b_s - Mpc y goes straight to FP32 with its FP64 norm.
     // Mpc^dag r_s only restarts the inner recursion, which runs in FP32: it is
     // applied by the inner operator (into ml, free between iterations)
-    if ((rc = ensure_tmp(hi, 3, N))) return rc;
+    if ((rc = ensure_tmp(hi, 3, N, s))) return rc;
     void *aD = hi->tmp[0];
     wck::fold_site_clover(hi->g, N, b->pd, clov_inv(hi, b->pd), 1.0 / hi->diag,
                           b->h16 ? b->st + 13 * N : nullptr, b->xl, b->xh, aD, hi->open_bc ? 1 : 0, s,
@@ -516,7 +517,7 @@ static int admit_impl(wc_batch *b, int slot, const void *src, const void *guess,
     if (rc == WC_OK)
       wck::site_clover<double>(hi->g, 1, pd, aD, 0.0, half(guess, pd), y1, nullptr, hi->open_bc ? 1 : 0, s);
     const hipError_t e = hipStreamSynchronize(s);
-    (void)wc_dev_free(aD);
+    (void)wc_dev_free_on(aD, s);
     if (rc) return rc;
     if (e != hipSuccess) WC_FAIL(WC_ERR_HIP, "wc_batch_admit: %s", hipGetErrorString(e));
   } else if (guess) {

@@ -325,9 +325,10 @@ struct BlockScratch {
   double2 *part = nullptr, *dmat = nullptr;  // Gram partials; (1 + kSlots) N x N matrices
   cplx *host = nullptr;                      // pinned, (1 + kSlots) N x N
   int next = 0;
+  hipStream_t s = nullptr;  // the stream the solve ran on
   ~BlockScratch() {
-    (void)wc_dev_free(part);
-    (void)wc_dev_free(dmat);
+    (void)wc_dev_free_on(part, s);
+    (void)wc_dev_free_on(dmat, s);
     if (host) (void)hipHostFree(host);
   }
 };
@@ -341,12 +342,13 @@ int block_cg_impl(wc_operator *op, int N, int pd, void *x, const void *b, double
                   long miniter, long maxiter, long *iterations, int *converged, double *history, hipStream_t s) {
   using C = typename c2<R>::t;
   int rc;
-  if ((rc = ensure_tmp(op, 10, N))) return rc;
-  if ((rc = ensure_red(op, N))) return rc;
+  if ((rc = ensure_tmp(op, 10, N, s))) return rc;
+  if ((rc = ensure_red(op, N, s))) return rc;
   const long rows = op->g.Vh * 12;
   const long n = rows * N;
   void *T = op->tmp[3], *Q = op->tmp[4], *D = op->tmp[5], *Z = op->tmp[9];
   BlockScratch bs;
+  bs.s = s;
   const int NN = N * N;
   const bool mf = block_mfma<R>(N);
   const int ntile = (N + 15) / 16;  // 16 x 16 tiles per dimension

@@ -25,6 +25,12 @@ extern thread_local std::string g_err;  // message of the last failure (wc_api.h
 // exact-size cache of freed blocks (wc_alloc.hip)
 hipError_t wc_dev_malloc_raw(void **p, size_t bytes);
 hipError_t wc_dev_free(const void *p);
+// the same for a block whose last users ran on stream s (waits for s only)
+hipError_t wc_dev_free_on(const void *p, hipStream_t s);
+// a device-wide wait that no graph capture on another thread is cut by; after
+// it, wc_dev_free_idle caches blocks without waiting again (teardown of many)
+void wc_device_wait();
+hipError_t wc_dev_free_idle(const void *p);
 // Frees by this thread between begin and end are held back; end(synced)
 // caches them, at once when synced (the stream their last users ran on has
 // been synchronized), else through wc_dev_free's wait for the device.
@@ -91,10 +97,11 @@ inline void drop_graphs(IterRun &ir) {
   ir.key.clear();
 }
 
+// (after wc_device_wait: the run's work has ended)
 inline void free_run(IterRun &ir) {
   drop_graphs(ir);
   if (ir.cap) (void)hipStreamDestroy(ir.cap);
-  (void)wc_dev_free(ir.dev);
+  (void)wc_dev_free_idle(ir.dev);
   if (ir.host) (void)hipHostFree(ir.host);
   ir = IterRun();
 }
@@ -190,11 +197,12 @@ inline double s16_delta_floor(int code) {
   return code == kS16Bf16 ? 1e-1 : code == kS16Half ? 1e-2 : 0.0;
 }
 
-// n scratch half fields of nrhs right-hand sides each
-inline int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
+// n scratch half fields of nrhs right-hand sides each; s: the stream of
+// the call (the previous fields' last users ran on it)
+inline int ensure_tmp(wc_operator *op, size_t n, int nrhs, hipStream_t s) {
   const size_t bytes = half_bytes(op) * (size_t)nrhs;
   if (bytes > op->tmp_bytes) {
-    for (void *q : op->tmp) (void)wc_dev_free(q);
+    for (void *q : op->tmp) (void)wc_dev_free_on(q, s);
     op->tmp.clear();
     op->tmp_bytes = bytes;
   }
@@ -206,25 +214,26 @@ inline int ensure_tmp(wc_operator *op, size_t n, int nrhs) {
   return WC_OK;
 }
 
-// reduction partials and CG scalars for nrhs right-hand sides
-inline int ensure_red(wc_operator *op, int nrhs) {
+// reduction partials and CG scalars for nrhs right-hand sides (s as for
+// ensure_tmp)
+inline int ensure_red(wc_operator *op, int nrhs, hipStream_t s) {
   const size_t np = (size_t)std::max(wck::kRedBlocks, wck::hop_blocks(op->g)) * nrhs;
   if (np > op->npartials) {
-    (void)wc_dev_free(op->partials);
+    (void)wc_dev_free_on(op->partials, s);
     op->partials = nullptr;
     WC_HIP(wc_dev_malloc((void **)&op->partials, sizeof(double) * np));
     op->npartials = np;
   }
   const size_t nd = (size_t)op->g.Vh * nrhs;
   if (nd > op->ndots) {
-    (void)wc_dev_free(op->dots);
+    (void)wc_dev_free_on(op->dots, s);
     op->dots = nullptr;
     WC_HIP(wc_dev_malloc((void **)&op->dots, sizeof(double) * nd));
     op->ndots = nd;
   }
   const size_t ns = (size_t)16 * nrhs;
   if (ns > op->nst) {
-    (void)wc_dev_free(op->st);
+    (void)wc_dev_free_on(op->st, s);
     if (op->host) (void)hipHostFree(op->host);
     op->st = nullptr;
     op->host = nullptr;
@@ -237,17 +246,18 @@ inline int ensure_red(wc_operator *op, int nrhs) {
 
 inline void free_op(wc_operator *op) {
   if (!op) return;
+  wc_device_wait();  // the operator's work has ended
   for (int p = 0; p < 2; ++p) {
-    (void)wc_dev_free(op->W[p]);
-    (void)wc_dev_free(op->A[p]);
-    (void)wc_dev_free(op->Ai[p]);
+    (void)wc_dev_free_idle(op->W[p]);
+    (void)wc_dev_free_idle(op->A[p]);
+    (void)wc_dev_free_idle(op->Ai[p]);
   }
-  (void)wc_dev_free(op->Unat);
-  (void)wc_dev_free(op->err);
-  for (void *q : op->tmp) (void)wc_dev_free(q);
-  (void)wc_dev_free(op->partials);
-  (void)wc_dev_free(op->dots);
-  (void)wc_dev_free(op->st);
+  (void)wc_dev_free_idle(op->Unat);
+  (void)wc_dev_free_idle(op->err);
+  for (void *q : op->tmp) (void)wc_dev_free_idle(q);
+  (void)wc_dev_free_idle(op->partials);
+  (void)wc_dev_free_idle(op->dots);
+  (void)wc_dev_free_idle(op->st);
   if (op->host) (void)hipHostFree(op->host);
   free_run(op->run);
   delete op;
@@ -480,9 +490,9 @@ inline int mpc_run(wc_operator *op, int nrhs, int pd, int which, void *out, cons
 template <typename R>
 inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, const void *in, const void *dotv,
                     hipStream_t s, const void *tD_in = nullptr, double *partials = nullptr, int half = 0) {
-  int rc = ensure_tmp(op, 3, nrhs);
+  int rc = ensure_tmp(op, 3, nrhs, s);
   if (rc) return rc;
-  if ((rc = ensure_red(op, nrhs))) return rc;
+  if ((rc = ensure_red(op, nrhs, s))) return rc;
   // the operator's partials exist only from here on
   if (!partials) partials = op->partials;
   return mpc_run<R>(op, nrhs, pd, which, out, in, dotv, s, op->tmp[0], op->tmp[1], op->tmp[2], tD_in, op->dots,
@@ -492,7 +502,7 @@ inline int mpc_impl(wc_operator *op, int nrhs, int pd, int which, void *out, con
 // per-RHS |x|^2 into dev_out[0..nrhs)
 template <typename R>
 inline int norm2_rhs(wc_operator *op, int nrhs, const void *x, double *dev_out, hipStream_t s) {
-  int rc = ensure_red(op, nrhs);
+  int rc = ensure_red(op, nrhs, s);
   if (rc) return rc;
   const long n = op->g.Vh * 12 * nrhs;
   wck::dot_partials_rhs<R>(n, nrhs, x, x, op->partials, nullptr, s);
@@ -649,7 +659,7 @@ constexpr int kFirstRun = 4;
 template <typename R, typename F>
 inline int schur_solve(wc_operator *op, int nrhs, int pd, void *dst, const void *src, hipStream_t s, F &&inner) {
   const int pc = 1 - pd;
-  int rc = ensure_tmp(op, 9, nrhs);
+  int rc = ensure_tmp(op, 9, nrhs, s);
   if (rc) return rc;
   const size_t hb = half_bytes(op) * (size_t)nrhs;
   auto half = [&](const void *f, int p) { return (void *)((char *)f + (size_t)p * hb); };
