@@ -2,7 +2,8 @@
 (wc_release_cached_memory): an operator released and created again on the
 same lattice takes its blocks back from the cache, allocating no new device
 memory, and computes the same bits; flushing the cache returns the blocks
-to the runtime (the device's free memory grows by what was reported)."""
+to the runtime (the device's free memory grows by what was reported).
+Frees on one thread do not cut graph captures of CG runs on another."""
 import ctypes
 
 import numpy as np
@@ -84,3 +85,73 @@ def test_parked_operators_are_destroyed_by_release_pool():
     w, out = meooe(U, src)
     assert torch.equal(out, ref)
     w.release()
+
+
+def test_frees_on_one_thread_do_not_cut_captures_on_another():
+    """one thread solves in fresh batches whose iteration runs are captured as
+    graphs (every batch captures anew) while another creates and destroys
+    operators (device-wide waits and frees): every solve converges to the
+    same bits as the first"""
+    import threading
+
+    from gpt_amd import serve
+    from oracle import wilson_clover as W
+
+    rng = R.random("alloc_capture")
+    U = G.random_links(R.grid(FD), rng, scale=0.5)
+    srcs = [rng.vspincolor(R.grid(FD)) for _ in range(2)]
+    errors, sols = [], []
+    stop = threading.Event()
+
+    def solver():
+        try:
+            w = g.qcd.fermion.wilson_clover(U, P)
+            w.set_iteration_chunk(8)
+            inner = w.converted(g.single)
+            for _ in range(4):
+                b = serve.batch(w, inner, 4)
+                for a in srcs:
+                    x = g.lattice(w.F_grid)
+                    x.from_numpy(a)
+                    b.admit(x, 1e-10, 10000)
+                done = {}
+                for _ in range(2000):
+                    for sl in b.step(8):
+                        x = g.lattice(w.F_grid)
+                        it, tres, conv = b.extract(sl, x)
+                        assert conv and tres <= 1e-10
+                        done[sl] = x.to_numpy()
+                    if len(done) == 2:
+                        break
+                b.close()
+                sols.append([done[k] for k in sorted(done)])
+            w.release()
+            inner.release()
+        except Exception as e:  # noqa: BLE001 -- reported below
+            errors.append(repr(e))
+        finally:
+            stop.set()
+
+    def churn():
+        try:
+            while not stop.is_set():
+                w = g.qcd.fermion.wilson_clover(U, dict(P, mass=0.2))
+                w.release()
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    t1, t2 = threading.Thread(target=solver), threading.Thread(target=churn)
+    t1.start()
+    t2.start()
+    t1.join(timeout=120)
+    stop.set()
+    t2.join(timeout=60)
+    assert not t1.is_alive() and not t2.is_alive()
+    assert errors == []
+    assert len(sols) == 4
+    ref = W.wilson_clover(U, **P)
+    for k, a in enumerate(srcs):
+        x = sols[0][k]
+        assert np.linalg.norm(a - ref.M(x)) / np.linalg.norm(a) <= 1.05e-10
+        for s in sols[1:]:
+            assert np.array_equal(s[k], x)
