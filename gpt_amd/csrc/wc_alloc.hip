@@ -234,6 +234,11 @@ hipError_t wc_dev_free_on(const void *cp, hipStream_t s) {
   }
   if (!ours || t_defer > 0) return wc_dev_free(p);
   if (t_capturing > 0) return hipErrorInvalidValue;
+  // s itself being captured by the caller: a wait would cut that capture, so
+  // the block is cached at once (callers that capture use the allocation-free
+  // entry point, which never gets here)
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) return release_block(p);
   // its last users ran on s: a stream-local wait, which no capture on
   // another stream minds
   const hipError_t e = hipStreamSynchronize(s);
