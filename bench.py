@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--serve", type=int, default=12,
                     help="serving stream: requests of the capacity burst (0: skip serving)")
     ap.add_argument("--serve-poisson", type=int, default=200, help="serving stream: Poisson-arrival requests")
+    ap.add_argument("--multi-mass", default="all",
+                    help="multi-mass batch cases after the serving stream: all, none, or one of "
+                         "one, other, three, twelve (that case alone, without the warm-up; also with --serve 0)")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson arrival rate as a fraction of the measured capacity")
     ap.add_argument("--serve-streams", type=int, default=4, help="HIP streams (worker threads) of the server")
@@ -505,7 +508,10 @@ def main():
     if args.serve > 0:
         serve_res = serve_measure(torch, g, dev, args.serve, npoisson=args.serve_poisson, load=args.serve_load,
                                   streams=args.serve_streams, rate=args.serve_rate, min_slots=args.serve_min_slots)
-        serve_res["multi_mass"] = multi_mass_measure(torch, g, dev)
+    if args.multi_mass != "none" and (args.serve > 0 or args.multi_mass != "all"):
+        mm = multi_mass_measure(torch, g, dev, only=None if args.multi_mass == "all" else args.multi_mass)
+        serve_res = serve_res if serve_res is not None else {}
+        serve_res["multi_mass"] = mm
 
     cg_res = None
     if not args.no_cg:
@@ -1016,7 +1022,7 @@ def serve_measure(torch, g, dev, nreq, seed=5001, nslots=12, load=0.8, tol=1e-10
     return out
 
 
-def multi_mass_measure(torch, g, dev, fd=(24, 24, 24, 48), nslots=12, tol=1e-10):
+def multi_mass_measure(torch, g, dev, fd=(24, 24, 24, 48), nslots=12, tol=1e-10, only=None):
     """Cost of several masses in one serving batch: the same 12 Gaussian
     sources in a 12-slot batch (FP64 outer, 12-real FP32 inner operator at
     mass 0.1, csw 1) with one mass (the operators'), one other mass, three
@@ -1024,7 +1030,8 @@ def multi_mass_measure(torch, g, dev, fd=(24, 24, 24, 48), nslots=12, tol=1e-10)
     admission until all 12 solves are extracted (tolerance tol on the true
     residual).  Slots of one mass share a clover-inverse set, and a batch
     whose occupied slots all use one set stages it per site, so the cost
-    grows with the number of distinct masses, not with the slots."""
+    grows with the number of distinct masses, not with the slots.  only: one
+    case (one, other, three, twelve) without the warm-up, for profiling."""
     from gpt_amd import serve
 
     fd = list(fd)
@@ -1044,6 +1051,11 @@ def multi_mass_measure(torch, g, dev, fd=(24, 24, 24, 48), nslots=12, tol=1e-10)
     cases = [("warm-up", [None] * nslots), ("one mass", [None] * nslots), ("one other mass", [0.08] * nslots),
              ("three masses", [0.08] * third + [None] * third + [0.12] * (nslots - 2 * third)),
              ("twelve masses", [0.1 + 0.005 * (j - 6) if j != 6 else None for j in range(nslots)])]
+    if only is not None:
+        names = {"one": "one mass", "other": "one other mass", "three": "three masses", "twelve": "twelve masses"}
+        if only not in names:
+            raise SystemExit(f"--multi-mass {only}: not one of all, none, {', '.join(names)}")
+        cases = [c for c in cases if c[0] == names[only]]
     out = {"lattice": "x".join(map(str, fd)), "slots": nslots, "tol": tol}
     dst = g.lattice(op.F_grid)
     for label, masses in cases:
