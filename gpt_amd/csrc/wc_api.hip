@@ -775,10 +775,12 @@ static std::string tune_tag(const wc_operator *op, int nrhs) {
   return t;
 }
 
-static bool load_tuned(const wc_operator *op, int nrhs, wck::HopShape &sh) {
+// a tuned shape (and, for the 16-bit picks, the form: "form" 0 vector, 1
+// matrix cores, -1 not decided) from the file of tag; false when there is
+// none or it does not fit the operator
+static bool load_tag(const wc_operator *op, int nrhs, const std::string &tag, wck::HopShape &sh, int *form) {
   const std::string dir = tune_cache_dir();
   if (dir.empty()) return false;
-  const std::string tag = tune_tag(op, nrhs);
   FILE *f = fopen((dir + "/" + tag + ".json").c_str(), "rt");
   if (!f) return false;
   char buf[1024];
@@ -792,25 +794,45 @@ static bool load_tuned(const wc_operator *op, int nrhs, wck::HopShape &sh) {
   wck::HopShape x;
   if (sscanf(buf + at, "\"params\": [%d, %d, %d, %d, %d]", &x.rb, &x.tpb, &x.occ, &x.seg, &x.ls) != 5) return false;
   if (!shape_valid(op, x) || x.rb > nrhs || !stage_fits(op, nrhs, x)) return false;
+  if (form) {
+    const size_t fa = txt.find("\"form\": ");
+    if (fa == std::string::npos || sscanf(buf + fa, "\"form\": %d", form) != 1 || *form < -1 || *form > 1)
+      return false;
+  }
   sh = x;
   return true;
 }
 
-static void save_tuned(const wc_operator *op, int nrhs, const wck::HopShape &sh, float ms) {
+static bool load_tuned(const wc_operator *op, int nrhs, wck::HopShape &sh) {
+  return load_tag(op, nrhs, tune_tag(op, nrhs), sh, nullptr);
+}
+
+static void save_tag(const std::string &tag, const wck::HopShape &sh, float ms, const int *form) {
   const std::string dir = tune_cache_dir();
   if (dir.empty()) return;
   mkdir(dir.c_str(), 0755);  // may exist already
-  const std::string tag = tune_tag(op, nrhs);
   const std::string fn = dir + "/" + tag + ".json";
   char tmp_name[64];
   snprintf(tmp_name, sizeof tmp_name, ".tmp%ld", (long)getpid());
   const std::string tmp = fn + tmp_name;
   FILE *f = fopen(tmp.c_str(), "wt");
   if (!f) return;  // an unwritable directory only costs the next process a re-tune
-  fprintf(f, "{\"tag\": \"%s\", \"params\": [%d, %d, %d, %d, %d], \"ms\": %.6g}\n", tag.c_str(), sh.rb, sh.tpb,
-          sh.occ, sh.seg, sh.ls, (double)ms);
+  fprintf(f, "{\"tag\": \"%s\", \"params\": [%d, %d, %d, %d, %d], ", tag.c_str(), sh.rb, sh.tpb, sh.occ, sh.seg,
+          sh.ls);
+  if (form) fprintf(f, "\"form\": %d, ", *form);
+  fprintf(f, "\"ms\": %.6g}\n", (double)ms);
   const bool ok = fclose(f) == 0;
   if (!ok || rename(tmp.c_str(), fn.c_str()) != 0) remove(tmp.c_str());
+}
+
+static void save_tuned(const wc_operator *op, int nrhs, const wck::HopShape &sh, float ms) {
+  save_tag(tune_tag(op, nrhs), sh, ms, nullptr);
+}
+
+// the file of the 16-bit picks: the FP32 operator's tag, and whether the
+// matrix-core form was a candidate
+static std::string s16_tag(const wc_operator *op, int nrhs, bool try_mma) {
+  return tune_tag(op, nrhs) + (try_mma ? "_s16_mma" : "_s16");
 }
 
 static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *ms_out) {
@@ -845,12 +867,26 @@ static int tune_hop(wc_operator *op, int nrhs, int reps, hipStream_t s, float *m
 // solve's tolerance.  The input is a pseudo-random FP16 pattern (all-zero
 // input runs faster, and not by the same amount for every candidate); the
 // candidates are timed in two alternating rounds, the better round of each
-// counting.  Scratch: tmp[0..1].
+// counting.  The picks persist like the tuner's shapes (one file per
+// lattice, link format, class and whether the matrix-core form was a
+// candidate).  Scratch: tmp[0..1].
 static int pick_s16(wc_operator *op, int N, hipStream_t s) {
   if (op->prec != WC_SINGLE) return WC_OK;
   {
     std::lock_guard<std::mutex> lk(g_shape_mu);
     if (g_shape16_cache.count(shape_key(op, N))) return WC_OK;
+  }
+  const bool try_mma = op->mma == 2 && N >= 4 && N % 4 == 0;
+  {
+    // an earlier process's picks
+    wck::HopShape sh;
+    int form = -1;
+    if (load_tag(op, N, s16_tag(op, N, try_mma), sh, &form)) {
+      std::lock_guard<std::mutex> lk(g_shape_mu);
+      g_shape16_cache[shape_key(op, N)] = sh;
+      if (try_mma && form >= 0) g_mma_cache[shape_key(op, N)] = form;
+      return WC_OK;
+    }
   }
   int rc = ensure_tmp(op, 2, N, s);
   if (rc) return rc;
@@ -881,7 +917,6 @@ static int pick_s16(wc_operator *op, int N, hipStream_t s) {
         sh.ls = N > rb ? 1 : 0;
         add(sh, 0);
       }
-  const bool try_mma = op->mma == 2 && N >= 4 && N % 4 == 0;
   if (try_mma) add(base, 1);
   hipEvent_t e0, e1;
   WC_HIP(hipEventCreate(&e0));
@@ -919,9 +954,14 @@ static int pick_s16(wc_operator *op, int N, hipStream_t s) {
     if (cand[i].second == 0 && (bv < 0 || best[i] < best[bv])) bv = (int)i;
     if (bm < 0 || best[i] < best[bm]) bm = (int)i;
   }
-  std::lock_guard<std::mutex> lk(g_shape_mu);
-  g_shape16_cache[shape_key(op, N)] = bv >= 0 ? cand[bv].first : base;
-  if (try_mma && bm >= 0) g_mma_cache[shape_key(op, N)] = cand[bm].second;
+  const wck::HopShape pick = bv >= 0 ? cand[bv].first : base;
+  const int form = try_mma && bm >= 0 ? cand[bm].second : -1;
+  {
+    std::lock_guard<std::mutex> lk(g_shape_mu);
+    g_shape16_cache[shape_key(op, N)] = pick;
+    if (form >= 0) g_mma_cache[shape_key(op, N)] = form;
+  }
+  save_tag(s16_tag(op, N, try_mma), pick, bv >= 0 ? best[bv] / 3 : 0.0f, &form);
   return WC_OK;
 }
 

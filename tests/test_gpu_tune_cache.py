@@ -7,6 +7,7 @@ a lattice where launch shapes matter (>= 2^17 site-RHS pairs) tunes once and
 writes its file; a file already there for the operator's key is used instead
 of tuning (a shape planted in it is the one the operator runs with); an
 invalid or foreign file is ignored; an empty WC_TUNE_CACHE writes nothing.
+The 16-bit hops' shape and form picks persist the same way.
 Results never depend on the shape, so none of this changes a solution.
 """
 import json
@@ -58,7 +59,7 @@ def test_tuned_shape_written_then_reused(links, tmp_path, monkeypatch):
     monkeypatch.setenv("WC_TUNE_CACHE", str(tmp_path))
     monkeypatch.delenv("WC_AUTOTUNE", raising=False)
     w8 = short_solve(links, 8)  # 16^4 / 2 sites x 8 right-hand sides = 2^18 pairs: tuned
-    files = sorted(os.listdir(tmp_path))
+    files = sorted(f for f in os.listdir(tmp_path) if "_s16" not in f)  # the FP32 hop's file
     assert len(files) == 1 and files[0].endswith("_rhs8.json"), files
     rec = json.load(open(tmp_path / files[0]))
     assert rec["tag"] == files[0][:-5]
@@ -95,6 +96,46 @@ def test_foreign_or_invalid_files_are_ignored(links, tmp_path, monkeypatch):
     w8c = short_solve(links, 8, compress=True)
     rec = json.load(open(tmp_path / (tag8c + ".json")))
     assert rec["tag"] == tag8c and tuple(rec["params"]) == w8c.hop_shape(8) and rec["params"][1] != 9
+
+
+def mixed16_solve(U, fd, n):
+    """a 16-bit mixed solve of n right-hand sides (picks the 16-bit hop shape
+    and form on first use); returns its inner operator"""
+    inv, pc = g.algorithms.inverter, g.qcd.fermion.preconditioner
+    w = g.qcd.fermion.wilson_clover(U, P, n_rhs=n)
+    rng = np.random.default_rng(8200 + n)
+    shape = tuple(reversed(fd)) + (4, 3)
+    fs = []
+    for _ in range(n):
+        x = g.lattice(w.F_grid.single_rhs())
+        x.from_numpy((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0))
+        fs.append(x)
+    m = inv.cg_mixed(eps=1e-6, maxiter=2000, inner_storage="half")
+    g(w.propagator(inv.preconditioned(pc.eo2_ne(), m)) * pack(fs))
+    torch.cuda.synchronize()
+    return m.inner_operator(w)
+
+
+def test_16bit_picks_written_then_reused(tmp_path, monkeypatch):
+    """the 16-bit hops' shape and form picks persist too: a solve writes them
+    (with the form, 0 vector or 1 matrix cores), and a planted file for
+    another class is read back instead of timed (a lattice no other test of
+    this process picks for, as picks are also kept process-wide)"""
+    fd = [16, 16, 16, 8]
+    U = G.smooth_links(fd, 0.3, 8002)
+    monkeypatch.setenv("WC_TUNE_CACHE", str(tmp_path))
+    monkeypatch.delenv("WC_AUTOTUNE", raising=False)
+    mixed16_solve(U, fd, 8)  # 16^3 x 8 / 2 sites x 8 right-hand sides = 2^17 pairs: picked
+    names = [f for f in os.listdir(tmp_path) if f.endswith("_rhs8_s16_mma.json")]
+    assert len(names) == 1, os.listdir(tmp_path)
+    rec = json.load(open(tmp_path / names[0]))
+    assert rec["tag"] == names[0][:-5] and rec["form"] in (0, 1) and len(rec["params"]) == 5
+    tag16 = rec["tag"].replace("_rhs8", "_rhs16")
+    with open(tmp_path / (tag16 + ".json"), "w") as f:
+        json.dump({"tag": tag16, "params": rec["params"], "form": 1, "ms": 1.0}, f)
+    inner = mixed16_solve(U, fd, 16)
+    assert inner.hop_mma(16)  # the planted form, read back
+    assert json.load(open(tmp_path / (tag16 + ".json")))["form"] == 1  # not re-timed, not rewritten
 
 
 def test_empty_setting_writes_nothing(links, tmp_path, monkeypatch):
