@@ -212,6 +212,13 @@ void wc_device_wait() {
   (void)hipDeviceSynchronize();
 }
 
+hipError_t wc_host_free(void *p) {
+  if (!p) return hipSuccess;
+  if (t_capturing > 0) return hipErrorInvalidValue;  // would cut this thread's capture
+  std::unique_lock<std::shared_mutex> q(g_capmu);
+  return hipHostFree(p);
+}
+
 hipError_t wc_dev_free_idle(const void *cp) {
   void *p = const_cast<void *>(cp);
   if (!p) return hipSuccess;

@@ -178,7 +178,7 @@ static void free_batch(wc_batch *b) {
   }
   (void)wc_dev_free_idle(b->roff);
   (void)wc_dev_free_idle(b->err);
-  if (b->host) (void)hipHostFree(b->host);
+  if (b->host) (void)wc_host_free(b->host);
   free_run(b->run);
   delete b;
 }

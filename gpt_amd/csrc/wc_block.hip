@@ -329,7 +329,7 @@ struct BlockScratch {
   ~BlockScratch() {
     (void)wc_dev_free_on(part, s);
     (void)wc_dev_free_on(dmat, s);
-    if (host) (void)hipHostFree(host);
+    if (host) (void)wc_host_free(host);
   }
 };
 

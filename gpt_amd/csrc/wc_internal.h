@@ -31,6 +31,8 @@ hipError_t wc_dev_free_on(const void *p, hipStream_t s);
 // it, wc_dev_free_idle caches blocks without waiting again (teardown of many)
 void wc_device_wait();
 hipError_t wc_dev_free_idle(const void *p);
+// hipHostFree, which may wait for the device, at a moment no capture runs
+hipError_t wc_host_free(void *p);
 // Frees by this thread between begin and end are held back; end(synced)
 // caches them, at once when synced (the stream their last users ran on has
 // been synchronized), else through wc_dev_free's wait for the device.
@@ -102,7 +104,7 @@ inline void free_run(IterRun &ir) {
   drop_graphs(ir);
   if (ir.cap) (void)hipStreamDestroy(ir.cap);
   (void)wc_dev_free_idle(ir.dev);
-  if (ir.host) (void)hipHostFree(ir.host);
+  if (ir.host) (void)wc_host_free(ir.host);
   ir = IterRun();
 }
 
@@ -231,10 +233,13 @@ inline int ensure_red(wc_operator *op, int nrhs, hipStream_t s) {
     WC_HIP(wc_dev_malloc((void **)&op->dots, sizeof(double) * nd));
     op->ndots = nd;
   }
-  const size_t ns = (size_t)16 * nrhs;
+  // the scalars (16 per right-hand side, 8 KB at most) are sized for the
+  // largest count at once: they never grow, so no pinned block is freed
+  // while other threads may capture
+  const size_t ns = (size_t)16 * wck::kMaxRhs;
   if (ns > op->nst) {
     (void)wc_dev_free_on(op->st, s);
-    if (op->host) (void)hipHostFree(op->host);
+    if (op->host) (void)wc_host_free(op->host);
     op->st = nullptr;
     op->host = nullptr;
     WC_HIP(wc_dev_malloc((void **)&op->st, sizeof(double) * ns));
@@ -258,7 +263,7 @@ inline void free_op(wc_operator *op) {
   (void)wc_dev_free_idle(op->partials);
   (void)wc_dev_free_idle(op->dots);
   (void)wc_dev_free_idle(op->st);
-  if (op->host) (void)hipHostFree(op->host);
+  if (op->host) (void)wc_host_free(op->host);
   free_run(op->run);
   delete op;
 }
